@@ -1,0 +1,418 @@
+/*
+ * backend.cpp — hip_backend_* C ABI (cvp9_hip/hip_backend.h).
+ *
+ * Mirrors the Vulkan backend's contract (src/backend/vulkan/vulkan_backend.c:
+ * ring :1037-1062, :1391-1426; get :1428-1455; DMA-BUF :1457-1486; export
+ * buffers :1488-1544) on one HIP stream:
+ *   decode: validate + stage (pinned) -> H2D -> stage kernels -> device
+ *           reference-pool update -> pack to the slot's VRAM I420 buffer ->
+ *           D2H into the slot's pinned output -> event
+ *   get:    poll / wait the oldest slot's event, hand out pinned pointers.
+ */
+#include <hip/hip_runtime.h>
+#include <hsa/hsa_ext_amd.h>
+#include <sys/mman.h>
+#include <unistd.h>
+#include <string.h>
+#include <stdlib.h>
+
+#include <new>
+
+#include "cvp9_hip/hip_backend.h"
+#include "cvp9_hip/testing.h"
+#include "engine.hpp"
+
+using namespace cvp9;
+
+namespace {
+
+constexpr uint32_t kSpinLimit = 1u << 22;   /* ~0.3 s of s_sleep per wait */
+
+struct Slot {
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    int64_t pts = 0;
+    uint32_t w = 0, h = 0;
+    uint8_t *stage_host = nullptr;  size_t stage_host_cap = 0;
+    uint8_t *stage_dev = nullptr;   size_t stage_dev_cap = 0;
+    uint8_t *out_host = nullptr;    size_t out_host_cap = 0;
+    uint8_t *out_dev = nullptr;     size_t out_dev_cap = 0;   /* packed I420, VRAM */
+    int out_fd = -1;
+    uint32_t *err_host = nullptr;
+};
+
+struct Backend {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    StreamState *st = nullptr;
+    Slot slots[CVP9_HIP_INFLIGHT];
+    uint32_t head = 0, count = 0;
+    uint8_t *pending_target = nullptr;   /* device address of the armed NV12 target */
+    uint64_t pending_target_size = 0;
+    uint32_t debug = 0;
+    uint32_t intra_wgs = 512;
+    bool dmabuf = false;
+};
+
+struct ExportPriv {
+    uint8_t *dev;
+    size_t size;
+    void *map;
+};
+
+inline size_t i420_bytes(uint32_t w, uint32_t h)
+{
+    return (size_t)w * h + 2 * (size_t)((w + 1) / 2) * ((h + 1) / 2);
+}
+
+/* DMA-BUF fd of a VRAM allocation, or -1 */
+int export_fd(void *ptr, size_t size)
+{
+    int fd = -1;
+    uint64_t off = 0;
+    if (hsa_amd_portable_export_dmabuf(ptr, size, &fd, &off) != HSA_STATUS_SUCCESS) return -1;
+    if (off != 0) {
+        hsa_amd_portable_close_dmabuf(fd);
+        return -1;
+    }
+    return fd;
+}
+
+hipError_t grow_pinned(uint8_t **p, size_t *cap, size_t need)
+{
+    if (*cap >= need) return hipSuccess;
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t c = need + need / 4;
+    hipError_t e = hipHostMalloc((void **)p, c, hipHostMallocDefault);
+    if (e == hipSuccess) *cap = c;
+    return e;
+}
+
+hipError_t grow_device(uint8_t **p, size_t *cap, size_t need)
+{
+    if (*cap >= need) return hipSuccess;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t c = need + need / 4;
+    hipError_t e = hipMalloc((void **)p, c);
+    if (e == hipSuccess) *cap = c;
+    return e;
+}
+
+cvp9_err_t to_err(hipError_t e)
+{
+    if (e == hipSuccess) return CVP9_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CVP9_ERR_NOMEM;
+    return CVP9_ERR_GPU;
+}
+
+void free_slot(Slot &s)
+{
+    if (s.done) hipEventDestroy(s.done);
+    if (s.stage_host) hipHostFree(s.stage_host);
+    if (s.stage_dev) hipFree(s.stage_dev);
+    if (s.out_host) hipHostFree(s.out_host);
+    if (s.out_fd >= 0) hsa_amd_portable_close_dmabuf(s.out_fd);
+    if (s.out_dev) hipFree(s.out_dev);
+    if (s.err_host) hipHostFree(s.err_host);
+    s = Slot();
+}
+
+/* oldest in-flight frame, released from the ring (vulkan_backend.c:1391-1426) */
+cvp9_err_t acquire_oldest(Backend *b, int wait, Slot **out)
+{
+    if (!b || b->count == 0) return CVP9_ERR_UNSUPPORTED;
+    Slot &s = b->slots[b->head];
+    if (!wait && b->count < CVP9_HIP_INFLIGHT) {
+        hipError_t q = hipEventQuery(s.done);
+        if (q == hipErrorNotReady) return CVP9_ERR_AGAIN;
+        if (q != hipSuccess) return CVP9_ERR_GPU;
+    } else {
+        if (hipEventSynchronize(s.done) != hipSuccess) return CVP9_ERR_GPU;
+    }
+    s.pending = false;
+    b->head = (b->head + 1) % CVP9_HIP_INFLIGHT;
+    b->count--;
+    *out = &s;
+    return CVP9_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+cvp9_err_t hip_backend_init(void **ctx)
+{
+    if (!ctx) return CVP9_ERR_INVALID_DATA;
+    *ctx = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CVP9_ERR_NO_BACKEND;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CVP9_ERR_NO_BACKEND;
+    if (init_device(dev) != hipSuccess) return CVP9_ERR_NO_BACKEND;
+    Backend *b = new (std::nothrow) Backend();
+    if (!b) return CVP9_ERR_NOMEM;
+    b->device = dev;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+        b->intra_wgs = 2u * (uint32_t)prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return CVP9_ERR_GPU;
+    }
+    b->st = new (std::nothrow) StreamState(dev);
+    if (!b->st) {
+        hip_backend_destroy(b);
+        return CVP9_ERR_NOMEM;
+    }
+    for (Slot &s : b->slots) {
+        if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void **)&s.err_host, 64, hipHostMallocDefault) != hipSuccess) {
+            hip_backend_destroy(b);
+            return CVP9_ERR_GPU;
+        }
+        *s.err_host = 0;
+    }
+    /* DMA-BUF export works on VRAM allocations when the driver supports it */
+    {
+        void *probe = nullptr;
+        if (hipMalloc(&probe, 4096) == hipSuccess) {
+            int fd = export_fd(probe, 4096);
+            if (fd >= 0) {
+                b->dmabuf = true;
+                hsa_amd_portable_close_dmabuf(fd);
+            }
+            hipFree(probe);
+        }
+    }
+    *ctx = b;
+    return CVP9_OK;
+}
+
+void hip_backend_destroy(void *ctx)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (Slot &s : b->slots) free_slot(s);
+    delete b->st;
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b || !pf) return CVP9_ERR_INVALID_DATA;
+    if (b->count == CVP9_HIP_INFLIGHT) return CVP9_ERR_AGAIN;
+    PrepLayout L;
+    cvp9_err_t err = plan_measure(pf, L);
+    if (err != CVP9_OK) return err;
+    /* an armed NV12 target must hold the whole frame */
+    if (b->pending_target && b->pending_target_size < i420_bytes(L.W, L.H)) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(b->device);
+
+    Slot &s = b->slots[(b->head + b->count) % CVP9_HIP_INFLIGHT];
+    const size_t batch_off = L.bytes;
+    const size_t need = batch_off + BatchBuilder::bytes_for(1);
+    hipError_t e;
+    if ((e = grow_pinned(&s.stage_host, &s.stage_host_cap, need)) != hipSuccess) return to_err(e);
+    if ((e = grow_device(&s.stage_dev, &s.stage_dev_cap, need)) != hipSuccess) return to_err(e);
+    const size_t ob = i420_bytes(L.W, L.H);
+    if ((e = grow_pinned(&s.out_host, &s.out_host_cap, ob)) != hipSuccess) return to_err(e);
+    if (s.out_dev_cap < ob) {
+        if (s.out_fd >= 0) hsa_amd_portable_close_dmabuf(s.out_fd);
+        s.out_fd = -1;
+        if ((e = grow_device(&s.out_dev, &s.out_dev_cap, ob)) != hipSuccess) return to_err(e);
+        if (b->dmabuf) s.out_fd = export_fd(s.out_dev, s.out_dev_cap);
+    }
+
+    plan_write(pf, L, s.stage_host);
+    if ((e = b->st->ensure_scratch(L, b->stream)) != hipSuccess) return to_err(e);
+    FrameBuf *out = nullptr;
+    if ((e = b->st->acquire(L.W, L.H, &out)) != hipSuccess) return to_err(e);
+    DJob job = b->st->make_job(L, s.stage_dev, out, b->pending_target);
+    if (b->debug & CVP9_HIP_DEBUG_NO_DEBLOCK) job.lf_thr = 0;
+    BatchBuilder bb;
+    bb.build(&job, 1, s.stage_host + batch_off, s.stage_dev + batch_off, b->intra_wgs, kSpinLimit);
+
+    if ((e = hipMemcpyAsync(s.stage_dev, s.stage_host, batch_off + bb.used(), hipMemcpyHostToDevice,
+                            b->stream)) != hipSuccess)
+        return to_err(e);
+    if ((e = BatchBuilder::launch(bb.batch(), bb.counts(), b->stream)) != hipSuccess) return to_err(e);
+    b->st->update_refs(L.key != 0, out);
+
+    /* packed I420 into the slot's VRAM buffer, then to the pinned output */
+    const DFrame f = out->dframe();
+    const uint32_t cw = (L.W + 1) / 2, ch = (L.H + 1) / 2;
+    uint8_t *dy = s.out_dev, *du = dy + (size_t)L.W * L.H, *dv = du + (size_t)cw * ch;
+    if ((e = hipMemcpy2DAsync(dy, L.W, f.y, f.sy, L.W, L.H, hipMemcpyDeviceToDevice, b->stream)) ||
+        (e = hipMemcpy2DAsync(du, cw, f.u, f.suv, cw, ch, hipMemcpyDeviceToDevice, b->stream)) ||
+        (e = hipMemcpy2DAsync(dv, cw, f.v, f.suv, cw, ch, hipMemcpyDeviceToDevice, b->stream)) ||
+        (e = hipMemcpyAsync(s.out_host, s.out_dev, ob, hipMemcpyDeviceToHost, b->stream)) ||
+        (e = hipMemcpyAsync(s.err_host, b->st->err_word(), 4, hipMemcpyDeviceToHost, b->stream)) ||
+        (e = hipEventRecord(s.done, b->stream)))
+        return to_err(e);
+
+    s.pending = true;
+    s.pts = pts;
+    s.w = L.W;
+    s.h = L.H;
+    b->count++;
+    b->pending_target = nullptr; /* one-shot (vulkan_backend.c:1059-1062) */
+    b->pending_target_size = 0;
+    return CVP9_OK;
+}
+
+cvp9_err_t hip_get_frame(void *ctx, cvp9_frame_info_t *info, int wait)
+{
+    Backend *b = (Backend *)ctx;
+    if (!info) return CVP9_ERR_INVALID_DATA;
+    if (b) hipSetDevice(b->device);
+    Slot *s = nullptr;
+    cvp9_err_t err = acquire_oldest(b, wait, &s);
+    if (err != CVP9_OK) return err;
+    if (*s->err_host) return CVP9_ERR_GPU;
+    const uint32_t w = s->w, h = s->h, cw = (w + 1) / 2, ch = (h + 1) / 2;
+    info->width = w;
+    info->height = h;
+    info->stride_y = w;
+    info->stride_uv = cw;
+    info->plane_y = s->out_host;
+    info->plane_u = s->out_host + (size_t)w * h;
+    info->plane_v = info->plane_u + (size_t)cw * ch;
+    info->pts = s->pts;
+    return CVP9_OK;
+}
+
+cvp9_err_t hip_get_frame_dmabuf(void *ctx, cvp9_dmabuf_frame_t *out)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b || !b->dmabuf) return CVP9_ERR_UNSUPPORTED;
+    if (!out) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(b->device);
+    Slot *s = nullptr;
+    cvp9_err_t err = acquire_oldest(b, 1, &s);
+    if (err != CVP9_OK) return err;
+    if (*s->err_host) return CVP9_ERR_GPU;
+    if (s->out_fd < 0) return CVP9_ERR_UNSUPPORTED;
+    const uint32_t w = s->w, h = s->h, cw = (w + 1) / 2, ch = (h + 1) / 2;
+    out->fd = dup(s->out_fd);
+    if (out->fd < 0) return CVP9_ERR_GPU;
+    out->size = s->out_dev_cap;
+    out->width = w;
+    out->height = h;
+    out->offsets[0] = 0;
+    out->offsets[1] = w * h;
+    out->offsets[2] = w * h + cw * ch;
+    out->pitches[0] = w;
+    out->pitches[1] = cw;
+    out->pitches[2] = cw;
+    out->pts = s->pts;
+    return CVP9_OK;
+}
+
+cvp9_err_t hip_export_buffer_alloc(void *ctx, uint64_t size, cvp9_export_buffer_t *out)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b || !out) return CVP9_ERR_INVALID_DATA;
+    if (!b->dmabuf || size == 0) return CVP9_ERR_UNSUPPORTED;
+    hipSetDevice(b->device);
+    const size_t sz = (size + 4095) & ~(size_t)4095;
+    uint8_t *dev = nullptr;
+    if (hipMalloc((void **)&dev, sz) != hipSuccess) return CVP9_ERR_GPU;
+    int fd = export_fd(dev, sz);
+    if (fd < 0) {
+        hipFree(dev);
+        return CVP9_ERR_UNSUPPORTED;
+    }
+    /* CPU view of the VRAM through the DMA-BUF (BAR mapping) */
+    void *map = mmap(nullptr, sz, PROT_READ | PROT_WRITE, MAP_SHARED, fd, 0);
+    if (map == MAP_FAILED) {
+        hsa_amd_portable_close_dmabuf(fd);
+        hipFree(dev);
+        return CVP9_ERR_UNSUPPORTED;
+    }
+    ExportPriv *p = new (std::nothrow) ExportPriv{dev, sz, map};
+    if (!p) {
+        munmap(map, sz);
+        hsa_amd_portable_close_dmabuf(fd);
+        hipFree(dev);
+        return CVP9_ERR_NOMEM;
+    }
+    out->fd = fd;
+    out->mapped = map;
+    out->size = size;
+    out->priv_buf = p;
+    out->priv_mem = dev;
+    return CVP9_OK;
+}
+
+void hip_export_buffer_free(void *ctx, cvp9_export_buffer_t *buf)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b || !buf || !buf->priv_buf) return;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    ExportPriv *p = (ExportPriv *)buf->priv_buf;
+    munmap(p->map, p->size);
+    if (buf->fd >= 0) hsa_amd_portable_close_dmabuf(buf->fd);
+    hipFree(p->dev);
+    delete p;
+    memset(buf, 0, sizeof(*buf));
+    buf->fd = -1;
+}
+
+cvp9_err_t hip_set_render_target(void *ctx, const cvp9_export_buffer_t *target)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b) return CVP9_ERR_INVALID_DATA;
+    if (!target) {
+        b->pending_target = nullptr;
+        b->pending_target_size = 0;
+        return CVP9_OK;
+    }
+    if (!b->dmabuf || !target->priv_buf) return CVP9_ERR_UNSUPPORTED;
+    const ExportPriv *p = (const ExportPriv *)target->priv_buf;
+    b->pending_target = p->dev;
+    b->pending_target_size = target->size;
+    return CVP9_OK;
+}
+
+/* ---- testing hooks (cvp9_hip/testing.h) --------------------------------- */
+
+const char *cvp9_hip_version(void) { return "cvp9_hip 0.1.0 (gfx950)"; }
+
+int cvp9_hip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cvp9_hip_idct_table(int n, double *out)
+{
+    if (!out || (n != 4 && n != 8 && n != 16 && n != 32)) return -1;
+    idct_matrix(n, out);
+    return 0;
+}
+
+cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf)
+{
+    PrepLayout L;
+    return plan_measure(pf, L);
+}
+
+cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b) return CVP9_ERR_INVALID_DATA;
+    b->debug = flags;
+    return CVP9_OK;
+}
+
+} // extern "C"

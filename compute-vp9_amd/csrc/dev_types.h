@@ -1,0 +1,96 @@
+/*
+ * dev_types.h — device-side data layout shared by the host planner and the
+ * HIP kernels.
+ *
+ * HBM layout
+ *   Frame buffer: one allocation per frame, planes back to back:
+ *     Y  [sy  x H ]   sy  = W rounded up to 64 bytes
+ *     U  [suv x ch]   suv = ceil(W/2) rounded up to 64 bytes, ch = ceil(H/2)
+ *     V  [suv x ch]
+ *   Every row starts 64-byte aligned, so 4/16-byte vector accesses at
+ *   4-aligned x never straddle alignment.
+ *
+ *   Prepared frame ("prep"): the parsed frame staged once into HBM:
+ *     DBlock[nblk]   24-byte block records (clamped, normalised modes)
+ *     inter[]        local indices of inter blocks          (motion comp.)
+ *     intra[]        local indices of intra blocks, list order
+ *     cls_blk[c][]   inter non-skip blocks with transform size class c
+ *     cls_pre[c][]   exclusive prefix of their transform counts
+ *     coeffs[]       the parser's int16 coefficients, verbatim
+ *
+ *   Job: one frame of one stream in a batch: output frame, the 8 reference
+ *   slots, the prep, and per-stream scratch (owner map, ready flags).
+ *   A batch is an array of jobs plus per-stage prefix sums of work items, so
+ *   every stage is one launch over all jobs of the batch.
+ */
+#ifndef CVP9_DEV_TYPES_H
+#define CVP9_DEV_TYPES_H
+
+#include <stdint.h>
+
+#define CVP9_CELL 4                 /* owner-map granularity in luma px */
+#define CVP9_OWNER_NONE 0xffffffffu
+#define CVP9_MAX_BATCH 4096
+
+/* block flags */
+#define DB_INTRA   1u
+#define DB_SKIP    2u   /* no residual (skip, or transform grid empty) */
+#define DB_NOREF   4u   /* inter block whose reference slot is empty */
+
+struct DBlock {          /* 24 bytes */
+    uint16_t x, y;       /* luma position */
+    uint8_t  bw, bh;     /* clamped to the frame, > 0 */
+    uint8_t  tx;         /* transform size class 0..3 (4 << tx px) */
+    uint8_t  flags;
+    uint8_t  ymode, uvmode; /* 0 DC, 1 V, 2 H, 3 TM */
+    uint8_t  ref;        /* reference slot 0..7 */
+    uint8_t  pad;
+    int16_t  mvx, mvy;   /* 1/8 pel */
+    uint32_t coeff;      /* element offset of the first transform */
+    uint32_t nx;         /* transforms per row (clamped width / T) */
+};
+
+struct DFrame {          /* 40 bytes */
+    uint8_t *y, *u, *v;
+    uint32_t w, h;       /* luma size */
+    uint32_t sy, suv;    /* plane strides */
+};
+
+struct DJob {
+    DFrame out;
+    DFrame ref[8];                 /* y == nullptr: empty slot */
+    const DBlock   *blocks;
+    const int16_t  *coeffs;
+    const uint32_t *inter;
+    const uint32_t *intra;
+    const uint32_t *cls_blk[4];
+    const uint32_t *cls_pre[4];
+    uint32_t       *owner;         /* ow x oh cells, local block index */
+    uint32_t       *ready;         /* per local block: epoch when final */
+    uint8_t        *nv12;          /* optional NV12 target, pitch W */
+    uint32_t       *err;           /* error word (spin timeout) */
+    uint32_t n_blocks, n_inter, n_intra;
+    uint32_t cls_nblk[4], cls_ntx[4];
+    uint32_t ow, oh;
+    uint32_t epoch;
+    int32_t  lf_thr;               /* 0: no deblocking */
+    uint32_t lf_wx, lf_wy;         /* deblocking windows per row / column */
+    uint32_t fill;                 /* 1: frame not fully covered by blocks */
+};
+
+/* Work-item counts per job, as exclusive prefixes (length njobs + 1). */
+struct DBatch {
+    const DJob     *jobs;
+    uint32_t        njobs;
+    const uint32_t *pre_blocks;    /* owner-map scatter: one wave per block */
+    const uint32_t *pre_inter;     /* motion compensation: one wave / block */
+    const uint32_t *pre_grp[4];    /* residual: groups of 64/N transforms */
+    const uint32_t *pre_intra;     /* intra: one ticket per block */
+    const uint32_t *pre_lf;        /* deblocking windows */
+    const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
+    const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
+    uint32_t       *ticket;        /* intra ticket counter */
+    uint32_t        spin_limit;    /* bound on every dependency wait */
+};
+
+#endif

@@ -1,0 +1,462 @@
+/*
+ * engine.cpp — frame planning, device stream state and batch launch.
+ */
+#include "engine.hpp"
+
+#include <math.h>
+#include <string.h>
+
+#include <mutex>
+
+namespace cvp9 {
+
+static inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
+static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+/* ---- IDCT tables (vp9_reconstruct.c:19-28) ------------------------------ */
+
+void idct_matrix(int n, double *m)
+{
+    /* Same expression and evaluation order as the reference; built with
+     * -ffp-contract=off and the C library's cos, so every entry is
+     * bit-identical to the table the CPU path builds. */
+    const double s0 = sqrt(1.0 / n);
+    const double s1 = sqrt(2.0 / n);
+    for (int i = 0; i < n; i++) {
+        const double s = i == 0 ? s0 : s1;
+        for (int j = 0; j < n; j++)
+            m[i * n + j] = s * cos(((2.0 * j + 1.0) * i * 3.141592653589793) / (2.0 * n));
+    }
+}
+
+hipError_t init_device(int device)
+{
+    static std::mutex mu;
+    static bool done[64];
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (done[device]) return hipSuccess;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    static double m4[16], m8[64], m16[256], m32[1024];
+    idct_matrix(4, m4);
+    idct_matrix(8, m8);
+    idct_matrix(16, m16);
+    idct_matrix(32, m32);
+    e = cvp9_set_idct_tables(m4, m8, m16, m32);
+    if (e == hipSuccess) done[device] = true;
+    return e;
+}
+
+/* ---- planning ---------------------------------------------------------- */
+
+namespace {
+
+/* sets bits [start, start+len) of a bitmap; false if any was already set */
+bool claim_bits(std::vector<uint64_t> &bits, uint64_t start, uint64_t len)
+{
+    while (len) {
+        const uint64_t w = start >> 6, o = start & 63;
+        const uint64_t n = len < 64 - o ? len : 64 - o;
+        const uint64_t mask = (n == 64 ? ~0ull : ((1ull << n) - 1)) << o;
+        if (bits[w] & mask) return false;
+        bits[w] |= mask;
+        start += n;
+        len -= n;
+    }
+    return true;
+}
+
+struct Clamped {
+    int bw, bh;
+};
+
+inline Clamped clamp_block(const vp9_macroblock_info_t &b, int W, int H)
+{
+    /* vp9_reconstruct.c:218-221 */
+    Clamped c;
+    c.bw = b.x + b.width > W ? W - b.x : b.width;
+    c.bh = b.y + b.height > H ? H - b.y : b.height;
+    return c;
+}
+
+inline uint8_t mode_class(uint8_t m)
+{
+    /* vp9_reconstruct.c:67-90: 0 DC, 1 V, 2 H, anything else TM */
+    return m <= 2 ? m : 3;
+}
+
+} // namespace
+
+cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
+{
+    if (!pf) return CVP9_ERR_INVALID_DATA;
+    L = PrepLayout();
+    const uint32_t W = pf->hdr.width, H = pf->hdr.height;
+    /* Sizes the kernels take. The CPU deblock reads one row (column) past
+     * the plane when H (W) is 1 mod 8, and the chroma bilinear clamp
+     * breaks on a 1-pixel reference; those go to the CPU path. */
+    if (W < 2 || H < 2 || W > 16384 || H > 16384) return CVP9_ERR_UNSUPPORTED;
+    if ((W & 7) == 1 || (H & 7) == 1) return CVP9_ERR_UNSUPPORTED;
+    if (pf->num_blocks && !pf->blocks) return CVP9_ERR_INVALID_DATA;
+    L.W = W;
+    L.H = H;
+    L.key = pf->hdr.frame_type == VP9_FRAME_KEY;
+    if (pf->hdr.filter_level) {
+        uint32_t thr = pf->hdr.filter_level; /* vp9_reconstruct.c:139-144 */
+        const uint32_t sh = pf->hdr.sharpness_level;
+        if (sh > 0) {
+            thr >>= (sh > 4) ? 2 : 1;
+            if (thr > 9 - sh) thr = 9 - sh;
+        }
+        if (thr == 0) thr = 1;
+        L.lf_thr = (int)thr;
+    }
+
+    const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
+    thread_local std::vector<uint64_t> bits;
+    bits.assign(((uint64_t)ow * oh + 63) / 64 + 1, 0);
+    uint64_t covered = 0;
+
+    for (uint32_t i = 0; i < pf->num_blocks; i++) {
+        const vp9_macroblock_info_t &b = pf->blocks[i];
+        const Clamped c = clamp_block(b, (int)W, (int)H);
+        if (c.bw <= 0 || c.bh <= 0) continue;
+        /* blocks on the 4x4 grid: neighbour ownership is exact per cell */
+        if ((b.x & 3) || (b.y & 3) || (b.width & 3) || (b.height & 3)) return CVP9_ERR_UNSUPPORTED;
+        const bool intra = b.is_intra != 0;
+        bool res = !b.skip;
+        if (res) {
+            if (b.tx_size > 3) return CVP9_ERR_UNSUPPORTED;
+            const uint32_t T = 4u << b.tx_size;
+            const uint64_t ntx = (uint64_t)(c.bw / T) * (c.bh / T);
+            if (ntx == 0) res = false;
+            else {
+                if (!pf->coeffs) return CVP9_ERR_INVALID_DATA;
+                if ((uint64_t)b.coeff_offset + ntx * T * T > pf->num_coeffs) return CVP9_ERR_INVALID_DATA;
+                if (!intra) {
+                    L.cls_nblk[b.tx_size]++;
+                    L.cls_ntx[b.tx_size] += (uint32_t)ntx;
+                }
+            }
+        }
+        const uint32_t cx0 = b.x >> 2, cx1 = (b.x + c.bw - 1) >> 2;
+        const uint32_t cy0 = b.y >> 2, cy1 = (b.y + c.bh - 1) >> 2;
+        for (uint32_t cy = cy0; cy <= cy1; cy++)
+            if (!claim_bits(bits, (uint64_t)cy * ow + cx0, cx1 - cx0 + 1)) return CVP9_ERR_UNSUPPORTED;
+        covered += (uint64_t)(cx1 - cx0 + 1) * (cy1 - cy0 + 1);
+        L.n_blocks++;
+        if (intra) L.n_intra++; else L.n_inter++;
+    }
+    L.fill = covered != (uint64_t)ow * oh;
+    L.num_coeffs = pf->coeffs ? pf->num_coeffs : 0;
+
+    uint64_t off = 0;
+    L.off_blocks = off; off = align256(off + (uint64_t)L.n_blocks * sizeof(DBlock));
+    L.off_inter = off;  off = align256(off + (uint64_t)L.n_inter * 4);
+    L.off_intra = off;  off = align256(off + (uint64_t)L.n_intra * 4);
+    for (int k = 0; k < 4; k++) {
+        L.off_cls_blk[k] = off; off = align256(off + (uint64_t)L.cls_nblk[k] * 4);
+        L.off_cls_pre[k] = off; off = align256(off + (uint64_t)(L.cls_nblk[k] + 1) * 4);
+    }
+    L.off_coeffs = off; off = align256(off + (uint64_t)L.num_coeffs * 2);
+    L.bytes = off;
+    return CVP9_OK;
+}
+
+void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
+{
+    const int W = (int)L.W, H = (int)L.H;
+    DBlock *blocks = (DBlock *)(dst + L.off_blocks);
+    uint32_t *inter = (uint32_t *)(dst + L.off_inter);
+    uint32_t *intra = (uint32_t *)(dst + L.off_intra);
+    uint32_t *cblk[4], *cpre[4], cn[4] = {0, 0, 0, 0}, ctx[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; k++) {
+        cblk[k] = (uint32_t *)(dst + L.off_cls_blk[k]);
+        cpre[k] = (uint32_t *)(dst + L.off_cls_pre[k]);
+    }
+    uint32_t n = 0, ni = 0, nn = 0;
+    for (uint32_t i = 0; i < pf->num_blocks; i++) {
+        const vp9_macroblock_info_t &b = pf->blocks[i];
+        const Clamped c = clamp_block(b, W, H);
+        if (c.bw <= 0 || c.bh <= 0) continue;
+        DBlock d;
+        memset(&d, 0, sizeof(d));
+        d.x = b.x;
+        d.y = b.y;
+        d.bw = (uint8_t)c.bw;
+        d.bh = (uint8_t)c.bh;
+        const bool is_intra = b.is_intra != 0;
+        d.flags = is_intra ? DB_INTRA : 0;
+        d.ymode = mode_class(b.y_mode);
+        d.uvmode = mode_class(b.uv_mode);
+        const int r = (int)b.ref_frame[0] - 1;
+        d.ref = (uint8_t)((r >= 0 && r < 8) ? r : 8);
+        d.mvx = b.mv[0][0];
+        d.mvy = b.mv[0][1];
+        d.coeff = b.coeff_offset;
+        bool res = !b.skip;
+        uint32_t ntx = 0;
+        if (res) {
+            const uint32_t T = 4u << b.tx_size;
+            d.tx = b.tx_size;
+            d.nx = (uint32_t)c.bw / T;
+            ntx = d.nx * ((uint32_t)c.bh / T);
+            if (ntx == 0) res = false;
+        }
+        if (!res) {
+            d.flags |= DB_SKIP;
+            d.tx = 0;
+            d.nx = 0;
+        }
+        blocks[n] = d;
+        if (is_intra) intra[ni++] = n;
+        else {
+            inter[nn++] = n;
+            if (res) {
+                const int k = d.tx;
+                cblk[k][cn[k]] = n;
+                cpre[k][cn[k]] = ctx[k];
+                cn[k]++;
+                ctx[k] += ntx;
+            }
+        }
+        n++;
+    }
+    for (int k = 0; k < 4; k++) cpre[k][cn[k]] = ctx[k];
+    if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
+}
+
+/* ---- frames and streams ------------------------------------------------ */
+
+size_t frame_bytes(uint32_t w, uint32_t h, uint32_t *sy, uint32_t *suv)
+{
+    const uint32_t s0 = (w + 63) & ~63u;
+    const uint32_t s1 = (((w + 1) / 2) + 63) & ~63u;
+    if (sy) *sy = s0;
+    if (suv) *suv = s1;
+    return (size_t)s0 * h + 2 * (size_t)s1 * ((h + 1) / 2);
+}
+
+DFrame FrameBuf::dframe() const
+{
+    DFrame f;
+    f.y = dev;
+    f.u = dev ? dev + (size_t)sy * h : nullptr;
+    f.v = dev ? f.u + (size_t)suv * ((h + 1) / 2) : nullptr;
+    f.w = w;
+    f.h = h;
+    f.sy = sy;
+    f.suv = suv;
+    return f;
+}
+
+StreamState::~StreamState()
+{
+    hipSetDevice(device_);
+    for (FrameBuf *f : all_) {
+        hipFree(f->dev);
+        delete f;
+    }
+    hipFree(owner_);
+    hipFree(ready_);
+    hipFree(err_);
+}
+
+hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out)
+{
+    uint32_t sy, suv;
+    const size_t need = frame_bytes(w, h, &sy, &suv);
+    FrameBuf *pick = nullptr;
+    for (FrameBuf *f : all_)
+        if (f->refcnt == 0 && f->cap >= need && (!pick || f->cap < pick->cap)) pick = f;
+    if (!pick) {
+        /* drop idle buffers of other sizes before allocating */
+        for (size_t i = 0; i < all_.size();) {
+            if (all_[i]->refcnt == 0) {
+                hipFree(all_[i]->dev);
+                delete all_[i];
+                all_[i] = all_.back();
+                all_.pop_back();
+            } else {
+                i++;
+            }
+        }
+        pick = new FrameBuf();
+        hipError_t e = hipMalloc(&pick->dev, need);
+        if (e != hipSuccess) {
+            delete pick;
+            return e;
+        }
+        pick->cap = need;
+        all_.push_back(pick);
+    }
+    pick->w = w;
+    pick->h = h;
+    pick->sy = sy;
+    pick->suv = suv;
+    *out = pick;
+    return hipSuccess;
+}
+
+void StreamState::release(FrameBuf *f)
+{
+    if (f && f->refcnt > 0) f->refcnt--;
+}
+
+void StreamState::update_refs(bool key, FrameBuf *out)
+{
+    for (int i = 0; i < 8; i++) {
+        if (!key && i != 0) continue;
+        FrameBuf *old = refs_[i];
+        refs_[i] = out;
+        out->refcnt++;
+        release(old);
+    }
+}
+
+hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
+{
+    hipError_t e;
+    const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
+    if (!err_) {
+        if ((e = hipMalloc(&err_, 256)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(err_, 0, 256, s)) != hipSuccess) return e;
+    }
+    if (cells > owner_cap_) {
+        if (owner_) {
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            hipFree(owner_);
+            owner_ = nullptr;
+        }
+        if ((e = hipMalloc(&owner_, cells * 4)) != hipSuccess) return e;
+        owner_cap_ = cells;
+    }
+    const size_t nb = L.n_blocks ? L.n_blocks : 1;
+    if (nb > ready_cap_) {
+        size_t cap = ready_cap_ ? ready_cap_ : 4096;
+        while (cap < nb) cap *= 2;
+        if (ready_) {
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            hipFree(ready_);
+            ready_ = nullptr;
+        }
+        if ((e = hipMalloc(&ready_, cap * 4)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(ready_, 0, cap * 4, s)) != hipSuccess) return e;
+        ready_cap_ = cap;
+        epoch_ = 0;
+    }
+    return hipSuccess;
+}
+
+DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *out, uint8_t *nv12)
+{
+    DJob j;
+    memset(&j, 0, sizeof(j));
+    j.out = out->dframe();
+    for (int i = 0; i < 8; i++) {
+        if (refs_[i]) j.ref[i] = refs_[i]->dframe();
+    }
+    j.blocks = (const DBlock *)(dev + L.off_blocks);
+    j.coeffs = (const int16_t *)(dev + L.off_coeffs);
+    j.inter = (const uint32_t *)(dev + L.off_inter);
+    j.intra = (const uint32_t *)(dev + L.off_intra);
+    for (int k = 0; k < 4; k++) {
+        j.cls_blk[k] = (const uint32_t *)(dev + L.off_cls_blk[k]);
+        j.cls_pre[k] = (const uint32_t *)(dev + L.off_cls_pre[k]);
+        j.cls_nblk[k] = L.cls_nblk[k];
+        j.cls_ntx[k] = L.cls_ntx[k];
+    }
+    j.owner = owner_;
+    j.ready = ready_;
+    j.nv12 = nv12;
+    j.err = err_;
+    j.n_blocks = L.n_blocks;
+    j.n_inter = L.n_inter;
+    j.n_intra = L.n_intra;
+    j.ow = (L.W + 3) / 4;
+    j.oh = (L.H + 3) / 4;
+    if (++epoch_ == 0) epoch_ = 1;
+    j.epoch = epoch_;
+    j.lf_thr = L.lf_thr;
+    j.lf_wx = (L.W + 2 + 63) / 64;
+    j.lf_wy = (L.H + 2 + 63) / 64;
+    j.fill = L.fill;
+    return j;
+}
+
+size_t StreamState::resident_bytes() const
+{
+    size_t b = owner_cap_ * 4 + ready_cap_ * 4 + (err_ ? 256 : 0);
+    for (const FrameBuf *f : all_) b += f->cap;
+    return b;
+}
+
+/* ---- batches ------------------------------------------------------------ */
+
+size_t BatchBuilder::bytes_for(uint32_t njobs)
+{
+    return align256((uint64_t)njobs * sizeof(DJob)) + align256(12ull * (njobs + 1) * 4) + 256;
+}
+
+void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_t *dev,
+                         uint32_t intra_wgs, uint32_t spin_limit)
+{
+    memcpy(host, jobs, (size_t)njobs * sizeof(DJob));
+    const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
+    uint32_t *pre = (uint32_t *)(host + pre_off);
+    const uint32_t stride = njobs + 1;
+    enum { P_FILL, P_BLK, P_INTER, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
+    for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
+    for (uint32_t i = 0; i < njobs; i++) {
+        const DJob &J = jobs[i];
+        uint32_t c[P_N];
+        uint32_t fill = 0;
+        if (J.fill) {
+            const uint64_t fb = (uint64_t)J.out.sy * J.out.h + 2ull * J.out.suv * ((J.out.h + 1) / 2);
+            const uint64_t ob = ((uint64_t)J.ow * J.oh * 4 + 15) & ~15ull;
+            fill = (uint32_t)((fb + 65535) >> 16) + (uint32_t)((ob + 65535) >> 16);
+        }
+        c[P_FILL] = fill;
+        c[P_BLK] = J.n_blocks;
+        c[P_INTER] = J.n_inter;
+        for (int k = 0; k < 4; k++) c[P_G0 + k] = div_up(J.cls_ntx[k], 64u >> (2 + k));
+        c[P_INTRA] = J.n_intra;
+        c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;
+        c[P_NV12] = J.nv12 ? div_up(J.out.w, 64) * div_up(J.out.h, 64) : 0;
+        for (int a = 0; a < P_N; a++) pre[a * stride + i + 1] = pre[a * stride + i] + c[a];
+    }
+    const uint32_t *dpre = (const uint32_t *)(dev + pre_off);
+    b_.jobs = (const DJob *)dev;
+    b_.njobs = njobs;
+    b_.pre_fill = dpre + P_FILL * stride;
+    b_.pre_blocks = dpre + P_BLK * stride;
+    b_.pre_inter = dpre + P_INTER * stride;
+    for (int k = 0; k < 4; k++) b_.pre_grp[k] = dpre + (P_G0 + k) * stride;
+    b_.pre_intra = dpre + P_INTRA * stride;
+    b_.pre_lf = dpre + P_LF * stride;
+    b_.pre_nv12 = dpre + P_NV12 * stride;
+    const uint64_t tk_off = pre_off + align256(12ull * stride * 4);
+    b_.ticket = (uint32_t *)(dev + tk_off);
+    b_.spin_limit = spin_limit;
+    n_.fill = pre[P_FILL * stride + njobs];
+    n_.blocks = pre[P_BLK * stride + njobs];
+    n_.inter = pre[P_INTER * stride + njobs];
+    for (int k = 0; k < 4; k++) n_.grp[k] = pre[(P_G0 + k) * stride + njobs];
+    n_.intra = pre[P_INTRA * stride + njobs];
+    n_.lf = pre[P_LF * stride + njobs];
+    n_.nv12 = pre[P_NV12 * stride + njobs];
+    n_.intra_wgs = intra_wgs;
+    used_ = tk_off + 256;
+}
+
+hipError_t BatchBuilder::launch(const DBatch &b, const StageCounts &n, hipStream_t s)
+{
+    for (int st = 0; st < STAGE_COUNT; st++) {
+        hipError_t e = cvp9_launch_stage(st, b, n, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+} // namespace cvp9

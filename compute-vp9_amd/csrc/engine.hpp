@@ -1,0 +1,114 @@
+/*
+ * engine.hpp — host side of the reconstruction engine (internal C++).
+ *
+ *   plan_*      validate a parsed frame and lay out its device image
+ *   StreamState one decode stream: device reference pool + scratch
+ *   BatchBuilder one launch per stage over the frames of many streams
+ */
+#ifndef CVP9_ENGINE_HPP
+#define CVP9_ENGINE_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+
+#include "cvp9_hip/abi_types.h"
+#include "dev_types.h"
+#include "kernels.h"
+
+namespace cvp9 {
+
+/* Byte layout of a prepared frame (host staging and device copy alike). */
+struct PrepLayout {
+    uint32_t W = 0, H = 0;
+    int key = 0;
+    int lf_thr = 0;
+    uint32_t fill = 0;
+    uint32_t n_blocks = 0, n_inter = 0, n_intra = 0;
+    uint32_t cls_nblk[4] = {0, 0, 0, 0}, cls_ntx[4] = {0, 0, 0, 0};
+    uint32_t num_coeffs = 0;
+    uint64_t off_blocks = 0, off_inter = 0, off_intra = 0;
+    uint64_t off_cls_blk[4] = {0, 0, 0, 0}, off_cls_pre[4] = {0, 0, 0, 0};
+    uint64_t off_coeffs = 0, bytes = 0;
+};
+
+/* Validates pf and fills L. Returns CVP9_OK, CVP9_ERR_UNSUPPORTED (layout
+ * this backend does not take: the caller falls back to the CPU path) or
+ * CVP9_ERR_INVALID_DATA (coefficient references outside the pool). */
+cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L);
+/* Writes the device image of pf into dst (L.bytes bytes). */
+void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst);
+
+/* Device frame buffer: planes Y, U, V back to back, 64-byte row pitch. */
+struct FrameBuf {
+    uint8_t *dev = nullptr;
+    size_t cap = 0;
+    uint32_t w = 0, h = 0, sy = 0, suv = 0;
+    int refcnt = 0;
+    DFrame dframe() const;
+};
+
+size_t frame_bytes(uint32_t w, uint32_t h, uint32_t *sy, uint32_t *suv);
+
+class StreamState {
+public:
+    explicit StreamState(int device) : device_(device) {}
+    ~StreamState();
+    StreamState(const StreamState &) = delete;
+    StreamState &operator=(const StreamState &) = delete;
+
+    /* frame for the next output; hipSuccess or an allocation error */
+    hipError_t acquire(uint32_t w, uint32_t h, FrameBuf **out);
+    void release(FrameBuf *f);
+    /* reference-pool rule of the CPU path: KEY -> all 8 slots, else slot 0
+     * (src/decoder/vp9_decoder.c:224-231) */
+    void update_refs(bool key, FrameBuf *out);
+    /* owner map and ready flags large enough for L; may synchronize s */
+    hipError_t ensure_scratch(const PrepLayout &L, hipStream_t s);
+    /* job descriptor for one frame of this stream */
+    DJob make_job(const PrepLayout &L, const uint8_t *dev_prep, FrameBuf *out, uint8_t *nv12);
+    /* bytes of HBM this stream holds */
+    size_t resident_bytes() const;
+    uint32_t *err_word() const { return err_; }
+    FrameBuf *ref(int i) const { return refs_[i]; }
+
+private:
+    int device_;
+    std::vector<FrameBuf *> all_;
+    FrameBuf *refs_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t *owner_ = nullptr;
+    size_t owner_cap_ = 0;
+    uint32_t *ready_ = nullptr;
+    size_t ready_cap_ = 0;
+    uint32_t *err_ = nullptr;
+    uint32_t epoch_ = 0;
+};
+
+/* Packs job descriptors + per-stage prefix sums for one batch into a host
+ * buffer mirrored at a device address, and launches the stage sequence. */
+class BatchBuilder {
+public:
+    static size_t bytes_for(uint32_t njobs);
+    /* host/dev point to bytes_for(njobs) bytes each */
+    void build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_t *dev,
+               uint32_t intra_wgs, uint32_t spin_limit);
+    size_t used() const { return used_; }
+    const DBatch &batch() const { return b_; }
+    const StageCounts &counts() const { return n_; }
+    static hipError_t launch(const DBatch &b, const StageCounts &n, hipStream_t s);
+
+private:
+    DBatch b_{};
+    StageCounts n_{};
+    size_t used_ = 0;
+};
+
+/* Once per device: upload the IDCT matrices (computed with the C library's
+ * cos exactly as the reference builds them). */
+hipError_t init_device(int device);
+/* Host copy of the N x N matrix the kernels use. */
+void idct_matrix(int n, double *m);
+
+} // namespace cvp9
+
+#endif

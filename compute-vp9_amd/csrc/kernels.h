@@ -1,0 +1,33 @@
+/* kernels.h — host-side interface of the reconstruction kernels. */
+#ifndef CVP9_KERNELS_H
+#define CVP9_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include "dev_types.h"
+
+enum Stage {
+    STAGE_FILL = 0,
+    STAGE_OWNER,
+    STAGE_MC,
+    STAGE_RES4,
+    STAGE_RES8,
+    STAGE_RES16,
+    STAGE_RES32,
+    STAGE_INTRA,
+    STAGE_LF,
+    STAGE_NV12,
+    STAGE_COUNT
+};
+
+/* Total work items of each stage over the whole batch (the last element of
+ * each prefix array in DBatch), plus launch parameters. */
+struct StageCounts {
+    uint32_t fill, blocks, inter, grp[4], intra, lf, nv12;
+    uint32_t intra_wgs;   /* persistent workgroups for the intra stage */
+};
+
+hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double *m16,
+                                const double *m32);
+hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s);
+
+#endif

@@ -1,0 +1,623 @@
+/*
+ * kernels.hip — CDNA4 (gfx950) reconstruction kernels.
+ *
+ * Every kernel runs over a batch of jobs (frames of independent streams);
+ * a single-stream decode is a batch of one. Work items are found from
+ * per-stage exclusive prefix sums over the jobs (dev_types.h).
+ *
+ * Arithmetic follows the reference CPU reconstruction exactly
+ * (src/decoder/vp9_reconstruct.c): FP64 matrix IDCT with separately rounded
+ * multiplies and adds in ascending k, bilinear 1/8-pel MC with edge clamp,
+ * whole-block intra in list order, 4-tap luma deblock. Compile with
+ * -ffp-contract=off; the IDCT also uses explicit __dmul_rn/__dadd_rn.
+ *
+ * Stage order per batch (one stream, in order):
+ *   k_fill (only jobs not fully covered) -> k_owner -> k_mc -> k_residual<N>
+ *   -> k_intra -> k_lf -> k_nv12
+ */
+#include <hip/hip_runtime.h>
+#include "dev_types.h"
+#include "kernels.h"
+
+__constant__ double c_M4[16];
+__constant__ double c_M8[64];
+__constant__ double c_M16[256];
+__constant__ double c_M32[1024];
+
+template <int N> __device__ __forceinline__ double mat(int idx);
+template <> __device__ __forceinline__ double mat<4>(int idx) { return c_M4[idx]; }
+template <> __device__ __forceinline__ double mat<8>(int idx) { return c_M8[idx]; }
+template <> __device__ __forceinline__ double mat<16>(int idx) { return c_M16[idx]; }
+template <> __device__ __forceinline__ double mat<32>(int idx) { return c_M32[idx]; }
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+/* largest j with pre[j] <= w  (pre non-decreasing, pre[0] = 0 <= w < pre[n]) */
+__device__ __forceinline__ uint32_t find_job(const uint32_t *pre, uint32_t n, uint32_t w)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= w) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t wave_ballot(bool p)
+{
+    return __ballot(p);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 128 fill of the output frame and NONE fill of the owner map, for jobs    */
+/* whose blocks do not cover the frame (vp9_frame.c:27-30 initial value).   */
+
+__global__ __launch_bounds__(256) void k_fill(DBatch B)
+{
+    const uint32_t w = blockIdx.x;
+    const uint32_t j = find_job(B.pre_fill, B.njobs, w);
+    const DJob &J = B.jobs[j];
+    const uint32_t chunk = w - B.pre_fill[j];
+    const uint64_t fbytes = (uint64_t)J.out.sy * J.out.h +
+                            2ull * J.out.suv * ((J.out.h + 1) / 2);
+    const uint64_t obytes = ((uint64_t)J.ow * J.oh * 4 + 15) & ~15ull;
+    const uint64_t fchunks = (fbytes + 65535) >> 16;
+    uint8_t *base;
+    uint64_t len;
+    uint4 v;
+    if (chunk < fchunks) {
+        base = J.out.y + ((uint64_t)chunk << 16);
+        len = fbytes - ((uint64_t)chunk << 16);
+        v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+    } else {
+        const uint64_t c = chunk - fchunks;
+        base = (uint8_t *)J.owner + (c << 16);
+        len = obytes - (c << 16);
+        v = make_uint4(CVP9_OWNER_NONE, CVP9_OWNER_NONE, CVP9_OWNER_NONE, CVP9_OWNER_NONE);
+    }
+    if (len > 65536) len = 65536;
+    uint4 *p = (uint4 *)base;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(len >> 4); i += 256) p[i] = v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
+/* Also resets the intra ticket counter for this batch.                     */
+
+__global__ __launch_bounds__(256) void k_owner(DBatch B)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (wv >= B.pre_blocks[B.njobs]) return;
+    const uint32_t j = find_job(B.pre_blocks, B.njobs, wv);
+    const DJob &J = B.jobs[j];
+    const uint32_t b = wv - B.pre_blocks[j];
+    const DBlock k = J.blocks[b];
+    const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
+    const uint32_t ncx = ((k.x + k.bw - 1) >> 2) - cx0 + 1;
+    const uint32_t ncy = ((k.y + k.bh - 1) >> 2) - cy0 + 1;
+    for (uint32_t i = lane; i < ncx * ncy; i += 64) {
+        const uint32_t r = i / ncx, c = i - r * ncx;
+        J.owner[(cy0 + r) * J.ow + cx0 + c] = b;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Motion compensation (vp9_reconstruct.c:94-131, :248-264, :312-334):       */
+/* one wave per inter block, each lane 4 horizontal pixels of a row.        */
+
+__device__ __forceinline__ void mc_plane(const uint8_t *__restrict__ ref, uint32_t rs,
+                                         int rw, int rh, uint8_t *__restrict__ dst,
+                                         uint32_t ds, int bx, int by, int w, int h,
+                                         int mvx, int mvy, int lane)
+{
+    const int px8 = bx * 8 + mvx, py8 = by * 8 + mvy;
+    const int ix = px8 >> 3, iy = py8 >> 3;
+    const int fx = px8 & 7, fy = py8 & 7;
+    const int nq = (w + 3) >> 2;
+    const int rows = 64 / nq;
+    const int rq = lane / nq, q = lane - rq * nq;
+    if (rq >= rows) return;
+    const int c0 = q * 4;
+    int xs[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) xs[e] = clampi(ix + c0 + e, 0, rw - 1);
+    for (int r = rq; r < h; r += rows) {
+        const uint8_t *r0 = ref + (size_t)clampi(iy + r, 0, rh - 1) * rs;
+        const uint8_t *r1 = ref + (size_t)clampi(iy + r + 1, 0, rh - 1) * rs;
+        uint32_t a[5], b[5];
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+            a[e] = r0[xs[e]];
+            b[e] = r1[xs[e]];
+        }
+        uint8_t *d = dst + (size_t)r * ds + c0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if (c0 + e < w) {
+                const uint32_t top = (a[e] * (8 - fx) + a[e + 1] * fx + 4) >> 3;
+                const uint32_t bot = (b[e] * (8 - fx) + b[e + 1] * fx + 4) >> 3;
+                d[e] = (uint8_t)((top * (8 - fy) + bot * fy + 4) >> 3);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void fill_plane(uint8_t *dst, uint32_t ds, int w, int h,
+                                           uint8_t v, int lane)
+{
+    for (int i = lane; i < w * h; i += 64) {
+        const int r = i / w, c = i - r * w;
+        dst[(size_t)r * ds + c] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mc(DBatch B)
+{
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wv >= B.pre_inter[B.njobs]) return;
+    const uint32_t j = find_job(B.pre_inter, B.njobs, wv);
+    const DJob &J = B.jobs[j];
+    const DBlock k = J.blocks[J.inter[wv - B.pre_inter[j]]];
+    const int bx = k.x, by = k.y, bw = k.bw, bh = k.bh;
+    const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
+    uint8_t *dy = J.out.y + (size_t)by * J.out.sy + bx;
+    uint8_t *du = J.out.u + (size_t)cby * J.out.suv + cbx;
+    uint8_t *dv = J.out.v + (size_t)cby * J.out.suv + cbx;
+    const DFrame *R = k.ref < 8 ? &J.ref[k.ref] : nullptr;
+    if (!R || !R->y) { /* missing reference: flat 127 (:258-263, :328-333) */
+        fill_plane(dy, J.out.sy, bw, bh, 127, lane);
+        fill_plane(du, J.out.suv, cbw, cbh, 127, lane);
+        fill_plane(dv, J.out.suv, cbw, cbh, 127, lane);
+        return;
+    }
+    mc_plane(R->y, R->sy, (int)R->w, (int)R->h, dy, J.out.sy, bx, by, bw, bh,
+             k.mvx, k.mvy, lane);
+    /* chroma: MV halved toward zero, bounds from half the luma size */
+    const int cmx = k.mvx / 2, cmy = k.mvy / 2;
+    const int rw = (int)R->w / 2, rh = (int)R->h / 2;
+    mc_plane(R->u, R->suv, rw, rh, du, J.out.suv, cbx, cby, cbw, cbh, cmx, cmy, lane);
+    mc_plane(R->v, R->suv, rw, rh, dv, J.out.suv, cbx, cby, cbw, cbh, cmx, cmy, lane);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 inverse DCT + residual add (vp9_reconstruct.c:13-60, :336-363).      */
+/*                                                                          */
+/* One wave transforms G = 64/N transforms of size N at once:               */
+/*   pass 1: lane (t, j) holds column j of transform t and accumulates      */
+/*           temp[i][j] = sum_k M[k][i] * in[k][j] for all i (k ascending); */
+/*           M[k][i] is wave-uniform (scalar loads from constant memory).   */
+/*   temp goes through LDS (row pitch N+1 doubles) to be read by rows.      */
+/*   pass 2: lane (t, r) accumulates out[r][c] = sum_k temp[r][k] * M[k][c].*/
+/* A k-step is skipped when its input row (pass 1) or column (pass 2) is    */
+/* zero in every transform of the wave. Adding a product with an exact      */
+/* zero operand leaves a nonzero sum unchanged and can only flip the sign   */
+/* of a zero sum, which the final round-half-away conversion ignores, so    */
+/* any superset of the nonzero terms taken in ascending k is bit-exact.     */
+
+template <int N>
+__device__ __forceinline__ void idct_add_wave(const int16_t *coef, uint8_t *dst,
+                                              uint32_t stride, double *lds, int lane)
+{
+    const int t = lane / N;
+    const int j = lane % N;
+    double acc[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] = 0.0;
+    bool col_nz = false;
+
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const int x = coef ? (int)coef[k * N + j] : 0;
+        col_nz |= (x != 0);
+        if (wave_ballot(x != 0)) {
+            const double xd = (double)x;
+#pragma unroll
+            for (int i = 0; i < N; i++) acc[i] = __dadd_rn(acc[i], __dmul_rn(mat<N>(k * N + i), xd));
+        }
+    }
+    double *tt = lds + t * (N * (N + 1));
+#pragma unroll
+    for (int i = 0; i < N; i++) tt[i * (N + 1) + j] = acc[i];
+
+    /* nonzero input columns over all transforms of the wave, folded to N bits */
+    uint64_t m = wave_ballot(col_nz);
+#pragma unroll
+    for (int s = 32; s >= N; s >>= 1) m |= m >> s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const int r = j;
+#pragma unroll
+    for (int c = 0; c < N; c++) acc[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if ((m >> k) & 1) {
+            const double u = tt[r * (N + 1) + k];
+#pragma unroll
+            for (int c = 0; c < N; c++) acc[c] = __dadd_rn(acc[c], __dmul_rn(u, mat<N>(k * N + c)));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+
+    if (coef) {
+        uint8_t *row = dst + (size_t)r * stride;
+#pragma unroll
+        for (int c4 = 0; c4 < N; c4 += 4) {
+            uint32_t px = *(const uint32_t *)(row + c4);
+            uint32_t o = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const double v = acc[c4 + e];
+                int32_t rr = (int32_t)__dadd_rn(v, v >= 0.0 ? 0.5 : -0.5);
+                rr = clampi(rr, -32768, 32767);
+                const int p = (int)((px >> (8 * e)) & 0xff);
+                o |= (uint32_t)clampi(p + rr, 0, 255) << (8 * e);
+            }
+            *(uint32_t *)(row + c4) = o;
+        }
+    }
+}
+
+/* lane's transform in a group of a job's size class c: coefficient pointer
+ * and destination origin, or coef = nullptr past the end */
+template <int N>
+__device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
+                                          const int16_t *&coef, uint8_t *&dst)
+{
+    coef = nullptr;
+    dst = nullptr;
+    if (txi >= J.cls_ntx[cls]) return;
+    const uint32_t *pre = J.cls_pre[cls];
+    uint32_t lo = 0, hi = J.cls_nblk[cls];
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= txi) lo = mid; else hi = mid;
+    }
+    const DBlock k = J.blocks[J.cls_blk[cls][lo]];
+    const uint32_t l = txi - pre[lo];
+    const uint32_t ty = l / k.nx, tx = l - ty * k.nx;
+    coef = J.coeffs + k.coeff + (size_t)l * (N * N);
+    dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_residual(DBatch B)
+{
+    constexpr int C = N == 4 ? 0 : N == 8 ? 1 : N == 16 ? 2 : 3;
+    constexpr int G = 64 / N;
+    __shared__ double lds[4][G * N * (N + 1)];
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const uint32_t *pre = B.pre_grp[C];
+    if (wv >= pre[B.njobs]) return;
+    const uint32_t j = find_job(pre, B.njobs, wv);
+    const DJob &J = B.jobs[j];
+    const uint32_t txi = (wv - pre[j]) * G + lane / N;
+    const int16_t *coef;
+    uint8_t *dst;
+    locate_tx<N>(J, C, txi, coef, dst);
+    idct_add_wave<N>(coef, dst, J.out.sy, lds[threadIdx.x >> 6], lane);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Intra blocks (vp9_reconstruct.c:63-91, :224-247, :272-311, :336-363).     */
+/*                                                                          */
+/* The reference predicts blocks strictly in list order, so a neighbour     */
+/* sample holds its final value only if its owner block comes earlier in    */
+/* the list; otherwise it still holds the initial 128. Inter blocks are     */
+/* final before this kernel starts. Intra blocks are taken by workgroups in */
+/* list order through an atomic ticket; a block waits (bounded spin) only   */
+/* on earlier intra neighbours, which running workgroups already hold, so   */
+/* every wait ends.                                                          */
+
+__device__ __forceinline__ uint8_t nb_sample(const DJob &J, const uint8_t *plane, uint32_t stride,
+                                             int px, int py, int cx, int cy, uint32_t b)
+{
+    const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
+    if (o == CVP9_OWNER_NONE || o >= b) return 128;
+    return plane[(size_t)py * stride + px];
+}
+
+template <int N>
+__device__ void intra_residual(const DJob &J, const DBlock &k, double *lds, int wave, int lane)
+{
+    constexpr int G = 64 / N;
+    const uint32_t ny = k.bh / N;
+    const uint32_t ntx = k.nx * ny;
+    for (uint32_t g = wave; g * G < ntx; g += 4) {
+        const uint32_t l = g * G + lane / N;
+        const int16_t *coef = nullptr;
+        uint8_t *dst = nullptr;
+        if (l < ntx) {
+            const uint32_t ty = l / k.nx, tx = l - ty * k.nx;
+            coef = J.coeffs + k.coeff + (size_t)l * (N * N);
+            dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
+        }
+        idct_add_wave<N>(coef, dst, J.out.sy, lds, lane);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_intra(DBatch B)
+{
+    __shared__ uint32_t s_ticket;
+    __shared__ uint8_t s_above[3][256];
+    __shared__ uint8_t s_left[3][256];
+    __shared__ uint32_t s_sum[3];
+    __shared__ uint8_t s_tl[3];
+    __shared__ double s_tmp[4][2 * 32 * 33];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t total = B.pre_intra[B.njobs];
+
+    for (;;) {
+        if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
+        __syncthreads();
+        const uint32_t tk = s_ticket;
+        if (tk >= total) break;
+        const uint32_t j = find_job(B.pre_intra, B.njobs, tk);
+        const DJob &J = B.jobs[j];
+        const uint32_t b = J.intra[tk - B.pre_intra[j]];
+        const DBlock k = J.blocks[b];
+        const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
+
+        /* 1. wait for earlier intra blocks owning a neighbour cell */
+        {
+            const int cx0 = x >> 2, cx1 = (x + bw - 1) >> 2;
+            const int cy0 = y >> 2, cy1 = (y + bh - 1) >> 2;
+            const int na = y > 0 ? cx1 - cx0 + 1 : 0;
+            const int nl = x > 0 ? cy1 - cy0 + 1 : 0;
+            const int nt = (x > 0 && y > 0) ? 1 : 0;
+            for (int i = tid; i < na + nl + nt; i += 256) {
+                int cx, cy;
+                if (i < na) { cx = cx0 + i; cy = (y - 1) >> 2; }
+                else if (i < na + nl) { cx = (x - 1) >> 2; cy = cy0 + (i - na); }
+                else { cx = (x - 1) >> 2; cy = (y - 1) >> 2; }
+                const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
+                if (o != CVP9_OWNER_NONE && o < b && (J.blocks[o].flags & DB_INTRA)) {
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(&J.ready[o], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) != J.epoch) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if (++spins > B.spin_limit) {
+                            atomicOr(J.err, 1u);
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+        /* 2. neighbours of Y, U, V (missing: 127; later or unowned: 128) */
+        const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
+        if (tid < 3) s_sum[tid] = 0;
+        for (int i = tid; i < 3 * 256; i += 256) {
+            const int p = i >> 8, e = i & 255;
+            const uint8_t *pl = p == 0 ? J.out.y : (p == 1 ? J.out.u : J.out.v);
+            const uint32_t st = p == 0 ? J.out.sy : J.out.suv;
+            const int px = p == 0 ? x : cbx, py = p == 0 ? y : cby;
+            const int w = p == 0 ? bw : cbw, h = p == 0 ? bh : cbh;
+            const int sh = p == 0 ? 2 : 1; /* sample -> owner cell */
+            if (e < w)
+                s_above[p][e] = py > 0 ? nb_sample(J, pl, st, px + e, py - 1, (px + e) >> sh,
+                                                    (py - 1) >> sh, b)
+                                       : (uint8_t)127;
+            if (e < h)
+                s_left[p][e] = px > 0 ? nb_sample(J, pl, st, px - 1, py + e, (px - 1) >> sh,
+                                                   (py + e) >> sh, b)
+                                      : (uint8_t)127;
+            if (e == 0)
+                s_tl[p] = (px > 0 && py > 0)
+                              ? nb_sample(J, pl, st, px - 1, py - 1, (px - 1) >> sh, (py - 1) >> sh, b)
+                              : (uint8_t)127;
+        }
+        __syncthreads();
+        /* DC sums */
+        for (int i = tid; i < 3 * 512; i += 256) {
+            const int p = i >> 9, e = i & 511;
+            const int w = p == 0 ? bw : cbw, h = p == 0 ? bh : cbh;
+            const int mode = p == 0 ? k.ymode : k.uvmode;
+            if (mode != 0) continue;
+            uint32_t v = 0;
+            if (e < 256) { if (e < w) v = s_above[p][e]; }
+            else if (e - 256 < h) v = s_left[p][e - 256];
+            if (v) atomicAdd(&s_sum[p], v);
+        }
+        __syncthreads();
+
+        /* 3. prediction */
+        for (int p = 0; p < 3; p++) {
+            uint8_t *pl = p == 0 ? J.out.y : (p == 1 ? J.out.u : J.out.v);
+            const uint32_t st = p == 0 ? J.out.sy : J.out.suv;
+            const int px = p == 0 ? x : cbx, py = p == 0 ? y : cby;
+            const int w = p == 0 ? bw : cbw, h = p == 0 ? bh : cbh;
+            const int mode = p == 0 ? k.ymode : k.uvmode;
+            const uint32_t dc = (s_sum[p] + (uint32_t)((w + h) >> 1)) / (uint32_t)(w + h);
+            const int tl = s_tl[p];
+            for (int i = tid; i < w * h; i += 256) {
+                const int r = i / w, c = i - r * w;
+                int v;
+                if (mode == 0) v = (int)(uint8_t)dc;
+                else if (mode == 1) v = s_above[p][c];
+                else if (mode == 2) v = s_left[p][r];
+                else v = clampi(s_above[p][c] + s_left[p][r] - tl, 0, 255);
+                pl[(size_t)(py + r) * st + px + c] = (uint8_t)v;
+            }
+        }
+        __syncthreads();
+
+        /* 4. luma residual of the block */
+        if (!(k.flags & DB_SKIP)) {
+            double *lds = s_tmp[wave];
+            switch (k.tx) {
+            case 0: intra_residual<4>(J, k, lds, wave, lane); break;
+            case 1: intra_residual<8>(J, k, lds, wave, lane); break;
+            case 2: intra_residual<16>(J, k, lds, wave, lane); break;
+            default: intra_residual<32>(J, k, lds, wave, lane); break;
+            }
+        }
+
+        /* 5. publish */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_store(&J.ready[b], J.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deblocking (vp9_reconstruct.c:134-191): windows [64i-2, 64i+62) in both  */
+/* axes. Each window holds every sample its own edges read or write, and    */
+/* the vertical pass reads the window's own horizontal-pass output, so one  */
+/* workgroup filters a window in LDS with no cross-window traffic.          */
+
+__device__ __forceinline__ void lf_edge(uint8_t &p1, uint8_t &p0, uint8_t &q0, uint8_t &q1, int thr)
+{
+    const int a = p1, b = p0, c = q0, d = q1;
+    if (abs(b - c) * 2 + abs(a - d) / 2 <= thr) {
+        const int delta = clampi((3 * (c - b) + 4) >> 3, -thr, thr);
+        p0 = (uint8_t)clampi(b + delta, 0, 255);
+        q0 = (uint8_t)clampi(c - delta, 0, 255);
+        p1 = (uint8_t)clampi(a + delta / 2, 0, 255);
+        q1 = (uint8_t)clampi(d - delta / 2, 0, 255);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lf(DBatch B)
+{
+    __shared__ uint8_t tile[64][65];
+    const uint32_t w = blockIdx.x;
+    const uint32_t j = find_job(B.pre_lf, B.njobs, w);
+    const DJob &J = B.jobs[j];
+    const uint32_t l = w - B.pre_lf[j];
+    const int wy = (int)(l / J.lf_wx), wx = (int)(l - (uint32_t)wy * J.lf_wx);
+    const int W = (int)J.out.w, H = (int)J.out.h;
+    const int r0 = max(64 * wy - 2, 0), r1 = min(64 * wy + 62, H);
+    const int c0 = max(64 * wx - 2, 0), c1 = min(64 * wx + 62, W);
+    const int nr = r1 - r0, nc = c1 - c0;
+    const int thr = J.lf_thr;
+    uint8_t *Y = J.out.y;
+    const uint32_t sy = J.out.sy;
+    const int tid = threadIdx.x;
+
+    for (int i = tid; i < 64 * 64; i += 256) {
+        const int r = i >> 6, c = i & 63;
+        if (r < nr && c < nc) tile[r][c] = Y[(size_t)(r0 + r) * sy + c0 + c];
+    }
+    __syncthreads();
+    for (int i = tid; i < 8 * 64; i += 256) { /* horizontal edges */
+        const int e = i >> 6, c = i & 63;
+        const int ey = 64 * wy + 8 * e;
+        if (ey >= 8 && ey < H && c < nc) {
+            const int lr = ey - r0;
+            lf_edge(tile[lr - 2][c], tile[lr - 1][c], tile[lr][c], tile[lr + 1][c], thr);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 8 * 64; i += 256) { /* vertical edges */
+        const int e = i >> 6, r = i & 63;
+        const int ex = 64 * wx + 8 * e;
+        if (ex >= 8 && ex < W && r < nr) {
+            const int lc = ex - c0;
+            lf_edge(tile[r][lc - 2], tile[r][lc - 1], tile[r][lc], tile[r][lc + 1], thr);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 64 * 64; i += 256) {
+        const int r = i >> 6, c = i & 63;
+        if (r < nr && c < nc) Y[(size_t)(r0 + r) * sy + c0 + c] = tile[r][c];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* I420 -> NV12 into the job's render target, pitch W (vp9_nv12_pack and    */
+/* vaapi_entrypoint.c:131-153 layout). One workgroup per 64x64 luma tile.    */
+
+__global__ __launch_bounds__(256) void k_nv12(DBatch B)
+{
+    const uint32_t w = blockIdx.x;
+    const uint32_t j = find_job(B.pre_nv12, B.njobs, w);
+    const DJob &J = B.jobs[j];
+    const uint32_t l = w - B.pre_nv12[j];
+    const uint32_t W = J.out.w, H = J.out.h, cw = (W + 1) / 2, ch = (H + 1) / 2;
+    const uint32_t tx = (W + 63) / 64;
+    const uint32_t ty0 = (l / tx) * 64, tx0 = (l % tx) * 64;
+    uint8_t *dst = J.nv12;
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += 256) {
+        const uint32_t r = ty0 + (i >> 6), c = tx0 + (i & 63);
+        if (r < H && c < W) dst[(size_t)r * W + c] = J.out.y[(size_t)r * J.out.sy + c];
+    }
+    uint8_t *uv = dst + (size_t)W * H;
+    for (uint32_t i = threadIdx.x; i < 32 * 32; i += 256) {
+        const uint32_t r = ty0 / 2 + (i >> 5), c = tx0 / 2 + (i & 31);
+        if (r < ch && c < cw) {
+            uv[(size_t)r * 2 * cw + 2 * c] = J.out.u[(size_t)r * J.out.suv + c];
+            uv[(size_t)r * 2 * cw + 2 * c + 1] = J.out.v[(size_t)r * J.out.suv + c];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* host-side launchers                                                       */
+
+hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double *m16,
+                                const double *m32)
+{
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_M4), m4, 16 * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_M8), m8, 64 * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_M16), m16, 256 * sizeof(double))) != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_M32), m32, 1024 * sizeof(double));
+}
+
+static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s)
+{
+    switch (stage) {
+    case STAGE_FILL:
+        if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b);
+        break;
+    case STAGE_OWNER:
+        hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 4) : 1), dim3(256), 0, s, b);
+        break;
+    case STAGE_MC:
+        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(div_up(n.inter, 4)), dim3(256), 0, s, b);
+        break;
+    case STAGE_RES4:
+        if (n.grp[0]) hipLaunchKernelGGL(k_residual<4>, dim3(div_up(n.grp[0], 4)), dim3(256), 0, s, b);
+        break;
+    case STAGE_RES8:
+        if (n.grp[1]) hipLaunchKernelGGL(k_residual<8>, dim3(div_up(n.grp[1], 4)), dim3(256), 0, s, b);
+        break;
+    case STAGE_RES16:
+        if (n.grp[2]) hipLaunchKernelGGL(k_residual<16>, dim3(div_up(n.grp[2], 4)), dim3(256), 0, s, b);
+        break;
+    case STAGE_RES32:
+        if (n.grp[3]) hipLaunchKernelGGL(k_residual<32>, dim3(div_up(n.grp[3], 4)), dim3(256), 0, s, b);
+        break;
+    case STAGE_INTRA:
+        if (n.intra) {
+            const uint32_t g = n.intra < n.intra_wgs ? n.intra : n.intra_wgs;
+            hipLaunchKernelGGL(k_intra, dim3(g), dim3(256), 0, s, b);
+        }
+        break;
+    case STAGE_LF:
+        if (n.lf) hipLaunchKernelGGL(k_lf, dim3(n.lf), dim3(256), 0, s, b);
+        break;
+    case STAGE_NV12:
+        if (n.nv12) hipLaunchKernelGGL(k_nv12, dim3(n.nv12), dim3(256), 0, s, b);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

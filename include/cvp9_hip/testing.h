@@ -1,0 +1,40 @@
+/*
+ * cvp9_hip/testing.h — test and diagnostics hooks of the HIP backend.
+ *
+ * No counterpart in the reference. These let the test suite pin pieces of
+ * the backend on their own: the IDCT tables the kernels use, the host-side
+ * validation that decides between this backend and the CPU path, and
+ * stage-level parity (deblocking switched off to compare with the CPU
+ * reconstruction before its deblocking pass).
+ */
+#ifndef CVP9_HIP_TESTING_H
+#define CVP9_HIP_TESTING_H
+
+#include "cvp9_hip/abi_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version string. */
+const char *cvp9_hip_version(void);
+
+/* Number of HIP devices visible (0 without a GPU; never initialises one). */
+int cvp9_hip_device_count(void);
+
+/* Host copy of the N x N inverse-DCT matrix uploaded to the kernels
+ * (row-major, M[i][j]); N in {4, 8, 16, 32}. Returns 0, or -1 for other N. */
+int cvp9_hip_idct_table(int n, double *out);
+
+/* Runs the backend's validation of pf without a device: CVP9_OK when the
+ * HIP path takes the frame, else the code hip_decode_frame would return. */
+cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf);
+
+/* Debug flags for one backend context (bit 0: skip the deblocking stage). */
+#define CVP9_HIP_DEBUG_NO_DEBLOCK 1u
+cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
