@@ -1,0 +1,332 @@
+"""compute_vp9_amd — Python bindings of the MI355X VP9 reconstruction backend.
+
+The product is the native library ``libcvp9_hip.so`` (HIP kernels for gfx950
+plus the C-ABI host side, declared in ``include/cvp9_hip/*.h``). This module
+only loads it with ctypes and mirrors its structs, so tests and the benchmark
+drive exactly the entry points a C caller (the reference decoder's
+``src/decoder/vp9_decoder.c`` dispatch) would use.
+
+The directory name is not a Python identifier; load it with
+``load_package()`` from ``_pkgload.py`` at the repository root, or import it
+after registering it under the name ``compute_vp9_amd``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libcvp9_hip.so")
+
+# ---- error codes and enums (include/compute_vp9/decoder.h:21-37) ---------
+CVP9_OK = 0
+CVP9_ERR_NOMEM = -1
+CVP9_ERR_NO_BACKEND = -2
+CVP9_ERR_INVALID_DATA = -3
+CVP9_ERR_UNSUPPORTED = -4
+CVP9_ERR_GPU = -5
+CVP9_ERR_AGAIN = -6
+ERR_NAMES = {0: "OK", -1: "NOMEM", -2: "NO_BACKEND", -3: "INVALID_DATA",
+             -4: "UNSUPPORTED", -5: "GPU", -6: "AGAIN"}
+
+VP9_FRAME_KEY = 0
+VP9_FRAME_NON_KEY = 1
+CVP9_HIP_INFLIGHT = 3
+CVP9_HIP_DEBUG_NO_DEBLOCK = 1
+
+
+class Cvp9Error(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"{what}: {ERR_NAMES.get(code, code)}")
+        self.code = code
+
+
+# ---- structs (layouts pinned against the reference headers in tests) ------
+class FrameHeader(C.Structure):
+    """vp9_frame_header_t (src/decoder/vp9_bitstream.h:39-87), 76 bytes."""
+    _fields_ = [
+        ("frame_type", C.c_int), ("width", C.c_uint32), ("height", C.c_uint32),
+        ("render_width", C.c_uint32), ("render_height", C.c_uint32),
+        ("profile", C.c_uint8), ("bit_depth", C.c_uint8), ("color_space", C.c_int),
+        ("color_range", C.c_bool), ("subsampling_x", C.c_uint8),
+        ("subsampling_y", C.c_uint8), ("show_frame", C.c_bool),
+        ("show_existing_frame", C.c_bool), ("frame_to_show_map_idx", C.c_uint8),
+        ("intra_only", C.c_bool), ("reset_frame_context", C.c_uint8),
+        ("error_resilient", C.c_bool), ("allow_high_precision_mv", C.c_bool),
+        ("interp_filter", C.c_int), ("refresh_frame_context", C.c_bool),
+        ("frame_parallel", C.c_bool), ("frame_context_idx", C.c_uint8),
+        ("log2_tile_cols", C.c_uint8), ("log2_tile_rows", C.c_uint8),
+        ("uncompressed_header_bytes", C.c_uint32), ("first_partition_size", C.c_uint16),
+        ("base_qindex", C.c_int16), ("y_dc_delta_q", C.c_int8),
+        ("uv_dc_delta_q", C.c_int8), ("uv_ac_delta_q", C.c_int8),
+        ("filter_level", C.c_uint8), ("sharpness_level", C.c_uint8),
+        ("refresh_frame_flags", C.c_uint8), ("ref_frame_idx", C.c_uint8 * 3),
+        ("ref_frame_sign_bias", C.c_uint8 * 4), ("segmentation_enabled", C.c_bool),
+    ]
+
+
+class BlockInfo(C.Structure):
+    """vp9_macroblock_info_t (src/decoder/vp9_parsed_frame.h:20-33), 32 bytes."""
+    _fields_ = [
+        ("x", C.c_uint16), ("y", C.c_uint16), ("width", C.c_uint8),
+        ("height", C.c_uint8), ("tx_size", C.c_uint8), ("is_intra", C.c_uint8),
+        ("y_mode", C.c_uint8), ("uv_mode", C.c_uint8), ("ref_frame", C.c_uint8 * 2),
+        ("mv", (C.c_int16 * 2) * 2), ("skip", C.c_uint8),
+        ("coeff_offset", C.c_uint32), ("eob", C.c_uint16),
+    ]
+
+
+BLOCK_DTYPE = np.dtype({
+    "names": ["x", "y", "width", "height", "tx_size", "is_intra", "y_mode", "uv_mode",
+              "ref_frame", "mv", "skip", "coeff_offset", "eob"],
+    "formats": ["<u2", "<u2", "u1", "u1", "u1", "u1", "u1", "u1", ("u1", 2),
+                ("<i2", (2, 2)), "u1", "<u4", "<u2"],
+    "offsets": [0, 2, 4, 5, 6, 7, 8, 9, 10, 12, 20, 24, 28],
+    "itemsize": 32,
+})
+
+
+class ParsedFrame(C.Structure):
+    """vp9_parsed_frame_t (src/decoder/vp9_parsed_frame.h:36-63), 160 bytes."""
+    _fields_ = [
+        ("hdr", FrameHeader), ("blocks", C.POINTER(BlockInfo)),
+        ("num_blocks", C.c_uint32), ("blocks_capacity", C.c_uint32),
+        ("coeffs", C.POINTER(C.c_int16)), ("num_coeffs", C.c_uint32),
+        ("coeffs_capacity", C.c_uint32), ("mv_grid", C.c_void_p),
+        ("mv_grid_width", C.c_uint32), ("mv_grid_height", C.c_uint32),
+        ("mi_width_grid", C.c_void_p), ("mi_height_grid", C.c_void_p),
+        ("mi_grid_width", C.c_uint32), ("mi_grid_height", C.c_uint32),
+        ("mi_block_grid", C.c_void_p),
+    ]
+
+
+class FrameInfo(C.Structure):
+    """cvp9_frame_info_t (include/compute_vp9/decoder.h:47-56)."""
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("stride_y", C.c_uint32),
+        ("stride_uv", C.c_uint32), ("plane_y", C.POINTER(C.c_uint8)),
+        ("plane_u", C.POINTER(C.c_uint8)), ("plane_v", C.POINTER(C.c_uint8)),
+        ("pts", C.c_int64),
+    ]
+
+
+class DmabufFrame(C.Structure):
+    """cvp9_dmabuf_frame_t (include/compute_vp9/decoder.h:126-134)."""
+    _fields_ = [
+        ("fd", C.c_int), ("size", C.c_uint64), ("width", C.c_uint32),
+        ("height", C.c_uint32), ("offsets", C.c_uint32 * 3),
+        ("pitches", C.c_uint32 * 3), ("pts", C.c_int64),
+    ]
+
+
+class ExportBuffer(C.Structure):
+    """cvp9_export_buffer_t (include/compute_vp9/decoder.h:152-158)."""
+    _fields_ = [
+        ("fd", C.c_int), ("mapped", C.c_void_p), ("size", C.c_uint64),
+        ("priv_buf", C.c_void_p), ("priv_mem", C.c_void_p),
+    ]
+
+
+class SynthParams(C.Structure):
+    """cvp9_synth_params_t (include/cvp9_hip/synth.h)."""
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("seed", C.c_uint64),
+        ("split_permille", C.c_uint32), ("intra_permille", C.c_uint32),
+        ("skip_permille", C.c_uint32), ("tx32_permille", C.c_uint32),
+        ("coef_fixed", C.c_uint32), ("coef_permille", C.c_uint32),
+        ("lap_scale", C.c_uint32), ("mv_range", C.c_int32),
+        ("filter_level", C.c_int32), ("sharpness", C.c_uint32),
+        ("key_interval", C.c_uint32), ("qindex_lo", C.c_int32), ("qindex_hi", C.c_int32),
+    ]
+
+
+PF_P = C.POINTER(ParsedFrame)
+
+# ---- exported symbols, as declared in include/cvp9_hip/*.h ----------------
+SIGNATURES = {
+    # hip_backend.h
+    "hip_backend_init": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "hip_backend_destroy": (None, [C.c_void_p]),
+    "hip_decode_frame": (C.c_int, [C.c_void_p, PF_P, C.c_int64]),
+    "hip_get_frame": (C.c_int, [C.c_void_p, C.POINTER(FrameInfo), C.c_int]),
+    "hip_get_frame_dmabuf": (C.c_int, [C.c_void_p, C.POINTER(DmabufFrame)]),
+    "hip_export_buffer_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(ExportBuffer)]),
+    "hip_export_buffer_free": (None, [C.c_void_p, C.POINTER(ExportBuffer)]),
+    "hip_set_render_target": (C.c_int, [C.c_void_p, C.POINTER(ExportBuffer)]),
+    # synth.h
+    "cvp9_synth_preset": (C.c_int, [C.POINTER(SynthParams), C.c_int, C.c_uint32,
+                                    C.c_uint32, C.c_uint64]),
+    "cvp9_synth_frame": (PF_P, [C.POINTER(SynthParams), C.c_uint32]),
+    "cvp9_synth_free": (None, [PF_P]),
+    "cvp9_synth_algo_bytes": (C.c_uint64, [PF_P, C.c_int]),
+    # testing.h
+    "cvp9_hip_version": (C.c_char_p, []),
+    "cvp9_hip_device_count": (C.c_int, []),
+    "cvp9_hip_idct_table": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+    "cvp9_hip_plan_check": (C.c_int, [PF_P]),
+    "hip_backend_set_debug": (C.c_int, [C.c_void_p, C.c_uint32]),
+}
+
+_lib = None
+
+
+def lib():
+    """The native library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code, what):
+    if code != CVP9_OK:
+        raise Cvp9Error(code, what)
+
+
+# ---- generator -------------------------------------------------------------
+def synth_params(preset, width, height, seed, **over):
+    p = SynthParams()
+    if lib().cvp9_synth_preset(C.byref(p), preset, width, height, seed) != 0:
+        raise ValueError(f"unknown preset {preset}")
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+class SynthFrame:
+    """Owned vp9_parsed_frame_t from the generator."""
+
+    def __init__(self, params, index):
+        self.ptr = lib().cvp9_synth_frame(C.byref(params), index)
+        if not self.ptr:
+            raise MemoryError("cvp9_synth_frame failed")
+
+    @property
+    def pf(self):
+        return self.ptr.contents
+
+    def blocks(self):
+        n = self.pf.num_blocks
+        if n == 0:
+            return np.zeros(0, BLOCK_DTYPE)
+        buf = (C.c_uint8 * (32 * n)).from_address(C.addressof(self.pf.blocks.contents))
+        return np.frombuffer(buf, BLOCK_DTYPE).copy()
+
+    def coeffs(self):
+        n = self.pf.num_coeffs
+        if n == 0:
+            return np.zeros(0, np.int16)
+        return np.ctypeslib.as_array(self.pf.coeffs, shape=(n,)).copy()
+
+    def algo_bytes(self, nv12=False):
+        return int(lib().cvp9_synth_algo_bytes(self.ptr, 1 if nv12 else 0))
+
+    def close(self):
+        if self.ptr:
+            lib().cvp9_synth_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+class OwnedFrame:
+    """A parsed frame built from numpy arrays (hand-made test cases)."""
+
+    def __init__(self, width, height, blocks, coeffs, frame_type=VP9_FRAME_NON_KEY,
+                 filter_level=0, sharpness=0):
+        self._blocks = np.ascontiguousarray(blocks, dtype=BLOCK_DTYPE)
+        self._coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
+        if self._coeffs.size == 0:
+            self._coeffs = np.zeros(1, np.int16)
+            ncoef = 0
+        else:
+            ncoef = self._coeffs.size
+        f = ParsedFrame()
+        f.hdr.frame_type = frame_type
+        f.hdr.width = width
+        f.hdr.height = height
+        f.hdr.filter_level = filter_level
+        f.hdr.sharpness_level = sharpness
+        f.blocks = self._blocks.ctypes.data_as(C.POINTER(BlockInfo))
+        f.num_blocks = f.blocks_capacity = self._blocks.size
+        f.coeffs = self._coeffs.ctypes.data_as(C.POINTER(C.c_int16))
+        f.num_coeffs = ncoef
+        f.coeffs_capacity = self._coeffs.size
+        self._f = f
+        self.ptr = C.pointer(f)
+
+    @property
+    def pf(self):
+        return self._f
+
+
+def frame_info_to_i420(info):
+    """Packed I420 bytes of a cvp9_frame_info_t (honours the strides)."""
+    w, h = info.width, info.height
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+
+    def plane(p, stride, pw, ph):
+        a = np.ctypeslib.as_array(p, shape=(stride * (ph - 1) + pw,))
+        out = np.empty((ph, pw), np.uint8)
+        for r in range(ph):
+            out[r] = a[r * stride:r * stride + pw]
+        return out
+
+    y = plane(info.plane_y, info.stride_y, w, h)
+    u = plane(info.plane_u, info.stride_uv, cw, ch)
+    v = plane(info.plane_v, info.stride_uv, cw, ch)
+    return np.concatenate([y.ravel(), u.ravel(), v.ravel()])
+
+
+def i420_size(w, h):
+    return w * h + 2 * ((w + 1) // 2) * ((h + 1) // 2)
+
+
+# ---- HIP backend (include/cvp9_hip/hip_backend.h) ----------------------------
+class HipBackend:
+    """Thin mirror of the eight hip_* entry points."""
+
+    def __init__(self):
+        ctx = C.c_void_p()
+        check(lib().hip_backend_init(C.byref(ctx)), "hip_backend_init")
+        self.ctx = ctx
+
+    def decode(self, frame, pts=0):
+        """Returns the cvp9_err_t code (AGAIN/UNSUPPORTED are not raised)."""
+        return lib().hip_decode_frame(self.ctx, frame.ptr, pts)
+
+    def get_frame(self, wait=True):
+        """(code, packed I420 numpy array or None, FrameInfo)."""
+        info = FrameInfo()
+        rc = lib().hip_get_frame(self.ctx, C.byref(info), 1 if wait else 0)
+        if rc != CVP9_OK:
+            return rc, None, info
+        return rc, frame_info_to_i420(info), info
+
+    def set_debug(self, flags):
+        check(lib().hip_backend_set_debug(self.ctx, flags), "hip_backend_set_debug")
+
+    def close(self):
+        if self.ctx:
+            lib().hip_backend_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

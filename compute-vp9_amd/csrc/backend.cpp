@@ -26,7 +26,7 @@ using namespace cvp9;
 
 namespace {
 
-constexpr uint32_t kSpinLimit = 1u << 22;   /* ~0.3 s of s_sleep per wait */
+constexpr uint32_t kSpinLimit = 1u << 21;   /* ~0.1 s of s_sleep per wait */
 
 struct Slot {
     hipEvent_t done = nullptr;

@@ -381,6 +381,8 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
                 else { cx = (x - 1) >> 2; cy = (y - 1) >> 2; }
                 const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
                 if (o != CVP9_OWNER_NONE && o < b && (J.blocks[o].flags & DB_INTRA)) {
+                    /* bounded: after spin_limit polls, or once any wait of
+                     * this stream has timed out, give up and flag the error */
                     uint32_t spins = 0;
                     while (__hip_atomic_load(&J.ready[o], __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT) != J.epoch) {
@@ -389,6 +391,9 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
                             atomicOr(J.err, 1u);
                             break;
                         }
+                        if ((spins & 255) == 0 &&
+                            __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                            break;
                     }
                 }
             }
