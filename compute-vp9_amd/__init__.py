@@ -217,7 +217,8 @@ class SynthFrame:
         if n == 0:
             return np.zeros(0, BLOCK_DTYPE)
         buf = (C.c_uint8 * (32 * n)).from_address(C.addressof(self.pf.blocks.contents))
-        return np.frombuffer(buf, BLOCK_DTYPE).copy()
+        # copy the raw bytes (a structured-array copy drops padding bytes)
+        return np.frombuffer(bytearray(buf), BLOCK_DTYPE)
 
     def coeffs(self):
         n = self.pf.num_coeffs

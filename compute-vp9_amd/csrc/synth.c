@@ -15,6 +15,7 @@
  */
 #include "cvp9_hip/synth.h"
 
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -171,7 +172,14 @@ static int push_block(gen_t *g, vp9_macroblock_info_t *b)
         pf->blocks = (vp9_macroblock_info_t *)nb;
         pf->blocks_capacity = cap;
     }
-    pf->blocks[pf->num_blocks++] = *b;
+    /* member stores leave padding unspecified (C11 6.2.6.1p6): copy, then
+     * clear the padding bytes so frames are byte-identical run to run */
+    unsigned char *d = (unsigned char *)&pf->blocks[pf->num_blocks++];
+    memcpy(d, b, sizeof(*b));
+    memset(d + offsetof(vp9_macroblock_info_t, skip) + 1, 0,
+           offsetof(vp9_macroblock_info_t, coeff_offset) - offsetof(vp9_macroblock_info_t, skip) - 1);
+    memset(d + offsetof(vp9_macroblock_info_t, eob) + 2, 0,
+           sizeof(*b) - offsetof(vp9_macroblock_info_t, eob) - 2);
     return 0;
 }
 

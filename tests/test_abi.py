@@ -1,0 +1,211 @@
+"""CPU checks of the C ABI: the library loads and exports every declared
+entry point, struct layouts match the reference's, host validation sends
+unsupported layouts to the CPU path, and the generator is deterministic and
+well-formed. No GPU compute is called here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include", "cvp9_hip")
+REFERENCE = "/root/reference"
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        text = open(os.path.join(INCLUDE, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(\w+)\s*\([^;{]*\)\s*;", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    names = declared_functions()
+    assert {"hip_backend_init", "hip_decode_frame", "hip_get_frame", "hip_get_frame_dmabuf",
+            "hip_export_buffer_alloc", "hip_export_buffer_free", "hip_set_render_target",
+            "hip_backend_destroy"} <= names
+    L = pkg.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python mirror binds all of them
+    assert names <= set(pkg.SIGNATURES), sorted(names - set(pkg.SIGNATURES))
+
+
+def test_struct_sizes(pkg):
+    assert C.sizeof(pkg.FrameHeader) == 76
+    assert C.sizeof(pkg.BlockInfo) == 32
+    assert C.sizeof(pkg.ParsedFrame) == 160
+    assert C.sizeof(pkg.FrameInfo) == 48
+    assert C.sizeof(pkg.DmabufFrame) == 56
+    assert C.sizeof(pkg.ExportBuffer) == 40
+    assert pkg.FrameHeader.filter_level.offset == 63
+    assert pkg.ParsedFrame.coeffs.offset == 96
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include HDR1
+#include HDR2
+#define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f))
+int main(void) {
+  printf("sizes %zu %zu %zu %zu %zu %zu\n", sizeof(vp9_frame_header_t),
+         sizeof(vp9_macroblock_info_t), sizeof(vp9_parsed_frame_t),
+         sizeof(cvp9_frame_info_t), sizeof(cvp9_dmabuf_frame_t), sizeof(cvp9_export_buffer_t));
+  P(vp9_frame_header_t, frame_type); P(vp9_frame_header_t, width); P(vp9_frame_header_t, height);
+  P(vp9_frame_header_t, base_qindex); P(vp9_frame_header_t, filter_level);
+  P(vp9_frame_header_t, sharpness_level); P(vp9_frame_header_t, refresh_frame_flags);
+  P(vp9_macroblock_info_t, x); P(vp9_macroblock_info_t, tx_size); P(vp9_macroblock_info_t, mv);
+  P(vp9_macroblock_info_t, skip); P(vp9_macroblock_info_t, coeff_offset);
+  P(vp9_parsed_frame_t, blocks); P(vp9_parsed_frame_t, num_blocks);
+  P(vp9_parsed_frame_t, coeffs); P(vp9_parsed_frame_t, num_coeffs);
+  P(cvp9_dmabuf_frame_t, offsets); P(cvp9_dmabuf_frame_t, pts);
+  P(cvp9_export_buffer_t, priv_mem); P(cvp9_frame_info_t, pts);
+  return 0; }
+"""
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "src", "decoder")),
+                    reason="reference headers not mounted")
+def test_layouts_match_reference_headers(tmp_path):
+    src = tmp_path / "probe.c"
+    src.write_text(PROBE)
+    outs = []
+    for h1, h2, inc in [
+        ('"compute_vp9/decoder.h"', '"decoder/vp9_parsed_frame.h"',
+         [f"-I{REFERENCE}/include", f"-I{REFERENCE}/src"]),
+        ('"cvp9_hip/abi_types.h"', '"cvp9_hip/hip_backend.h"', [f"-I{ROOT}/include"]),
+    ]:
+        exe = tmp_path / f"probe{len(outs)}"
+        subprocess.run(["gcc", "-std=c11", f"-DHDR1={h1}", f"-DHDR2={h2}", *inc, str(src),
+                        "-o", str(exe)], check=True)
+        outs.append(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    assert outs[0] == outs[1]
+
+
+# ---- host validation (what goes to the CPU path) ------------------------------
+
+def blocks(pkg, specs):
+    b = np.zeros(len(specs), pkg.BLOCK_DTYPE)
+    for i, s in enumerate(specs):
+        for k, v in s.items():
+            b[i][k] = v
+    return b
+
+
+def plan(pkg, frame):
+    return pkg.lib().cvp9_hip_plan_check(frame.ptr)
+
+
+def test_plan_accepts_generated_frames(pkg):
+    for preset, w, h in [(0, 1280, 720), (1, 854, 480), (2, 642, 362), (1, 64, 64)]:
+        p = pkg.synth_params(preset, w, h, 3)
+        for i in range(2):
+            assert plan(pkg, pkg.SynthFrame(p, i)) == pkg.CVP9_OK
+
+
+def test_plan_rejections(pkg):
+    ok = dict(x=0, y=0, width=16, height=16, tx_size=1, is_intra=1)
+    coef = np.zeros(256)
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [ok]), coef)
+    assert plan(pkg, f) == pkg.CVP9_OK
+    # off the 4x4 grid -> CPU path
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [dict(ok, x=2, width=8, height=8)]), coef)
+    assert plan(pkg, f) == pkg.CVP9_ERR_UNSUPPORTED
+    # overlapping blocks -> CPU path
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [dict(ok, width=8, height=8, skip=1),
+                                            dict(ok, x=4, y=4, width=8, height=8, skip=1)]), coef)
+    assert plan(pkg, f) == pkg.CVP9_ERR_UNSUPPORTED
+    # transform size class above 32x32 -> CPU path
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [dict(ok, tx_size=4)]), coef)
+    assert plan(pkg, f) == pkg.CVP9_ERR_UNSUPPORTED
+    # ... unless the block has no residual
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [dict(ok, tx_size=4, skip=1)]), coef)
+    assert plan(pkg, f) == pkg.CVP9_OK
+    # coefficients referenced past the pool -> invalid data
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [dict(ok, coeff_offset=200)]), coef)
+    assert plan(pkg, f) == pkg.CVP9_ERR_INVALID_DATA
+    # W or H = 1 mod 8: the CPU deblock reads past the plane -> CPU path
+    f = pkg.OwnedFrame(17, 16, blocks(pkg, [ok]), coef)
+    assert plan(pkg, f) == pkg.CVP9_ERR_UNSUPPORTED
+    f = pkg.OwnedFrame(16, 9, blocks(pkg, [ok]), coef)
+    assert plan(pkg, f) == pkg.CVP9_ERR_UNSUPPORTED
+    # blocks wholly outside the frame are ignored, like the reference
+    f = pkg.OwnedFrame(16, 16, blocks(pkg, [ok, dict(ok, x=3, y=40)]), coef)
+    assert plan(pkg, f) == pkg.CVP9_OK
+    assert pkg.lib().cvp9_hip_plan_check(None) == pkg.CVP9_ERR_INVALID_DATA
+
+
+def test_backend_without_gpu_reports_no_backend(pkg):
+    if pkg.lib().cvp9_hip_device_count() > 0:
+        pytest.skip("a GPU is present")
+    ctx = C.c_void_p()
+    assert pkg.lib().hip_backend_init(C.byref(ctx)) == pkg.CVP9_ERR_NO_BACKEND
+    assert not ctx.value
+
+
+# ---- generator -------------------------------------------------------------------
+
+def test_generator_deterministic(pkg):
+    p = pkg.synth_params(1, 640, 360, 42)
+    a, b = pkg.SynthFrame(p, 3), pkg.SynthFrame(p, 3)
+    assert a.blocks().tobytes() == b.blocks().tobytes()
+    assert np.array_equal(a.coeffs(), b.coeffs())
+    c = pkg.SynthFrame(pkg.synth_params(1, 640, 360, 43), 3)
+    assert a.blocks().tobytes() != c.blocks().tobytes()
+
+
+@pytest.mark.parametrize("preset,w,h", [(0, 1280, 720), (1, 854, 480), (2, 1000, 562)])
+def test_generator_layout(pkg, preset, w, h):
+    p = pkg.synth_params(preset, w, h, 9)
+    for idx in range(2):
+        f = pkg.SynthFrame(p, idx)
+        b = f.blocks()
+        owner = -np.ones(((h + 3) // 4, (w + 3) // 4), np.int64)
+        ncoef = 0
+        for i, k in enumerate(b):
+            x, y, s = int(k["x"]), int(k["y"]), int(k["width"])
+            assert x % 8 == 0 and y % 8 == 0 and s in (8, 16, 32, 64)
+            assert x < w and y < h
+            bw, bh = min(s, w - x), min(s, h - y)
+            cells = owner[y // 4:(y + bh + 3) // 4, x // 4:(x + bw + 3) // 4]
+            assert (cells == -1).all(), "overlap"
+            cells[:] = i
+            # Z order: the block above-left, above and left were emitted earlier
+            if y > 0:
+                assert owner[(y - 1) // 4, x // 4] < i
+            if x > 0:
+                assert owner[y // 4, (x - 1) // 4] < i
+            T = 4 << int(k["tx_size"])
+            assert T <= s
+            if idx == 0:
+                assert k["is_intra"] == 1
+            if not k["skip"]:
+                assert k["coeff_offset"] == ncoef
+                ncoef += (bw // T) * (bh // T) * T * T
+        assert (owner >= 0).all(), "frame not covered"
+        assert f.pf.num_coeffs == ncoef
+        assert f.pf.hdr.frame_type == (0 if idx == 0 else 1)
+
+
+def test_generator_coefficient_density(pkg):
+    # preset 1: 25% of each transform nonzero at the lowest zigzag positions
+    f = pkg.SynthFrame(pkg.synth_params(1, 256, 256, 5), 1)
+    c = f.coeffs()
+    frac = np.count_nonzero(c) / c.size
+    assert 0.22 < frac <= 0.25
+    # preset 0: about 6 nonzero per transform, at zigzag 0..5
+    f = pkg.SynthFrame(pkg.synth_params(0, 256, 256, 5), 1)
+    for k in f.blocks():
+        T = 4 << int(k["tx_size"])
+        x, y, s = int(k["x"]), int(k["y"]), int(k["width"])
+        blk = f.coeffs()[k["coeff_offset"]:k["coeff_offset"] + T * T].reshape(T, T)
+        nz = np.argwhere(blk != 0)
+        assert len(nz) <= 6 and all(r + c <= 2 for r, c in nz)
+        break
