@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X VP9 reconstruction backend.
+
+Default run (N=1):
+  * single stream, 1920x1080 preset 1 (25% dense coefficients, mixed 4..32
+    transforms, 30% intra blocks on inter frames), a 100-frame sequence
+    prepared in HBM; a step reconstructs one frame through the plain
+    per-stage kernels. `value` = frames/s over exactly K timed steps.
+  * 64-stream mix (16 each of 640x360, 854x480, 1280x720, 1920x1080), one
+    batch of 64 frames per step, one launch per stage: aggregate frames/s
+    and 1080p-equivalent frames/s (sum of luma pixels / 1920*1080).
+  * roofline of the dominant kernel from HIP events on the engine's stream
+    over the timed region, against MI355X HBM peak.
+  * cpu_baseline: the reference's own CPU reconstruction (oracle/_ref, one
+    thread) on a bounded sample of the same 1080p sequence.
+Prints ONE JSON line.
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+from _pkgload import load_package  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TOPS = 78.6   # MI355X vector FP64 (FMA counted as 2); mul+add separately: half
+METRIC = "recon frames/s @1080p (1 vs 64 streams), p50/p99 frame latency, HBM GB/s"
+MIX = [(640, 360), (854, 480), (1280, 720), (1920, 1080)]
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def gen_frames(P, specs, workers=16):
+    """specs: list of (preset, w, h, seed, index). Generated in parallel
+    threads (ctypes releases the GIL)."""
+    def one(s):
+        preset, w, h, seed, idx = s
+        return P.SynthFrame(P.synth_params(preset, w, h, seed), idx)
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, specs))
+
+
+def stage_bytes(P, frame):
+    """Algorithmic HBM bytes per stage for one frame (numpy over blocks)."""
+    b = frame.blocks()
+    W, H = frame.pf.hdr.width, frame.pf.hdr.height
+    x = b["x"].astype(np.int64)
+    y = b["y"].astype(np.int64)
+    bw = np.minimum(b["width"].astype(np.int64), W - x)
+    bh = np.minimum(b["height"].astype(np.int64), H - y)
+    ok = (bw > 0) & (bh > 0)
+    intra = (b["is_intra"] != 0) & ok
+    inter = (b["is_intra"] == 0) & ok
+    T = 4 << b["tx_size"].astype(np.int64)
+    area = np.where(ok & (b["skip"] == 0), (bw // T) * (bh // T) * T * T, 0)
+    cbw, cbh = (bw + 1) // 2, (bh + 1) // 2
+    out = {}
+    out["mc"] = int(((bw + 1) * (bh + 1) + 2 * (cbw + 1) * (cbh + 1) + bw * bh + 2 * cbw * cbh)[inter].sum())
+    for k, n in enumerate((4, 8, 16, 32)):
+        sel = inter & (b["tx_size"] == k)
+        out[f"residual{n}"] = int(4 * area[sel].sum())
+    out["intra"] = int((bw * bh + 2 * cbw * cbh)[intra].sum() + 4 * area[intra].sum())
+    out["deblock"] = 2 * W * H if frame.pf.hdr.filter_level else 0
+    out["fill"] = 0
+    out["owner"] = int((32 + ((bw + 3) // 4) * ((bh + 3) // 4) * 4)[ok].sum())
+    out["nv12"] = 0
+    # FP64 operations of the IDCT (multiply + add per MAC, dense bound)
+    out["fp64_ops"] = {n: int(2 * 2 * n * area[inter & (b["tx_size"] == k)].sum())
+                       for k, n in enumerate((4, 8, 16, 32))}
+    return out
+
+
+def single_stream(P, args):
+    W, H = 1920, 1080
+    F = args.frames
+    log(f"generating {F} frames {W}x{H} preset 1")
+    frames = gen_frames(P, [(1, W, H, 20250, i) for i in range(F)])
+    eng = P.Engine(0)
+    preps = [eng.prepare(f) for f in frames]
+    sid = eng.open_stream()
+    order = [(args.warmup + k) % F for k in range(args.steps)]
+    for k in range(args.warmup):
+        eng.run([sid], [preps[k % F]])
+    eng.sync()
+    # untimed-events pass of the same steps: the clean throughput figure
+    eng.set_timing(False)
+    t0 = time.perf_counter()
+    for i in order:
+        eng.run([sid], [preps[i]])
+    eng.sync()
+    clean_s = time.perf_counter() - t0
+    # timed region with per-stage HIP events on the engine's stream
+    for k in range(args.warmup):
+        eng.run([sid], [preps[k % F]])
+    eng.sync()
+    eng.stage_stats(reset=True)
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    for i in order:
+        eng.run([sid], [preps[i]])
+    eng.sync()
+    timed_s = time.perf_counter() - t0
+    stats = eng.stage_stats(reset=True)
+    eng.set_timing(False)
+    per_stage = {}
+    sb = [stage_bytes(P, frames[i]) for i in range(F)]
+    for name, (ms, n) in stats.items():
+        if n == 0:
+            continue
+        tot_bytes = sum(sb[i].get(name, 0) for i in order)
+        per_stage[name] = {"ms_total": round(ms, 4), "launches": n,
+                           "us_per_launch": round(1e3 * ms / n, 3),
+                           "algo_GBps": round(tot_bytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+        if name.startswith("residual"):
+            N = int(name[8:])
+            ops = sum(sb[i]["fp64_ops"][N] for i in order)
+            per_stage[name]["fp64_dense_TOPs"] = round(ops / (ms * 1e-3) / 1e12, 2)
+    algo = sum(frames[i].algo_bytes() for i in order)
+    eng.close()
+    return {"clean_s": clean_s, "timed_s": timed_s, "stats": per_stage, "algo_bytes": algo,
+            "frames": frames}
+
+
+def multi_stream(P, args):
+    per_res = args.streams // 4
+    F = args.mframes
+    specs, layout = [], []
+    for r, (w, h) in enumerate(MIX):
+        for s in range(per_res):
+            sid = r * per_res + s
+            layout.append((w, h))
+            for i in range(F):
+                specs.append((3, w, h, 7000 + sid, i))
+    log(f"generating {len(specs)} frames for {len(layout)} streams")
+    frames = gen_frames(P, specs)
+    eng = P.Engine(0)
+    preps = [eng.prepare(f) for f in frames]
+    del frames
+    sids = [eng.open_stream() for _ in layout]
+    steps = args.msteps
+
+    def batch(k):
+        idx = k if k < F else 1 + (k - 1) % (F - 1)   # frame 0 (KEY) once, then cycle
+        eng.run(sids, [preps[s * F + idx] for s in range(len(sids))])
+
+    for k in range(args.mwarmup):
+        batch(k)
+    eng.sync()
+    t0 = time.perf_counter()
+    for k in range(args.mwarmup, args.mwarmup + steps):
+        batch(k)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    px = sum(w * h for w, h in layout) * steps
+    eng.close()
+    return {"streams": len(layout), "steps": steps, "seconds": dt,
+            "frames_per_s": len(layout) * steps / dt,
+            "p1080_equiv_per_s": px / (1920 * 1080) / dt,
+            "ms_per_batch": 1e3 * dt / steps}
+
+
+def cpu_baseline(P, frames, n):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    sample = frames[:n]
+    kind = "reference" if pyoracle.have_ref() else "port"
+    run = pyoracle.run_reference if kind == "reference" else pyoracle.run_oracle
+    t0 = time.perf_counter()
+    run(sample)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(sample) / dt, 3), "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": f"first {len(sample)} frames of the 1920x1080 preset-1 sequence, "
+                      f"1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--frames", type=int, default=100)
+    ap.add_argument("--streams", type=int, default=64)
+    ap.add_argument("--mframes", type=int, default=60)
+    ap.add_argument("--msteps", type=int, default=50)
+    ap.add_argument("--mwarmup", type=int, default=5)
+    ap.add_argument("--cpu-frames", type=int, default=100)
+    ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(streams shard across GPUs with no collectives; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    P = load_package()
+    if P.lib().cvp9_hip_device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    ss = single_stream(P, args)
+    value = args.steps / ss["clean_s"]
+    stats = ss["stats"]
+    dom = max(stats, key=lambda k: stats[k]["ms_total"])
+    d = stats[dom]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": d["algo_GBps"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(d["algo_GBps"] / HBM_PEAK_GBS, 4) if d["algo_GBps"] else None,
+            "traffic": None, "us_per_launch": d["us_per_launch"]}
+    if "fp64_dense_TOPs" in d:
+        roof["fp64_dense_TOPs"] = d["fp64_dense_TOPs"]
+        roof["fp64_peak_no_fma_TOPs"] = FP64_PEAK_TOPS / 2
+    log("stages:", json.dumps(stats))
+
+    multi = None if args.no_multi else multi_stream(P, args)
+    if multi:
+        log("multi:", json.dumps(multi))
+    cpu = None if args.no_cpu else cpu_baseline(P, ss["frames"], args.cpu_frames)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * ss["clean_s"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i16 (fp64 transform)",
+        "data": "synthetic parsed frames (seeded generator), inputs resident in HBM",
+        "config": {"workload": "1920x1080 single stream, 100-frame preset-1 sequence, "
+                               "plain per-stage kernels", "frames": args.frames,
+                   "step": "one frame"},
+        "timed_with_events_fps": round(args.steps / ss["timed_s"], 2),
+        "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1),
+        "stages": stats,
+        "roofline": roof,
+        "multi_stream": None if not multi else {
+            "workload": f"{multi['streams']} streams (16 each 640x360/854x480/1280x720/1920x1080), "
+                        "one batch per step",
+            "frames_per_s": round(multi["frames_per_s"], 1),
+            "p1080_equiv_frames_per_s": round(multi["p1080_equiv_per_s"], 1),
+            "ms_per_batch": round(multi["ms_per_batch"], 4),
+            "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

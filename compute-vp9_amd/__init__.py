@@ -159,6 +159,23 @@ SIGNATURES = {
     "cvp9_synth_frame": (PF_P, [C.POINTER(SynthParams), C.c_uint32]),
     "cvp9_synth_free": (None, [PF_P]),
     "cvp9_synth_algo_bytes": (C.c_uint64, [PF_P, C.c_int]),
+    # engine.h
+    "cvp9_engine_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "cvp9_engine_destroy": (None, [C.c_void_p]),
+    "cvp9_engine_stream_open": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "cvp9_engine_stream_close": (None, [C.c_void_p, C.c_int]),
+    "cvp9_engine_stream_bytes": (C.c_uint64, [C.c_void_p, C.c_int]),
+    "cvp9_engine_prepare": (C.c_int, [C.c_void_p, PF_P, C.POINTER(C.c_void_p)]),
+    "cvp9_engine_prep_free": (None, [C.c_void_p, C.c_void_p]),
+    "cvp9_engine_prep_bytes": (C.c_uint64, [C.c_void_p]),
+    "cvp9_engine_run": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "cvp9_engine_sync": (C.c_int, [C.c_void_p]),
+    "cvp9_engine_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint8), C.c_uint64,
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "cvp9_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_stage_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_uint32), C.c_int]),
     # testing.h
     "cvp9_hip_version": (C.c_char_p, []),
     "cvp9_hip_device_count": (C.c_int, []),
@@ -331,3 +348,79 @@ class HipBackend:
             self.close()
         except Exception:
             pass
+
+
+# ---- multi-stream engine (include/cvp9_hip/engine.h) ------------------------------
+STAGE_NAMES = ["fill", "owner", "mc", "residual4", "residual8", "residual16", "residual32",
+               "intra", "deblock", "nv12"]
+
+
+class Engine:
+    """Mirror of the cvp9_engine_* entry points."""
+
+    def __init__(self, device=0):
+        e = C.c_void_p()
+        check(lib().cvp9_engine_create(device, C.byref(e)), "cvp9_engine_create")
+        self.e = e
+        self._preps = []
+
+    def open_stream(self):
+        sid = C.c_int()
+        check(lib().cvp9_engine_stream_open(self.e, C.byref(sid)), "stream_open")
+        return sid.value
+
+    def close_stream(self, sid):
+        lib().cvp9_engine_stream_close(self.e, sid)
+
+    def stream_bytes(self, sid):
+        return int(lib().cvp9_engine_stream_bytes(self.e, sid))
+
+    def prepare(self, frame):
+        p = C.c_void_p()
+        check(lib().cvp9_engine_prepare(self.e, frame.ptr, C.byref(p)), "cvp9_engine_prepare")
+        return p
+
+    def free_prep(self, p):
+        lib().cvp9_engine_prep_free(self.e, p)
+
+    @staticmethod
+    def prep_bytes(p):
+        return int(lib().cvp9_engine_prep_bytes(p))
+
+    def run(self, streams, preps, nv12=None):
+        n = len(streams)
+        s = (C.c_int * n)(*streams)
+        p = (C.c_void_p * n)(*[x.value if isinstance(x, C.c_void_p) else x for x in preps])
+        nv = None if nv12 is None else (C.c_void_p * n)(*nv12)
+        check(lib().cvp9_engine_run(self.e, n, s, p, nv), "cvp9_engine_run")
+
+    def sync(self):
+        check(lib().cvp9_engine_sync(self.e), "cvp9_engine_sync")
+
+    def read(self, sid, w, h):
+        out = np.zeros(i420_size(w, h), np.uint8)
+        ww, hh = C.c_uint32(), C.c_uint32()
+        check(lib().cvp9_engine_read(self.e, sid, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                     out.size, C.byref(ww), C.byref(hh)), "cvp9_engine_read")
+        assert (ww.value, hh.value) == (w, h)
+        return out
+
+    def set_timing(self, on):
+        check(lib().cvp9_engine_set_timing(self.e, 1 if on else 0), "set_timing")
+
+    def stage_stats(self, reset=True):
+        ms = (C.c_double * 10)()
+        n = (C.c_uint32 * 10)()
+        check(lib().cvp9_engine_stage_stats(self.e, ms, n, 1 if reset else 0), "stage_stats")
+        return {STAGE_NAMES[i]: (ms[i], n[i]) for i in range(10)}
+
+    def close(self):
+        if self.e:
+            lib().cvp9_engine_destroy(self.e)
+            self.e = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,378 @@
+/*
+ * engine_api.cpp — multi-stream engine C ABI (cvp9_hip/engine.h).
+ */
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "cvp9_hip/engine.h"
+#include "engine.hpp"
+
+using namespace cvp9;
+
+struct cvp9_prep {
+    PrepLayout L;
+    uint8_t *dev = nullptr;
+    uint64_t algo = 0;
+};
+
+namespace {
+
+constexpr int kBatchSlots = 4;
+constexpr uint32_t kSpinLimit = 1u << 21;
+
+struct BatchSlot {
+    uint8_t *host = nullptr, *dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool used = false;
+};
+
+struct TimedLaunch {
+    int stage;
+    hipEvent_t a, b;
+};
+
+bool stage_has_work(int st, const StageCounts &n)
+{
+    switch (st) {
+    case STAGE_FILL: return n.fill != 0;
+    case STAGE_OWNER: return true;
+    case STAGE_MC: return n.inter != 0;
+    case STAGE_RES4: return n.grp[0] != 0;
+    case STAGE_RES8: return n.grp[1] != 0;
+    case STAGE_RES16: return n.grp[2] != 0;
+    case STAGE_RES32: return n.grp[3] != 0;
+    case STAGE_INTRA: return n.intra != 0;
+    case STAGE_LF: return n.lf != 0;
+    case STAGE_NV12: return n.nv12 != 0;
+    }
+    return false;
+}
+
+cvp9_err_t to_err(hipError_t e)
+{
+    if (e == hipSuccess) return CVP9_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CVP9_ERR_NOMEM;
+    return CVP9_ERR_GPU;
+}
+
+uint64_t prep_algo_bytes(const vp9_parsed_frame_t *pf)
+{
+    /* coefficients of transforms taken + 32-byte block records + reference
+     * footprints of inter blocks + I420 write */
+    const int W = (int)pf->hdr.width, H = (int)pf->hdr.height;
+    uint64_t b = 0;
+    for (uint32_t i = 0; i < pf->num_blocks; i++) {
+        const vp9_macroblock_info_t &k = pf->blocks[i];
+        const int bw = k.x + k.width > W ? W - k.x : k.width;
+        const int bh = k.y + k.height > H ? H - k.y : k.height;
+        b += 32;
+        if (bw <= 0 || bh <= 0) continue;
+        if (!k.skip && k.tx_size <= 3) {
+            const int T = 4 << k.tx_size;
+            b += 2ull * (uint64_t)((bw / T) * (bh / T) * T * T);
+        }
+        if (!k.is_intra) {
+            const int cbw = (bw + 1) / 2, cbh = (bh + 1) / 2;
+            b += (uint64_t)((bw + 1) * (bh + 1) + 2 * (cbw + 1) * (cbh + 1));
+        }
+    }
+    return b + (uint64_t)W * H + 2ull * ((W + 1) / 2) * ((H + 1) / 2);
+}
+
+} // namespace
+
+struct cvp9_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<StreamState *> streams;
+    BatchSlot slots[kBatchSlots];
+    int next = 0;
+    uint32_t intra_wgs = 512;
+    bool timing = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> free_events;
+    double stage_ms[STAGE_COUNT] = {0};
+    uint32_t stage_n[STAGE_COUNT] = {0};
+    std::vector<uint32_t *> err_words;
+
+    hipEvent_t take_event()
+    {
+        if (!free_events.empty()) {
+            hipEvent_t e = free_events.back();
+            free_events.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+
+    void resolve_timing()
+    {
+        for (TimedLaunch &t : timed) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+                stage_ms[t.stage] += ms;
+                stage_n[t.stage]++;
+            }
+            free_events.push_back(t.a);
+            free_events.push_back(t.b);
+        }
+        timed.clear();
+    }
+};
+
+extern "C" {
+
+cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
+{
+    if (!out) return CVP9_ERR_INVALID_DATA;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return CVP9_ERR_NO_BACKEND;
+    if (init_device(device) != hipSuccess) return CVP9_ERR_NO_BACKEND;
+    cvp9_engine_t *e = new (std::nothrow) cvp9_engine();
+    if (!e) return CVP9_ERR_NOMEM;
+    e->device = device;
+    hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        e->intra_wgs = 2u * (uint32_t)prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return CVP9_ERR_GPU;
+    }
+    for (BatchSlot &s : e->slots) {
+        if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+            cvp9_engine_destroy(e);
+            return CVP9_ERR_GPU;
+        }
+    }
+    *out = e;
+    return CVP9_OK;
+}
+
+void cvp9_engine_destroy(cvp9_engine_t *e)
+{
+    if (!e) return;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    for (StreamState *s : e->streams) delete s;
+    for (BatchSlot &s : e->slots) {
+        if (s.done) hipEventDestroy(s.done);
+        if (s.host) hipHostFree(s.host);
+        if (s.dev) hipFree(s.dev);
+    }
+    e->resolve_timing();
+    for (hipEvent_t ev : e->free_events) hipEventDestroy(ev);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+cvp9_err_t cvp9_engine_stream_open(cvp9_engine_t *e, int *id)
+{
+    if (!e || !id) return CVP9_ERR_INVALID_DATA;
+    StreamState *s = new (std::nothrow) StreamState(e->device);
+    if (!s) return CVP9_ERR_NOMEM;
+    for (size_t i = 0; i < e->streams.size(); i++)
+        if (!e->streams[i]) {
+            e->streams[i] = s;
+            *id = (int)i;
+            return CVP9_OK;
+        }
+    e->streams.push_back(s);
+    *id = (int)e->streams.size() - 1;
+    return CVP9_OK;
+}
+
+void cvp9_engine_stream_close(cvp9_engine_t *e, int id)
+{
+    if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return;
+    hipSetDevice(e->device);
+    hipStreamSynchronize(e->stream); /* in-flight batches may still read it */
+    delete e->streams[id];
+    e->streams[id] = nullptr;
+}
+
+uint64_t cvp9_engine_stream_bytes(cvp9_engine_t *e, int id)
+{
+    if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return 0;
+    return e->streams[id]->resident_bytes();
+}
+
+cvp9_err_t cvp9_engine_prepare(cvp9_engine_t *e, const vp9_parsed_frame_t *pf, cvp9_prep_t **out)
+{
+    if (!e || !pf || !out) return CVP9_ERR_INVALID_DATA;
+    *out = nullptr;
+    cvp9_prep_t *p = new (std::nothrow) cvp9_prep();
+    if (!p) return CVP9_ERR_NOMEM;
+    cvp9_err_t err = plan_measure(pf, p->L);
+    if (err != CVP9_OK) {
+        delete p;
+        return err;
+    }
+    hipSetDevice(e->device);
+    std::vector<uint8_t> host(p->L.bytes);
+    plan_write(pf, p->L, host.data());
+    hipError_t he = hipMalloc(&p->dev, p->L.bytes);
+    if (he == hipSuccess) he = hipMemcpy(p->dev, host.data(), p->L.bytes, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        if (p->dev) hipFree(p->dev);
+        delete p;
+        return to_err(he);
+    }
+    p->algo = prep_algo_bytes(pf);
+    *out = p;
+    return CVP9_OK;
+}
+
+void cvp9_engine_prep_free(cvp9_engine_t *e, cvp9_prep_t *p)
+{
+    if (!p) return;
+    if (e) {
+        hipSetDevice(e->device);
+        hipStreamSynchronize(e->stream);
+    }
+    hipFree(p->dev);
+    delete p;
+}
+
+uint64_t cvp9_engine_prep_bytes(const cvp9_prep_t *p)
+{
+    return p ? p->algo : 0;
+}
+
+cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *const *preps,
+                           uint8_t *const *nv12)
+{
+    if (!e || n <= 0 || n > CVP9_MAX_BATCH || !streams || !preps) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    std::vector<DJob> jobs((size_t)n);
+    /* one frame per stream per batch: the stream's scratch is per batch */
+    for (int i = 0; i < n; i++) {
+        const int id = streams[i];
+        if (id < 0 || id >= (int)e->streams.size() || !e->streams[id] || !preps[i])
+            return CVP9_ERR_INVALID_DATA;
+        for (int k = 0; k < i; k++)
+            if (streams[k] == id) return CVP9_ERR_INVALID_DATA;
+    }
+    hipError_t he;
+    for (int i = 0; i < n; i++) {
+        StreamState *s = e->streams[streams[i]];
+        const PrepLayout &L = preps[i]->L;
+        if ((he = s->ensure_scratch(L, e->stream)) != hipSuccess) return to_err(he);
+        FrameBuf *out = nullptr;
+        if ((he = s->acquire(L.W, L.H, &out)) != hipSuccess) return to_err(he);
+        jobs[i] = s->make_job(L, preps[i]->dev, out, nv12 ? nv12[i] : nullptr);
+        s->update_refs(L.key != 0, out);
+    }
+
+    BatchSlot &bs = e->slots[e->next];
+    e->next = (e->next + 1) % kBatchSlots;
+    if (bs.used && hipEventSynchronize(bs.done) != hipSuccess) return CVP9_ERR_GPU;
+    const size_t need = BatchBuilder::bytes_for((uint32_t)n);
+    if (bs.cap < need) {
+        if (bs.host) hipHostFree(bs.host);
+        if (bs.dev) hipFree(bs.dev);
+        bs.host = bs.dev = nullptr;
+        bs.cap = 0;
+        if ((he = hipHostMalloc((void **)&bs.host, need, hipHostMallocDefault)) != hipSuccess ||
+            (he = hipMalloc((void **)&bs.dev, need)) != hipSuccess)
+            return to_err(he);
+        bs.cap = need;
+    }
+    BatchBuilder bb;
+    bb.build(jobs.data(), (uint32_t)n, bs.host, bs.dev, e->intra_wgs, kSpinLimit);
+    if ((he = hipMemcpyAsync(bs.dev, bs.host, bb.used(), hipMemcpyHostToDevice, e->stream)) != hipSuccess)
+        return to_err(he);
+    for (int st = 0; st < STAGE_COUNT; st++) {
+        const bool timed = e->timing && stage_has_work(st, bb.counts());
+        TimedLaunch t{st, nullptr, nullptr};
+        if (timed) {
+            t.a = e->take_event();
+            t.b = e->take_event();
+            if (!t.a || !t.b) return CVP9_ERR_GPU;
+            hipEventRecord(t.a, e->stream);
+        }
+        if ((he = cvp9_launch_stage(st, bb.batch(), bb.counts(), e->stream)) != hipSuccess)
+            return to_err(he);
+        if (timed) {
+            hipEventRecord(t.b, e->stream);
+            e->timed.push_back(t);
+        }
+    }
+    if ((he = hipEventRecord(bs.done, e->stream)) != hipSuccess) return to_err(he);
+    bs.used = true;
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e)
+{
+    if (!e) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    e->resolve_timing();
+    for (StreamState *s : e->streams) {
+        if (!s || !s->err_word()) continue;
+        uint32_t v = 0;
+        if (hipMemcpy(&v, s->err_word(), 4, hipMemcpyDeviceToHost) != hipSuccess) return CVP9_ERR_GPU;
+        if (v) return CVP9_ERR_GPU;
+    }
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int id, uint8_t *dst, uint64_t cap, uint32_t *width,
+                            uint32_t *height)
+{
+    if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return CVP9_ERR_INVALID_DATA;
+    FrameBuf *f = e->streams[id]->ref(0);
+    if (!f) return CVP9_ERR_UNSUPPORTED;
+    const uint32_t w = f->w, h = f->h, cw = (w + 1) / 2, ch = (h + 1) / 2;
+    if (width) *width = w;
+    if (height) *height = h;
+    const uint64_t need = (uint64_t)w * h + 2ull * cw * ch;
+    if (!dst || cap < need) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    const DFrame d = f->dframe();
+    if (hipMemcpy2D(dst, w, d.y, d.sy, w, h, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy2D(dst + (size_t)w * h, cw, d.u, d.suv, cw, ch, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy2D(dst + (size_t)w * h + (size_t)cw * ch, cw, d.v, d.suv, cw, ch,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return CVP9_ERR_GPU;
+    uint32_t v = 0;
+    if (e->streams[id]->err_word() &&
+        hipMemcpy(&v, e->streams[id]->err_word(), 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return CVP9_ERR_GPU;
+    return v ? CVP9_ERR_GPU : CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_set_timing(cvp9_engine_t *e, int on)
+{
+    if (!e) return CVP9_ERR_INVALID_DATA;
+    e->timing = on != 0;
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_stage_stats(cvp9_engine_t *e, double *ms, uint32_t *launches, int reset)
+{
+    if (!e) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    e->resolve_timing();
+    for (int i = 0; i < STAGE_COUNT; i++) {
+        if (ms) ms[i] = e->stage_ms[i];
+        if (launches) launches[i] = e->stage_n[i];
+        if (reset) {
+            e->stage_ms[i] = 0;
+            e->stage_n[i] = 0;
+        }
+    }
+    return CVP9_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,72 @@
+/*
+ * cvp9_hip/engine.h — multi-stream reconstruction engine, C ABI.
+ *
+ * No counterpart in the reference, which decodes one stream per context
+ * (src/decoder/vp9_decoder.c). This layer sits beside hip_backend.h for
+ * callers that serve many streams on one GPU: every stream keeps its
+ * reference pool resident in HBM, parsed frames are staged into HBM once
+ * ("prepared"), and a batch of frames from different streams goes through
+ * each reconstruction stage in a single launch. Per-stream output is the
+ * same, bit for bit, as decoding that stream alone through hip_backend.h.
+ */
+#ifndef CVP9_HIP_ENGINE_H
+#define CVP9_HIP_ENGINE_H
+
+#include "cvp9_hip/abi_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cvp9_engine cvp9_engine_t;
+typedef struct cvp9_prep   cvp9_prep_t;
+
+/* Engine on HIP device `device` with its own stream. */
+cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out);
+void       cvp9_engine_destroy(cvp9_engine_t *e);
+
+/* A decode stream: an empty 8-slot reference pool on the device. */
+cvp9_err_t cvp9_engine_stream_open(cvp9_engine_t *e, int *stream_id);
+void       cvp9_engine_stream_close(cvp9_engine_t *e, int stream_id);
+/* HBM held by a stream (frame pool, scratch). */
+uint64_t   cvp9_engine_stream_bytes(cvp9_engine_t *e, int stream_id);
+
+/* Validates pf (same rules and codes as hip_decode_frame) and copies its
+ * device image into HBM. pf is not referenced afterwards. */
+cvp9_err_t cvp9_engine_prepare(cvp9_engine_t *e, const vp9_parsed_frame_t *pf,
+                               cvp9_prep_t **out);
+void       cvp9_engine_prep_free(cvp9_engine_t *e, cvp9_prep_t *p);
+/* HBM bytes of a prepared frame, and its algorithmic bytes (coefficients,
+ * block records, reference footprints and the I420 write). */
+uint64_t   cvp9_engine_prep_bytes(const cvp9_prep_t *p);
+
+/* Enqueues one batch: frame preps[i] on stream streams[i] (each stream at
+ * most once per batch), one launch per stage. Asynchronous; a prep must
+ * stay alive until the batch completes. nv12[i] (or nv12 == NULL) is an
+ * optional device pointer receiving the frame as NV12 (pitch W). */
+cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams,
+                           cvp9_prep_t *const *preps, uint8_t *const *nv12);
+/* Waits for all enqueued batches. CVP9_ERR_GPU if a device error was
+ * flagged for any stream. */
+cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e);
+
+/* Copies the stream's most recent frame (packed I420: Y W*H, U, V
+ * ceil(W/2)*ceil(H/2)) to host memory; synchronises. */
+cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int stream_id, uint8_t *dst, uint64_t cap,
+                            uint32_t *width, uint32_t *height);
+
+/* Per-stage timing with HIP events on the engine's stream. When on, every
+ * stage launch is bracketed by events; cvp9_engine_stage_stats returns, for
+ * each of the CVP9_ENGINE_STAGES stages, the summed milliseconds and the
+ * number of launches since the last reset (it synchronises). Stage order:
+ * fill, owner, mc, residual4, residual8, residual16, residual32, intra,
+ * deblock, nv12. */
+#define CVP9_ENGINE_STAGES 10
+cvp9_err_t cvp9_engine_set_timing(cvp9_engine_t *e, int on);
+cvp9_err_t cvp9_engine_stage_stats(cvp9_engine_t *e, double *ms, uint32_t *launches,
+                                   int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
