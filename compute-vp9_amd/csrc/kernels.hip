@@ -356,11 +356,16 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t total = B.pre_intra[B.njobs];
 
-    for (;;) {
-        if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
-        __syncthreads();
-        const uint32_t tk = s_ticket;
-        if (tk >= total) break;
+    /* The ticket is broadcast through LDS and then made wave-uniform with
+     * readfirstlane, so the loop condition is a scalar branch. With an LDS
+     * value as the exit test the compiler treats the exit as divergent and
+     * restructures the loop so that thread 0's ticket fetch and flag store
+     * run outside the loop holding the barriers: the other lanes then
+     * repeat one ticket forever. */
+    if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
+    __syncthreads();
+    uint32_t tk = __builtin_amdgcn_readfirstlane(s_ticket);
+    while (tk < total) {
         const uint32_t j = find_job(B.pre_intra, B.njobs, tk);
         const DJob &J = B.jobs[j];
         const uint32_t b = J.intra[tk - B.pre_intra[j]];
@@ -470,11 +475,15 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
             }
         }
 
-        /* 5. publish */
+        /* 5. publish, then take the next ticket */
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
-        if (tid == 0)
+        if (tid == 0) {
             __hip_atomic_store(&J.ready[b], J.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            s_ticket = atomicAdd(B.ticket, 1u);
+        }
+        __syncthreads();
+        tk = __builtin_amdgcn_readfirstlane(s_ticket);
     }
 }
 
