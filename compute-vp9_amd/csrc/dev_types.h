@@ -32,6 +32,14 @@
 #define CVP9_OWNER_NONE 0xffffffffu
 #define CVP9_MAX_BATCH 4096
 
+/* Edge record of an intra block: its final bottom row and right column of
+ * Y, U and V, 4 samples per 64-bit word with the frame epoch in the upper 32
+ * bits, so each word carries its own validity. Word ranges:
+ *   Y bottom [0,16)  Y right [16,32)  U bottom [32,40)  U right [40,48)
+ *   V bottom [48,56) V right [56,64)   (intra blocks are at most 64x64) */
+#define CVP9_EDGE_WORDS 64
+#define CVP9_MAX_INTRA_DIM 64
+
 /* block flags */
 #define DB_INTRA   1u
 #define DB_SKIP    2u   /* no residual (skip, or transform grid empty) */
@@ -66,7 +74,8 @@ struct DJob {
     const uint32_t *cls_blk[4];
     const uint32_t *cls_pre[4];
     uint32_t       *owner;         /* ow x oh cells, local block index */
-    uint32_t       *ready;         /* per local block: epoch when final */
+    uint64_t       *edge;          /* per local block: CVP9_EDGE_WORDS edge words */
+    int16_t        *resid;         /* intra residuals, indexed like coeffs */
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
     uint32_t       *err;           /* error word (spin timeout) */
     uint32_t n_blocks, n_inter, n_intra;

@@ -125,6 +125,8 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
         /* blocks on the 4x4 grid: neighbour ownership is exact per cell */
         if ((b.x & 3) || (b.y & 3) || (b.width & 3) || (b.height & 3)) return CVP9_ERR_UNSUPPORTED;
         const bool intra = b.is_intra != 0;
+        /* intra blocks publish their edges in fixed 64-sample records */
+        if (intra && (c.bw > CVP9_MAX_INTRA_DIM || c.bh > CVP9_MAX_INTRA_DIM)) return CVP9_ERR_UNSUPPORTED;
         bool res = !b.skip;
         if (res) {
             if (b.tx_size > 3) return CVP9_ERR_UNSUPPORTED;
@@ -134,10 +136,8 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
             else {
                 if (!pf->coeffs) return CVP9_ERR_INVALID_DATA;
                 if ((uint64_t)b.coeff_offset + ntx * T * T > pf->num_coeffs) return CVP9_ERR_INVALID_DATA;
-                if (!intra) {
-                    L.cls_nblk[b.tx_size]++;
-                    L.cls_ntx[b.tx_size] += (uint32_t)ntx;
-                }
+                L.cls_nblk[b.tx_size]++;
+                L.cls_ntx[b.tx_size] += (uint32_t)ntx;
             }
         }
         const uint32_t cx0 = b.x >> 2, cx1 = (b.x + c.bw - 1) >> 2;
@@ -211,15 +211,13 @@ void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
         }
         blocks[n] = d;
         if (is_intra) intra[ni++] = n;
-        else {
-            inter[nn++] = n;
-            if (res) {
-                const int k = d.tx;
-                cblk[k][cn[k]] = n;
-                cpre[k][cn[k]] = ctx[k];
-                cn[k]++;
-                ctx[k] += ntx;
-            }
+        else inter[nn++] = n;
+        if (res) { /* transforms of inter and intra blocks alike */
+            const int k = d.tx;
+            cblk[k][cn[k]] = n;
+            cpre[k][cn[k]] = ctx[k];
+            cn[k]++;
+            ctx[k] += ntx;
         }
         n++;
     }
@@ -259,7 +257,8 @@ StreamState::~StreamState()
         delete f;
     }
     hipFree(owner_);
-    hipFree(ready_);
+    hipFree(edge_);
+    hipFree(resid_);
     hipFree(err_);
 }
 
@@ -333,18 +332,31 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
         owner_cap_ = cells;
     }
     const size_t nb = L.n_blocks ? L.n_blocks : 1;
-    if (nb > ready_cap_) {
-        size_t cap = ready_cap_ ? ready_cap_ : 4096;
+    if (nb > edge_cap_) {
+        size_t cap = edge_cap_ ? edge_cap_ : 4096;
         while (cap < nb) cap *= 2;
-        if (ready_) {
+        if (edge_) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            hipFree(ready_);
-            ready_ = nullptr;
+            hipFree(edge_);
+            edge_ = nullptr;
         }
-        if ((e = hipMalloc(&ready_, cap * 4)) != hipSuccess) return e;
-        if ((e = hipMemsetAsync(ready_, 0, cap * 4, s)) != hipSuccess) return e;
-        ready_cap_ = cap;
-        epoch_ = 0;
+        const size_t bytes = cap * CVP9_EDGE_WORDS * sizeof(uint64_t);
+        if ((e = hipMalloc(&edge_, bytes)) != hipSuccess) return e;
+        /* epochs start at 1: zeroed words never look valid */
+        if ((e = hipMemsetAsync(edge_, 0, bytes, s)) != hipSuccess) return e;
+        edge_cap_ = cap;
+    }
+    const size_t nc = L.num_coeffs ? L.num_coeffs : 1;
+    if (nc > resid_cap_) {
+        size_t cap = resid_cap_ ? resid_cap_ : 65536;
+        while (cap < nc) cap *= 2;
+        if (resid_) {
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            hipFree(resid_);
+            resid_ = nullptr;
+        }
+        if ((e = hipMalloc(&resid_, cap * sizeof(int16_t))) != hipSuccess) return e;
+        resid_cap_ = cap;
     }
     return hipSuccess;
 }
@@ -368,7 +380,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
         j.cls_ntx[k] = L.cls_ntx[k];
     }
     j.owner = owner_;
-    j.ready = ready_;
+    j.edge = edge_;
+    j.resid = resid_;
     j.nv12 = nv12;
     j.err = err_;
     j.n_blocks = L.n_blocks;
@@ -387,7 +400,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
 
 size_t StreamState::resident_bytes() const
 {
-    size_t b = owner_cap_ * 4 + ready_cap_ * 4 + (err_ ? 256 : 0);
+    size_t b = owner_cap_ * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 + resid_cap_ * 2 + (err_ ? 256 : 0);
     for (const FrameBuf *f : all_) b += f->cap;
     return b;
 }

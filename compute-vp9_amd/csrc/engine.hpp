@@ -63,7 +63,8 @@ public:
     /* reference-pool rule of the CPU path: KEY -> all 8 slots, else slot 0
      * (src/decoder/vp9_decoder.c:224-231) */
     void update_refs(bool key, FrameBuf *out);
-    /* owner map and ready flags large enough for L; may synchronize s */
+    /* owner map, edge records and residual scratch large enough for L;
+     * may synchronize s */
     hipError_t ensure_scratch(const PrepLayout &L, hipStream_t s);
     /* job descriptor for one frame of this stream */
     DJob make_job(const PrepLayout &L, const uint8_t *dev_prep, FrameBuf *out, uint8_t *nv12);
@@ -78,8 +79,10 @@ private:
     FrameBuf *refs_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t *owner_ = nullptr;
     size_t owner_cap_ = 0;
-    uint32_t *ready_ = nullptr;
-    size_t ready_cap_ = 0;
+    uint64_t *edge_ = nullptr;
+    size_t edge_cap_ = 0;
+    int16_t *resid_ = nullptr;
+    size_t resid_cap_ = 0;
     uint32_t *err_ = nullptr;
     uint32_t epoch_ = 0;
 };

@@ -13,6 +13,7 @@
  *
  * Stage order per batch (one stream, in order):
  *   k_fill (only jobs not fully covered) -> k_owner -> k_mc -> k_residual<N>
+ *   (inter: added to the prediction; intra: int16 residual kept aside)
  *   -> k_intra -> k_lf -> k_nv12
  */
 #include <hip/hip_runtime.h>
@@ -202,7 +203,8 @@ __global__ __launch_bounds__(256) void k_mc(DBatch B)
 
 template <int N>
 __device__ __forceinline__ void idct_add_wave(const int16_t *coef, uint8_t *dst,
-                                              uint32_t stride, double *lds, int lane)
+                                              uint32_t stride, int16_t *rout, double *lds,
+                                              int lane)
 {
     const int t = lane / N;
     const int j = lane % N;
@@ -247,7 +249,16 @@ __device__ __forceinline__ void idct_add_wave(const int16_t *coef, uint8_t *dst,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
 
-    if (coef) {
+    if (coef && rout) {
+        /* intra transform: keep the int16 residual for the intra stage */
+        int16_t *o = rout + r * N;
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            const double v = acc[c];
+            const int32_t rr = (int32_t)__dadd_rn(v, v >= 0.0 ? 0.5 : -0.5);
+            o[c] = (int16_t)clampi(rr, -32768, 32767);
+        }
+    } else if (coef) {
         uint8_t *row = dst + (size_t)r * stride;
 #pragma unroll
         for (int c4 = 0; c4 < N; c4 += 4) {
@@ -270,10 +281,11 @@ __device__ __forceinline__ void idct_add_wave(const int16_t *coef, uint8_t *dst,
  * and destination origin, or coef = nullptr past the end */
 template <int N>
 __device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
-                                          const int16_t *&coef, uint8_t *&dst)
+                                          const int16_t *&coef, uint8_t *&dst, int16_t *&rout)
 {
     coef = nullptr;
     dst = nullptr;
+    rout = nullptr;
     if (txi >= J.cls_ntx[cls]) return;
     const uint32_t *pre = J.cls_pre[cls];
     uint32_t lo = 0, hi = J.cls_nblk[cls];
@@ -285,7 +297,10 @@ __device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
     const uint32_t l = txi - pre[lo];
     const uint32_t ty = l / k.nx, tx = l - ty * k.nx;
     coef = J.coeffs + k.coeff + (size_t)l * (N * N);
-    dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
+    if (k.flags & DB_INTRA)
+        rout = J.resid + k.coeff + (size_t)l * (N * N);
+    else
+        dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
 }
 
 template <int N>
@@ -303,8 +318,9 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
     const uint32_t txi = (wv - pre[j]) * G + lane / N;
     const int16_t *coef;
     uint8_t *dst;
-    locate_tx<N>(J, C, txi, coef, dst);
-    idct_add_wave<N>(coef, dst, J.out.sy, lds[threadIdx.x >> 6], lane);
+    int16_t *rout;
+    locate_tx<N>(J, C, txi, coef, dst, rout);
+    idct_add_wave<N>(coef, dst, J.out.sy, rout, lds[threadIdx.x >> 6], lane);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -313,55 +329,91 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
 /* The reference predicts blocks strictly in list order, so a neighbour     */
 /* sample holds its final value only if its owner block comes earlier in    */
 /* the list; otherwise it still holds the initial 128. Inter blocks are     */
-/* final before this kernel starts. Intra blocks are taken by workgroups in */
-/* list order through an atomic ticket; a block waits (bounded spin) only   */
-/* on earlier intra neighbours, which running workgroups already hold, so   */
-/* every wait ends.                                                          */
+/* final before this kernel starts, and the residuals of intra blocks were  */
+/* computed by the residual stage (they do not depend on neighbours), so    */
+/* the ordered part is prediction + residual add only.                      */
+/*                                                                          */
+/* Intra blocks are taken by workgroups in list order through an atomic     */
+/* ticket. Each publishes its final bottom row and right column of Y, U, V  */
+/* as 64-bit words (4 samples + the frame epoch), written and read with     */
+/* device-scope atomics: a word is valid exactly when its epoch matches, so */
+/* no fences or flags are needed. A neighbour sample of block b owned by an */
+/* earlier intra block o always lies on o's bottom row or right column:     */
+/* o covers the sample and not the sample's neighbour inside b, and blocks  */
+/* are disjoint rectangles. A block waits (bounded) only on earlier intra   */
+/* blocks, which running workgroups already hold, so every wait ends.        */
 
-__device__ __forceinline__ uint8_t nb_sample(const DJob &J, const uint8_t *plane, uint32_t stride,
-                                             int px, int py, int cx, int cy, uint32_t b)
+__device__ __forceinline__ int nb_sample(const DJob &J, int p, int px, int py, uint32_t b,
+                                         uint32_t spin_limit)
 {
-    const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
+    const int sh = p == 0 ? 2 : 1; /* plane sample -> 4x4 luma owner cell */
+    const uint32_t o = J.owner[(uint32_t)(py >> sh) * J.ow + (uint32_t)(px >> sh)];
     if (o == CVP9_OWNER_NONE || o >= b) return 128;
-    return plane[(size_t)py * stride + px];
+    const DBlock k = J.blocks[o];
+    if (!(k.flags & DB_INTRA)) {
+        const uint8_t *pl = p == 0 ? J.out.y : (p == 1 ? J.out.u : J.out.v);
+        const uint32_t st = p == 0 ? J.out.sy : J.out.suv;
+        return pl[(size_t)py * st + px];
+    }
+    int ox, oy, ow, oh;
+    if (p == 0) {
+        ox = k.x; oy = k.y; ow = k.bw; oh = k.bh;
+    } else {
+        ox = k.x >> 1; oy = k.y >> 1; ow = (k.bw + 1) >> 1; oh = (k.bh + 1) >> 1;
+    }
+    const int bottom = p == 0 ? 0 : (p == 1 ? 32 : 48);
+    int base, idx;
+    if (py == oy + oh - 1) {
+        base = bottom;
+        idx = px - ox;
+    } else if (px == ox + ow - 1) {
+        base = bottom + (p == 0 ? 16 : 8);
+        idx = py - oy;
+    } else {
+        atomicOr(J.err, 2u); /* not on an edge: impossible for disjoint blocks */
+        return 128;
+    }
+    const uint64_t *w = J.edge + (size_t)o * CVP9_EDGE_WORDS + base + (idx >> 2);
+    uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while ((uint32_t)(v >> 32) != J.epoch) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > spin_limit) {
+            atomicOr(J.err, 1u);
+            break;
+        }
+        if ((spins & 255) == 0 &&
+            __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+        v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return (int)((v >> (8 * (idx & 3))) & 0xff);
 }
 
-template <int N>
-__device__ void intra_residual(const DJob &J, const DBlock &k, double *lds, int wave, int lane)
+__device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc)
 {
-    constexpr int G = 64 / N;
-    const uint32_t ny = k.bh / N;
-    const uint32_t ntx = k.nx * ny;
-    for (uint32_t g = wave; g * G < ntx; g += 4) {
-        const uint32_t l = g * G + lane / N;
-        const int16_t *coef = nullptr;
-        uint8_t *dst = nullptr;
-        if (l < ntx) {
-            const uint32_t ty = l / k.nx, tx = l - ty * k.nx;
-            coef = J.coeffs + k.coeff + (size_t)l * (N * N);
-            dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
-        }
-        idct_add_wave<N>(coef, dst, J.out.sy, lds, lane);
-    }
+    if (mode == 0) return dc;
+    if (mode == 1) return a;
+    if (mode == 2) return l;
+    return clampi(a + l - tl, 0, 255);
 }
 
 __global__ __launch_bounds__(256) void k_intra(DBatch B)
 {
     __shared__ uint32_t s_ticket;
-    __shared__ uint8_t s_above[3][256];
-    __shared__ uint8_t s_left[3][256];
-    __shared__ uint32_t s_sum[3];
+    __shared__ uint8_t s_nb[3][2][64];   /* above / left samples per plane */
     __shared__ uint8_t s_tl[3];
-    __shared__ double s_tmp[4][2 * 32 * 33];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __shared__ uint32_t s_sum[3];
+    __shared__ uint8_t s_edge[3][2][64]; /* final bottom row / right column */
+    const int tid = threadIdx.x;
     const uint32_t total = B.pre_intra[B.njobs];
 
-    /* The ticket is broadcast through LDS and then made wave-uniform with
+    /* The ticket is broadcast through LDS and made wave-uniform with
      * readfirstlane, so the loop condition is a scalar branch. With an LDS
      * value as the exit test the compiler treats the exit as divergent and
-     * restructures the loop so that thread 0's ticket fetch and flag store
-     * run outside the loop holding the barriers: the other lanes then
-     * repeat one ticket forever. */
+     * restructures the loop so that thread 0's ticket fetch runs outside
+     * the loop holding the barriers: the other lanes then repeat one ticket
+     * forever. */
     if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
     __syncthreads();
     uint32_t tk = __builtin_amdgcn_readfirstlane(s_ticket);
@@ -371,117 +423,113 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
         const uint32_t b = J.intra[tk - B.pre_intra[j]];
         const DBlock k = J.blocks[b];
         const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
+        const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
 
-        /* 1. wait for earlier intra blocks owning a neighbour cell */
-        {
-            const int cx0 = x >> 2, cx1 = (x + bw - 1) >> 2;
-            const int cy0 = y >> 2, cy1 = (y + bh - 1) >> 2;
-            const int na = y > 0 ? cx1 - cx0 + 1 : 0;
-            const int nl = x > 0 ? cy1 - cy0 + 1 : 0;
-            const int nt = (x > 0 && y > 0) ? 1 : 0;
-            for (int i = tid; i < na + nl + nt; i += 256) {
-                int cx, cy;
-                if (i < na) { cx = cx0 + i; cy = (y - 1) >> 2; }
-                else if (i < na + nl) { cx = (x - 1) >> 2; cy = cy0 + (i - na); }
-                else { cx = (x - 1) >> 2; cy = (y - 1) >> 2; }
-                const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
-                if (o != CVP9_OWNER_NONE && o < b && (J.blocks[o].flags & DB_INTRA)) {
-                    /* bounded: after spin_limit polls, or once any wait of
-                     * this stream has timed out, give up and flag the error */
-                    uint32_t spins = 0;
-                    while (__hip_atomic_load(&J.ready[o], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) != J.epoch) {
-                        __builtin_amdgcn_s_sleep(2);
-                        if (++spins > B.spin_limit) {
-                            atomicOr(J.err, 1u);
-                            break;
-                        }
-                        if ((spins & 255) == 0 &&
-                            __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                            break;
-                    }
-                }
+        /* 1. neighbours: 127 outside the frame, else the owner rule */
+        if (tid < 3) s_sum[tid] = 0;
+        const int nY = bw + bh + 1, nC = cbw + cbh + 1;
+        for (int i = tid; i < nY + 2 * nC; i += 256) {
+            int p, e;
+            if (i < nY) { p = 0; e = i; }
+            else if (i < nY + nC) { p = 1; e = i - nY; }
+            else { p = 2; e = i - nY - nC; }
+            const int px = p ? cbx : x, py = p ? cby : y;
+            const int w = p ? cbw : bw, h = p ? cbh : bh;
+            if (e < w) {
+                s_nb[p][0][e] = (uint8_t)(py > 0 ? nb_sample(J, p, px + e, py - 1, b, B.spin_limit) : 127);
+            } else if (e < w + h) {
+                const int r = e - w;
+                s_nb[p][1][r] = (uint8_t)(px > 0 ? nb_sample(J, p, px - 1, py + r, b, B.spin_limit) : 127);
+            } else {
+                s_tl[p] = (uint8_t)((px > 0 && py > 0) ? nb_sample(J, p, px - 1, py - 1, b, B.spin_limit)
+                                                       : 127);
             }
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-        /* 2. neighbours of Y, U, V (missing: 127; later or unowned: 128) */
-        const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
-        if (tid < 3) s_sum[tid] = 0;
-        for (int i = tid; i < 3 * 256; i += 256) {
-            const int p = i >> 8, e = i & 255;
-            const uint8_t *pl = p == 0 ? J.out.y : (p == 1 ? J.out.u : J.out.v);
-            const uint32_t st = p == 0 ? J.out.sy : J.out.suv;
-            const int px = p == 0 ? x : cbx, py = p == 0 ? y : cby;
-            const int w = p == 0 ? bw : cbw, h = p == 0 ? bh : cbh;
-            const int sh = p == 0 ? 2 : 1; /* sample -> owner cell */
-            if (e < w)
-                s_above[p][e] = py > 0 ? nb_sample(J, pl, st, px + e, py - 1, (px + e) >> sh,
-                                                    (py - 1) >> sh, b)
-                                       : (uint8_t)127;
-            if (e < h)
-                s_left[p][e] = px > 0 ? nb_sample(J, pl, st, px - 1, py + e, (px - 1) >> sh,
-                                                   (py + e) >> sh, b)
-                                      : (uint8_t)127;
-            if (e == 0)
-                s_tl[p] = (px > 0 && py > 0)
-                              ? nb_sample(J, pl, st, px - 1, py - 1, (px - 1) >> sh, (py - 1) >> sh, b)
-                              : (uint8_t)127;
-        }
-        __syncthreads();
-        /* DC sums */
-        for (int i = tid; i < 3 * 512; i += 256) {
-            const int p = i >> 9, e = i & 511;
-            const int w = p == 0 ? bw : cbw, h = p == 0 ? bh : cbh;
-            const int mode = p == 0 ? k.ymode : k.uvmode;
-            if (mode != 0) continue;
+        /* 2. DC sums over w above + h left samples */
+        for (int i = tid; i < 3 * 128; i += 256) {
+            const int p = i >> 7, e = i & 127;
+            if ((p == 0 ? k.ymode : k.uvmode) != 0) continue;
+            const int w = p ? cbw : bw, h = p ? cbh : bh;
             uint32_t v = 0;
-            if (e < 256) { if (e < w) v = s_above[p][e]; }
-            else if (e - 256 < h) v = s_left[p][e - 256];
+            if (e < 64) { if (e < w) v = s_nb[p][0][e]; }
+            else if (e - 64 < h) v = s_nb[p][1][e - 64];
             if (v) atomicAdd(&s_sum[p], v);
         }
         __syncthreads();
 
-        /* 3. prediction */
-        for (int p = 0; p < 3; p++) {
-            uint8_t *pl = p == 0 ? J.out.y : (p == 1 ? J.out.u : J.out.v);
-            const uint32_t st = p == 0 ? J.out.sy : J.out.suv;
-            const int px = p == 0 ? x : cbx, py = p == 0 ? y : cby;
-            const int w = p == 0 ? bw : cbw, h = p == 0 ? bh : cbh;
-            const int mode = p == 0 ? k.ymode : k.uvmode;
-            const uint32_t dc = (s_sum[p] + (uint32_t)((w + h) >> 1)) / (uint32_t)(w + h);
-            const int tl = s_tl[p];
-            for (int i = tid; i < w * h; i += 256) {
-                const int r = i / w, c = i - r * w;
-                int v;
-                if (mode == 0) v = (int)(uint8_t)dc;
-                else if (mode == 1) v = s_above[p][c];
-                else if (mode == 2) v = s_left[p][r];
-                else v = clampi(s_above[p][c] + s_left[p][r] - tl, 0, 255);
-                pl[(size_t)(py + r) * st + px + c] = (uint8_t)v;
+        /* 3. prediction + residual, luma 4 samples per thread */
+        {
+            const int dc = (int)(uint8_t)((s_sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
+            const int tl = s_tl[0], mode = k.ymode;
+            const int lt = k.tx + 2, T = 1 << lt;
+            const bool res = !(k.flags & DB_SKIP);
+            const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
+            const int nq = (bw + 3) >> 2;
+            uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
+            for (int i = tid; i < nq * bh; i += 256) {
+                const int r = i / nq, c0 = (i - r * nq) * 4;
+                const int l = s_nb[0][1][r];
+                const int16_t *rs = nullptr;
+                if (r < rrows && c0 < rcols)
+                    rs = J.resid + k.coeff +
+                         (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
+                         ((r & (T - 1)) << lt) + (c0 & (T - 1));
+                uint32_t word = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int c = c0 + e;
+                    if (c < bw) {
+                        int v = intra_pred(mode, s_nb[0][0][c], l, tl, dc);
+                        if (rs) v = clampi(v + rs[e], 0, 255);
+                        word |= (uint32_t)v << (8 * e);
+                        if (r == bh - 1) s_edge[0][0][c] = (uint8_t)v;
+                        if (c == bw - 1) s_edge[0][1][r] = (uint8_t)v;
+                    }
+                }
+                uint8_t *d = Y + (size_t)r * J.out.sy + c0;
+                if (c0 + 3 < bw) {
+                    *(uint32_t *)d = word;
+                } else {
+                    for (int e = 0; c0 + e < bw; e++) d[e] = (uint8_t)(word >> (8 * e));
+                }
+            }
+            /* chroma: prediction only (the reference adds no chroma residual) */
+            for (int i = tid; i < 2 * cbw * cbh; i += 256) {
+                const int p = i < cbw * cbh ? 1 : 2;
+                const int q = i - (p - 1) * cbw * cbh;
+                const int r = q / cbw, c = q - r * cbw;
+                const int dcc = (int)(uint8_t)((s_sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
+                                               (uint32_t)(cbw + cbh));
+                const int v = intra_pred(k.uvmode, s_nb[p][0][c], s_nb[p][1][r], s_tl[p], dcc);
+                uint8_t *pl = p == 1 ? J.out.u : J.out.v;
+                pl[(size_t)(cby + r) * J.out.suv + cbx + c] = (uint8_t)v;
+                if (r == cbh - 1) s_edge[p][0][c] = (uint8_t)v;
+                if (c == cbw - 1) s_edge[p][1][r] = (uint8_t)v;
             }
         }
         __syncthreads();
 
-        /* 4. luma residual of the block */
-        if (!(k.flags & DB_SKIP)) {
-            double *lds = s_tmp[wave];
-            switch (k.tx) {
-            case 0: intra_residual<4>(J, k, lds, wave, lane); break;
-            case 1: intra_residual<8>(J, k, lds, wave, lane); break;
-            case 2: intra_residual<16>(J, k, lds, wave, lane); break;
-            default: intra_residual<32>(J, k, lds, wave, lane); break;
+        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
+         *    V bottom, V right */
+        if (tid < CVP9_EDGE_WORDS) {
+            const int t = tid;
+            int p, dir, wi, len;
+            if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
+            else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
+            if (4 * wi < len) {
+                uint64_t d = (uint64_t)J.epoch << 32;
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (4 * wi + e < len) d |= (uint64_t)s_edge[p][dir][4 * wi + e] << (8 * e);
+                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             }
         }
 
-        /* 5. publish, then take the next ticket */
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (tid == 0) {
-            __hip_atomic_store(&J.ready[b], J.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            s_ticket = atomicAdd(B.ticket, 1u);
-        }
+        /* next ticket; the barrier also orders LDS reuse */
+        if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
         __syncthreads();
         tk = __builtin_amdgcn_readfirstlane(s_ticket);
     }
