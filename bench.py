@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=100)
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="only run the calibration copy kernel (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -204,6 +206,14 @@ def main():
     if P.lib().cvp9_hip_device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+
+    copy_bytes = 1 << 30
+    copy_gbps = P.lib().cvp9_hip_copy_probe(copy_bytes, 5)
+    if args.copy_probe_only:
+        print(json.dumps({"copy_probe_bytes_per_launch": 2 * copy_bytes,
+                          "copy_GBps": round(copy_gbps, 1)}), flush=True)
+        return
+    log(f"copy probe: {copy_gbps:.1f} GB/s")
 
     ss = single_stream(P, args)
     value = args.steps / ss["clean_s"]
@@ -243,6 +253,7 @@ def main():
         "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1),
         "stages": stats,
         "roofline": roof,
+        "copy_peak_GBps": round(copy_gbps, 1),
         "multi_stream": None if not multi else {
             "workload": f"{multi['streams']} streams (16 each 640x360/854x480/1280x720/1920x1080), "
                         "one batch per step",

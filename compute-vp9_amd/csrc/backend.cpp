@@ -407,6 +407,34 @@ cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf)
     return plan_measure(pf, L);
 }
 
+double cvp9_hip_copy_probe(uint64_t bytes, int iters)
+{
+    if (bytes < 16 || (bytes & 15) || iters < 1) return -1.0;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double gbps = -1.0;
+    if (hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&b, bytes) == hipSuccess &&
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+        hipMemsetAsync(a, 1, bytes, s) == hipSuccess &&
+        cvp9_launch_copy_probe(a, b, bytes, s) == hipSuccess &&
+        hipEventRecord(e0, s) == hipSuccess) {
+        bool ok = true;
+        for (int i = 0; i < iters && ok; i++) ok = cvp9_launch_copy_probe(a, b, bytes, s) == hipSuccess;
+        float ms = 0.f;
+        if (ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f)
+            gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    hipFree(a);
+    hipFree(b);
+    return gbps;
+}
+
 cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags)
 {
     Backend *b = (Backend *)ctx;

@@ -28,6 +28,7 @@ struct StageCounts {
 
 hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double *m16,
                                 const double *m32);
+hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t s);
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s);
 
 #endif

@@ -628,6 +628,24 @@ __global__ __launch_bounds__(256) void k_nv12(DBatch B)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Plain streaming copy: the reference kernel for bandwidth and counter      */
+/* calibration (bytes moved are known exactly).                              */
+
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4 *__restrict__ src,
+                                                    uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------------ */
 /* host-side launchers                                                       */
 
 hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double *m16,

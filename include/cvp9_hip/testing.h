@@ -30,6 +30,12 @@ int cvp9_hip_idct_table(int n, double *out);
  * HIP path takes the frame, else the code hip_decode_frame would return. */
 cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf);
 
+/* Copies `bytes` (multiple of 16) device to device `iters` times with a
+ * plain streaming kernel on the current device; returns the achieved GB/s
+ * (read + write bytes / time), or a negative value on error. Used to
+ * calibrate HBM traffic counters and to report a measured copy peak. */
+double cvp9_hip_copy_probe(uint64_t bytes, int iters);
+
 /* Debug flags for one backend context (bit 0: skip the deblocking stage). */
 #define CVP9_HIP_DEBUG_NO_DEBLOCK 1u
 cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags);
