@@ -34,6 +34,26 @@ METRIC = "recon frames/s @1080p (1 vs 64 streams), p50/p99 frame latency, HBM GB
 MIX = [(640, 360), (854, 480), (1280, 720), (1920, 1080)]
 
 
+KERNEL_OF = {"intra": "k_intra", "mc": "k_mc", "deblock": "k_lf", "owner": "k_owner",
+             "fill": "k_fill", "nv12": "k_nv12", "residual4": "k_residual<4>",
+             "residual8": "k_residual<8>", "residual16": "k_residual<16>",
+             "residual32": "k_residual<32>"}
+SINGLE_WORKLOAD = ("1920x1080 single stream, 100-frame preset-1 sequence, "
+                   "plain per-stage kernels")
+
+
+def measured_traffic(workload, stage):
+    """HBM-side bytes per launch of a stage's kernel from the committed
+    rocprofv3 counter summary (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f)[workload]
+        return entry["kernels"][KERNEL_OF[stage]]["traffic_bytes"], entry["profile"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -118,6 +138,7 @@ def single_stream(P, args):
         tot_bytes = sum(sb[i].get(name, 0) for i in order)
         per_stage[name] = {"ms_total": round(ms, 4), "launches": n,
                            "us_per_launch": round(1e3 * ms / n, 3),
+                           "algo_bytes_per_launch": round(tot_bytes / n),
                            "algo_GBps": round(tot_bytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
         if name.startswith("residual"):
             N = int(name[8:])
@@ -220,9 +241,13 @@ def main():
     stats = ss["stats"]
     dom = max(stats, key=lambda k: stats[k]["ms_total"])
     d = stats[dom]
-    roof = {"kernel": dom, "bound": "hbm", "achieved": d["algo_GBps"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(d["algo_GBps"] / HBM_PEAK_GBS, 4) if d["algo_GBps"] else None,
-            "traffic": None, "us_per_launch": d["us_per_launch"]}
+    traffic, tsrc = measured_traffic(SINGLE_WORKLOAD, dom)
+    roof = {"kernel": KERNEL_OF.get(dom, dom), "bound": "hbm", "achieved": d["algo_GBps"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["algo_GBps"] / HBM_PEAK_GBS, 4) if d["algo_GBps"] else None,
+            "traffic": traffic, "traffic_source": tsrc,
+            "algo_bytes_per_launch": d.get("algo_bytes_per_launch"),
+            "us_per_launch": d["us_per_launch"]}
     if "fp64_dense_TOPs" in d:
         roof["fp64_dense_TOPs"] = d["fp64_dense_TOPs"]
         roof["fp64_peak_no_fma_TOPs"] = FP64_PEAK_TOPS / 2
@@ -246,8 +271,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u8/i16 (fp64 transform)",
         "data": "synthetic parsed frames (seeded generator), inputs resident in HBM",
-        "config": {"workload": "1920x1080 single stream, 100-frame preset-1 sequence, "
-                               "plain per-stage kernels", "frames": args.frames,
+        "config": {"workload": SINGLE_WORKLOAD, "frames": args.frames,
                    "step": "one frame"},
         "timed_with_events_fps": round(args.steps / ss["timed_s"], 2),
         "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1),
