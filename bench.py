@@ -181,8 +181,16 @@ def multi_stream(P, args):
     eng.sync()
     dt = time.perf_counter() - t0
     px = sum(w * h for w, h in layout) * steps
+    # per-stage view of the same batches, HIP events around each launch
+    eng.stage_stats(reset=True)
+    eng.set_timing(True)
+    for k in range(args.mwarmup, args.mwarmup + steps):
+        batch(k)
+    stats = {n: {"us_per_launch": round(1e3 * ms / c, 2), "launches": c}
+             for n, (ms, c) in eng.stage_stats(reset=True).items() if c}
+    eng.set_timing(False)
     eng.close()
-    return {"streams": len(layout), "steps": steps, "seconds": dt,
+    return {"streams": len(layout), "steps": steps, "seconds": dt, "stages": stats,
             "frames_per_s": len(layout) * steps / dt,
             "p1080_equiv_per_s": px / (1920 * 1080) / dt,
             "ms_per_batch": 1e3 * dt / steps}
@@ -284,6 +292,7 @@ def main():
             "frames_per_s": round(multi["frames_per_s"], 1),
             "p1080_equiv_frames_per_s": round(multi["p1080_equiv_per_s"], 1),
             "ms_per_batch": round(multi["ms_per_batch"], 4),
+            "stages": multi["stages"],
             "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2)},
         "cpu_baseline": cpu,
     }

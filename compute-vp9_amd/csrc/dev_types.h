@@ -40,6 +40,21 @@
 #define CVP9_EDGE_WORDS 64
 #define CVP9_MAX_INTRA_DIM 64
 
+/* Motion compensation splits an inter block into strips of rows; a strip
+ * is one wave, one quad of 4 luma samples per lane. */
+#if defined(__HIPCC__)
+#define CVP9_HD __host__ __device__
+#else
+#define CVP9_HD
+#endif
+CVP9_HD static inline uint32_t cvp9_mc_rows(uint32_t bw)
+{
+    const uint32_t nq = (bw + 3) / 4;
+    uint32_t p2 = 1;
+    while (p2 < nq) p2 <<= 1;
+    return p2 >= 64 ? 1 : 64 / p2;
+}
+
 /* block flags */
 #define DB_INTRA   1u
 #define DB_SKIP    2u   /* no residual (skip, or transform grid empty) */
@@ -70,6 +85,7 @@ struct DJob {
     const DBlock   *blocks;
     const int16_t  *coeffs;
     const uint32_t *inter;
+    const uint32_t *mc_pre;        /* exclusive prefix of MC strips per inter block */
     const uint32_t *intra;
     const uint32_t *cls_blk[4];
     const uint32_t *cls_pre[4];
@@ -78,7 +94,7 @@ struct DJob {
     int16_t        *resid;         /* intra residuals, indexed like coeffs */
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
     uint32_t       *err;           /* error word (spin timeout) */
-    uint32_t n_blocks, n_inter, n_intra;
+    uint32_t n_blocks, n_inter, n_intra, n_mc;
     uint32_t cls_nblk[4], cls_ntx[4];
     uint32_t ow, oh;
     uint32_t epoch;
@@ -92,7 +108,7 @@ struct DBatch {
     const DJob     *jobs;
     uint32_t        njobs;
     const uint32_t *pre_blocks;    /* owner-map scatter: one wave per block */
-    const uint32_t *pre_inter;     /* motion compensation: one wave / block */
+    const uint32_t *pre_inter;     /* motion compensation: one wave / strip */
     const uint32_t *pre_grp[4];    /* residual: groups of 64/N transforms */
     const uint32_t *pre_intra;     /* intra: one ticket per block */
     const uint32_t *pre_lf;        /* deblocking windows */

@@ -146,7 +146,12 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
             if (!claim_bits(bits, (uint64_t)cy * ow + cx0, cx1 - cx0 + 1)) return CVP9_ERR_UNSUPPORTED;
         covered += (uint64_t)(cx1 - cx0 + 1) * (cy1 - cy0 + 1);
         L.n_blocks++;
-        if (intra) L.n_intra++; else L.n_inter++;
+        if (intra) L.n_intra++;
+        else {
+            L.n_inter++;
+            const uint32_t rows = cvp9_mc_rows((uint32_t)c.bw);
+            L.n_mc += ((uint32_t)c.bh + rows - 1) / rows;
+        }
     }
     L.fill = covered != (uint64_t)ow * oh;
     L.num_coeffs = pf->coeffs ? pf->num_coeffs : 0;
@@ -154,6 +159,7 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
     uint64_t off = 0;
     L.off_blocks = off; off = align256(off + (uint64_t)L.n_blocks * sizeof(DBlock));
     L.off_inter = off;  off = align256(off + (uint64_t)L.n_inter * 4);
+    L.off_mc_pre = off; off = align256(off + (uint64_t)(L.n_inter + 1) * 4);
     L.off_intra = off;  off = align256(off + (uint64_t)L.n_intra * 4);
     for (int k = 0; k < 4; k++) {
         L.off_cls_blk[k] = off; off = align256(off + (uint64_t)L.cls_nblk[k] * 4);
@@ -169,6 +175,8 @@ void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
     const int W = (int)L.W, H = (int)L.H;
     DBlock *blocks = (DBlock *)(dst + L.off_blocks);
     uint32_t *inter = (uint32_t *)(dst + L.off_inter);
+    uint32_t *mc_pre = (uint32_t *)(dst + L.off_mc_pre);
+    uint32_t mc_acc = 0;
     uint32_t *intra = (uint32_t *)(dst + L.off_intra);
     uint32_t *cblk[4], *cpre[4], cn[4] = {0, 0, 0, 0}, ctx[4] = {0, 0, 0, 0};
     for (int k = 0; k < 4; k++) {
@@ -211,7 +219,12 @@ void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
         }
         blocks[n] = d;
         if (is_intra) intra[ni++] = n;
-        else inter[nn++] = n;
+        else {
+            mc_pre[nn] = mc_acc;
+            const uint32_t rows = cvp9_mc_rows(d.bw);
+            mc_acc += (d.bh + rows - 1) / rows;
+            inter[nn++] = n;
+        }
         if (res) { /* transforms of inter and intra blocks alike */
             const int k = d.tx;
             cblk[k][cn[k]] = n;
@@ -222,6 +235,7 @@ void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
         n++;
     }
     for (int k = 0; k < 4; k++) cpre[k][cn[k]] = ctx[k];
+    mc_pre[nn] = mc_acc;
     if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
 }
 
@@ -233,7 +247,9 @@ size_t frame_bytes(uint32_t w, uint32_t h, uint32_t *sy, uint32_t *suv)
     const uint32_t s1 = (((w + 1) / 2) + 63) & ~63u;
     if (sy) *sy = s0;
     if (suv) *suv = s1;
-    return (size_t)s0 * h + 2 * (size_t)s1 * ((h + 1) / 2);
+    /* + 64 bytes of slack: the motion-compensation fast path reads an
+     * aligned 8-byte window that may extend 3 bytes past the last sample */
+    return (size_t)s0 * h + 2 * (size_t)s1 * ((h + 1) / 2) + 64;
 }
 
 DFrame FrameBuf::dframe() const
@@ -372,6 +388,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.blocks = (const DBlock *)(dev + L.off_blocks);
     j.coeffs = (const int16_t *)(dev + L.off_coeffs);
     j.inter = (const uint32_t *)(dev + L.off_inter);
+    j.mc_pre = (const uint32_t *)(dev + L.off_mc_pre);
     j.intra = (const uint32_t *)(dev + L.off_intra);
     for (int k = 0; k < 4; k++) {
         j.cls_blk[k] = (const uint32_t *)(dev + L.off_cls_blk[k]);
@@ -386,6 +403,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.err = err_;
     j.n_blocks = L.n_blocks;
     j.n_inter = L.n_inter;
+    j.n_mc = L.n_mc;
     j.n_intra = L.n_intra;
     j.ow = (L.W + 3) / 4;
     j.oh = (L.H + 3) / 4;
@@ -432,7 +450,7 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
         }
         c[P_FILL] = fill;
         c[P_BLK] = J.n_blocks;
-        c[P_INTER] = J.n_inter;
+        c[P_INTER] = J.n_mc;
         for (int k = 0; k < 4; k++) c[P_G0 + k] = div_up(J.cls_ntx[k], 64u >> (2 + k));
         c[P_INTRA] = J.n_intra;
         c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;

@@ -27,7 +27,8 @@ struct PrepLayout {
     uint32_t n_blocks = 0, n_inter = 0, n_intra = 0;
     uint32_t cls_nblk[4] = {0, 0, 0, 0}, cls_ntx[4] = {0, 0, 0, 0};
     uint32_t num_coeffs = 0;
-    uint64_t off_blocks = 0, off_inter = 0, off_intra = 0;
+    uint32_t n_mc = 0;              /* motion-compensation strips (one wave each) */
+    uint64_t off_blocks = 0, off_inter = 0, off_intra = 0, off_mc_pre = 0;
     uint64_t off_cls_blk[4] = {0, 0, 0, 0}, off_cls_pre[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
 };

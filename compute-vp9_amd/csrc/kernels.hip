@@ -108,52 +108,58 @@ __global__ __launch_bounds__(256) void k_owner(DBatch B)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Motion compensation (vp9_reconstruct.c:94-131, :248-264, :312-334):       */
-/* one wave per inter block, each lane 4 horizontal pixels of a row.        */
+/* Motion compensation (vp9_reconstruct.c:94-131, :248-264, :312-334).      */
+/* One wave per strip of an inter block: cvp9_mc_rows(bw) luma rows, one    */
+/* quad of 4 samples per lane, plus the chroma rows under those luma rows.  */
+/* Quads whose 5-sample source window lies inside the reference row read it */
+/* with two aligned dword loads; the others take each tap with the edge     */
+/* clamp.                                                                   */
 
-__device__ __forceinline__ void mc_plane(const uint8_t *__restrict__ ref, uint32_t rs,
-                                         int rw, int rh, uint8_t *__restrict__ dst,
-                                         uint32_t ds, int bx, int by, int w, int h,
-                                         int mvx, int mvy, int lane)
+__device__ __forceinline__ uint32_t mc_quad(const uint8_t *__restrict__ ref, uint32_t rs, int rw,
+                                            int rh, int ix, int iy, int fx, int fy, int c0, int r)
 {
-    const int px8 = bx * 8 + mvx, py8 = by * 8 + mvy;
-    const int ix = px8 >> 3, iy = py8 >> 3;
-    const int fx = px8 & 7, fy = py8 & 7;
-    const int nq = (w + 3) >> 2;
-    const int rows = 64 / nq;
-    const int rq = lane / nq, q = lane - rq * nq;
-    if (rq >= rows) return;
-    const int c0 = q * 4;
-    int xs[5];
-#pragma unroll
-    for (int e = 0; e < 5; e++) xs[e] = clampi(ix + c0 + e, 0, rw - 1);
-    for (int r = rq; r < h; r += rows) {
-        const uint8_t *r0 = ref + (size_t)clampi(iy + r, 0, rh - 1) * rs;
-        const uint8_t *r1 = ref + (size_t)clampi(iy + r + 1, 0, rh - 1) * rs;
-        uint32_t a[5], b[5];
+    const uint8_t *r0 = ref + (size_t)clampi(iy + r, 0, rh - 1) * rs;
+    const uint8_t *r1 = ref + (size_t)clampi(iy + r + 1, 0, rh - 1) * rs;
+    const int sx = ix + c0;
+    uint32_t a[5], b[5];
+    if (sx >= 0 && sx + 4 <= rw - 1) {
+        /* rows start 64-byte aligned, so both windows share one offset; the
+         * 8-byte window may pass the row end by 3 bytes, inside the row pitch
+         * or the frame's tail slack */
+        const uint32_t o = (uint32_t)sx & 3;
+        const uint32_t *qa = (const uint32_t *)(r0 + (sx - (int)o));
+        const uint32_t *qb = (const uint32_t *)(r1 + (sx - (int)o));
+        const uint64_t va = (((uint64_t)qa[1] << 32) | qa[0]) >> (8 * o);
+        const uint64_t vb = (((uint64_t)qb[1] << 32) | qb[0]) >> (8 * o);
 #pragma unroll
         for (int e = 0; e < 5; e++) {
-            a[e] = r0[xs[e]];
-            b[e] = r1[xs[e]];
+            a[e] = (uint32_t)(va >> (8 * e)) & 0xff;
+            b[e] = (uint32_t)(vb >> (8 * e)) & 0xff;
         }
-        uint8_t *d = dst + (size_t)r * ds + c0;
+    } else {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            if (c0 + e < w) {
-                const uint32_t top = (a[e] * (8 - fx) + a[e + 1] * fx + 4) >> 3;
-                const uint32_t bot = (b[e] * (8 - fx) + b[e + 1] * fx + 4) >> 3;
-                d[e] = (uint8_t)((top * (8 - fy) + bot * fy + 4) >> 3);
-            }
+        for (int e = 0; e < 5; e++) {
+            const int x = clampi(sx + e, 0, rw - 1);
+            a[e] = r0[x];
+            b[e] = r1[x];
         }
     }
+    uint32_t out = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint32_t top = (a[e] * (8 - fx) + a[e + 1] * fx + 4) >> 3;
+        const uint32_t bot = (b[e] * (8 - fx) + b[e + 1] * fx + 4) >> 3;
+        out |= ((top * (8 - fy) + bot * fy + 4) >> 3) << (8 * e);
+    }
+    return out;
 }
 
-__device__ __forceinline__ void fill_plane(uint8_t *dst, uint32_t ds, int w, int h,
-                                           uint8_t v, int lane)
+__device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
 {
-    for (int i = lane; i < w * h; i += 64) {
-        const int r = i / w, c = i - r * w;
-        dst[(size_t)r * ds + c] = v;
+    if (n >= 4 && ((uintptr_t)d & 3) == 0) {
+        *(uint32_t *)d = v;
+    } else {
+        for (int e = 0; e < n && e < 4; e++) d[e] = (uint8_t)(v >> (8 * e));
     }
 }
 
@@ -164,53 +170,93 @@ __global__ __launch_bounds__(256) void k_mc(DBatch B)
     if (wv >= B.pre_inter[B.njobs]) return;
     const uint32_t j = find_job(B.pre_inter, B.njobs, wv);
     const DJob &J = B.jobs[j];
-    const DBlock k = J.blocks[J.inter[wv - B.pre_inter[j]]];
-    const int bx = k.x, by = k.y, bw = k.bw, bh = k.bh;
-    const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
-    uint8_t *dy = J.out.y + (size_t)by * J.out.sy + bx;
-    uint8_t *du = J.out.u + (size_t)cby * J.out.suv + cbx;
-    uint8_t *dv = J.out.v + (size_t)cby * J.out.suv + cbx;
-    const DFrame *R = k.ref < 8 ? &J.ref[k.ref] : nullptr;
-    if (!R || !R->y) { /* missing reference: flat 127 (:258-263, :328-333) */
-        fill_plane(dy, J.out.sy, bw, bh, 127, lane);
-        fill_plane(du, J.out.suv, cbw, cbh, 127, lane);
-        fill_plane(dv, J.out.suv, cbw, cbh, 127, lane);
-        return;
+    const uint32_t l = wv - B.pre_inter[j];
+    uint32_t lo = 0, hi = J.n_inter;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (J.mc_pre[mid] <= l) lo = mid; else hi = mid;
     }
-    mc_plane(R->y, R->sy, (int)R->w, (int)R->h, dy, J.out.sy, bx, by, bw, bh,
-             k.mvx, k.mvy, lane);
-    /* chroma: MV halved toward zero, bounds from half the luma size */
-    const int cmx = k.mvx / 2, cmy = k.mvy / 2;
-    const int rw = (int)R->w / 2, rh = (int)R->h / 2;
-    mc_plane(R->u, R->suv, rw, rh, du, J.out.suv, cbx, cby, cbw, cbh, cmx, cmy, lane);
-    mc_plane(R->v, R->suv, rw, rh, dv, J.out.suv, cbx, cby, cbw, cbh, cmx, cmy, lane);
+    const DBlock k = J.blocks[J.inter[lo]];
+    const int bx = k.x, by = k.y, bw = k.bw, bh = k.bh;
+    const int rows = (int)cvp9_mc_rows((uint32_t)bw);
+    const int r0 = (int)(l - J.mc_pre[lo]) * rows;
+    const int r1 = min(r0 + rows, bh);
+    const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1;
+    const int cr0 = (r0 + 1) >> 1, cr1 = (r1 + 1) >> 1; /* chroma rows under [r0, r1) */
+    const int lanes_per_row = 64 / rows;                 /* power of two >= quads per row */
+    const int nq = (bw + 3) >> 2, ncq = (cbw + 3) >> 2;
+    const DFrame *R = k.ref < 8 ? &J.ref[k.ref] : nullptr;
+    const bool missing = !R || !R->y; /* empty slot: flat 127 (:258-263, :328-333) */
+
+    /* luma */
+    {
+        const int rr = lane / lanes_per_row, q = lane - rr * lanes_per_row;
+        const int r = r0 + rr;
+        if (q < nq && r < r1) {
+            uint32_t v = 0x7f7f7f7fu;
+            if (!missing) {
+                const int px8 = bx * 8 + k.mvx, py8 = by * 8 + k.mvy;
+                v = mc_quad(R->y, R->sy, (int)R->w, (int)R->h, px8 >> 3, py8 >> 3, px8 & 7,
+                            py8 & 7, 4 * q, r);
+            }
+            store_quad(J.out.y + (size_t)(by + r) * J.out.sy + bx + 4 * q, v, bw - 4 * q);
+        }
+    }
+    /* chroma: MV halved toward zero, bounds from half the reference luma size */
+    {
+        const int crows = cr1 - cr0;
+        const int cmx = k.mvx / 2, cmy = k.mvy / 2;
+        for (int i = lane; i < 2 * crows * ncq; i += 64) {
+            const int p = i / (crows * ncq);
+            const int t = i - p * crows * ncq;
+            const int cr = cr0 + t / ncq, cq = t - (t / ncq) * ncq;
+            uint32_t v = 0x7f7f7f7fu;
+            if (!missing) {
+                const int px8 = cbx * 8 + cmx, py8 = cby * 8 + cmy;
+                v = mc_quad(p ? R->v : R->u, R->suv, (int)R->w / 2, (int)R->h / 2, px8 >> 3,
+                            py8 >> 3, px8 & 7, py8 & 7, 4 * cq, cr);
+            }
+            uint8_t *pl = p ? J.out.v : J.out.u;
+            store_quad(pl + (size_t)(cby + cr) * J.out.suv + cbx + 4 * cq, v, cbw - 4 * cq);
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
 /* FP64 inverse DCT + residual add (vp9_reconstruct.c:13-60, :336-363).      */
 /*                                                                          */
-/* One wave transforms G = 64/N transforms of size N at once:               */
-/*   pass 1: lane (t, j) holds column j of transform t and accumulates      */
-/*           temp[i][j] = sum_k M[k][i] * in[k][j] for all i (k ascending); */
-/*           M[k][i] is wave-uniform (scalar loads from constant memory).   */
-/*   temp goes through LDS (row pitch N+1 doubles) to be read by rows.      */
-/*   pass 2: lane (t, r) accumulates out[r][c] = sum_k temp[r][k] * M[k][c].*/
+/* One workgroup (4 waves) transforms G = 64/N transforms of size N. In     */
+/* every wave lane (t, j) stands for column j of transform t; wave g owns   */
+/* rows [g*R, g*R+R) in pass 1 and columns [g*R, g*R+R) in pass 2, R = N/4. */
+/*   pass 1: temp[i][j] = sum_k M[k][i] * in[k][j] for the wave's rows i,   */
+/*           k ascending; M[k][i] is wave-uniform (scalar loads).           */
+/*   temp is shared through LDS (row pitch N+1 doubles).                    */
+/*   pass 2: lane (t, r): out[r][c] = sum_k temp[r][k] * M[k][c] for the    */
+/*           wave's columns c.                                              */
+/* Every product and sum is rounded separately, in the reference's order.   */
 /* A k-step is skipped when its input row (pass 1) or column (pass 2) is    */
-/* zero in every transform of the wave. Adding a product with an exact      */
+/* zero in every transform of the group. Adding a product with an exact     */
 /* zero operand leaves a nonzero sum unchanged and can only flip the sign   */
 /* of a zero sum, which the final round-half-away conversion ignores, so    */
 /* any superset of the nonzero terms taken in ascending k is bit-exact.     */
 
-template <int N>
-__device__ __forceinline__ void idct_add_wave(const int16_t *coef, uint8_t *dst,
-                                              uint32_t stride, int16_t *rout, double *lds,
-                                              int lane)
+__device__ __forceinline__ int round_clamp16(double v)
 {
+    const int32_t r = (int32_t)__dadd_rn(v, v >= 0.0 ? 0.5 : -0.5);
+    return clampi(r, -32768, 32767);
+}
+
+template <int N>
+__device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, uint32_t stride,
+                                           int16_t *rout, double *lds, int wave, int lane)
+{
+    constexpr int R = N / 4;
     const int t = lane / N;
     const int j = lane % N;
-    double acc[N];
+    const int i0 = wave * R;
+    double acc[R];
 #pragma unroll
-    for (int i = 0; i < N; i++) acc[i] = 0.0;
+    for (int i = 0; i < R; i++) acc[i] = 0.0;
     bool col_nz = false;
 
 #pragma unroll
@@ -220,59 +266,56 @@ __device__ __forceinline__ void idct_add_wave(const int16_t *coef, uint8_t *dst,
         if (wave_ballot(x != 0)) {
             const double xd = (double)x;
 #pragma unroll
-            for (int i = 0; i < N; i++) acc[i] = __dadd_rn(acc[i], __dmul_rn(mat<N>(k * N + i), xd));
+            for (int i = 0; i < R; i++)
+                acc[i] = __dadd_rn(acc[i], __dmul_rn(mat<N>(k * N + i0 + i), xd));
         }
     }
     double *tt = lds + t * (N * (N + 1));
 #pragma unroll
-    for (int i = 0; i < N; i++) tt[i * (N + 1) + j] = acc[i];
+    for (int i = 0; i < R; i++) tt[(i0 + i) * (N + 1) + j] = acc[i];
 
-    /* nonzero input columns over all transforms of the wave, folded to N bits */
+    /* nonzero input columns over all transforms of the group, folded to N bits */
     uint64_t m = wave_ballot(col_nz);
 #pragma unroll
     for (int s = 32; s >= N; s >>= 1) m |= m >> s;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __syncthreads();
 
     const int r = j;
 #pragma unroll
-    for (int c = 0; c < N; c++) acc[c] = 0.0;
+    for (int c = 0; c < R; c++) acc[c] = 0.0;
 #pragma unroll
     for (int k = 0; k < N; k++) {
         if ((m >> k) & 1) {
             const double u = tt[r * (N + 1) + k];
 #pragma unroll
-            for (int c = 0; c < N; c++) acc[c] = __dadd_rn(acc[c], __dmul_rn(u, mat<N>(k * N + c)));
+            for (int c = 0; c < R; c++)
+                acc[c] = __dadd_rn(acc[c], __dmul_rn(u, mat<N>(k * N + i0 + c)));
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
 
     if (coef && rout) {
         /* intra transform: keep the int16 residual for the intra stage */
-        int16_t *o = rout + r * N;
+        int16_t *o = rout + r * N + i0;
 #pragma unroll
-        for (int c = 0; c < N; c++) {
-            const double v = acc[c];
-            const int32_t rr = (int32_t)__dadd_rn(v, v >= 0.0 ? 0.5 : -0.5);
-            o[c] = (int16_t)clampi(rr, -32768, 32767);
-        }
+        for (int c = 0; c < R; c++) o[c] = (int16_t)round_clamp16(acc[c]);
     } else if (coef) {
-        uint8_t *row = dst + (size_t)r * stride;
+        uint8_t *row = dst + (size_t)r * stride + i0;
+        if constexpr (R >= 4) {
 #pragma unroll
-        for (int c4 = 0; c4 < N; c4 += 4) {
-            uint32_t px = *(const uint32_t *)(row + c4);
-            uint32_t o = 0;
+            for (int c4 = 0; c4 < R; c4 += 4) {
+                const uint32_t px = *(const uint32_t *)(row + c4);
+                uint32_t o = 0;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const double v = acc[c4 + e];
-                int32_t rr = (int32_t)__dadd_rn(v, v >= 0.0 ? 0.5 : -0.5);
-                rr = clampi(rr, -32768, 32767);
-                const int p = (int)((px >> (8 * e)) & 0xff);
-                o |= (uint32_t)clampi(p + rr, 0, 255) << (8 * e);
+                for (int e = 0; e < 4; e++) {
+                    const int p = (int)((px >> (8 * e)) & 0xff);
+                    o |= (uint32_t)clampi(p + round_clamp16(acc[c4 + e]), 0, 255) << (8 * e);
+                }
+                *(uint32_t *)(row + c4) = o;
             }
-            *(uint32_t *)(row + c4) = o;
+        } else {
+#pragma unroll
+            for (int c = 0; c < R; c++)
+                row[c] = (uint8_t)clampi((int)row[c] + round_clamp16(acc[c]), 0, 255);
         }
     }
 }
@@ -308,19 +351,18 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
 {
     constexpr int C = N == 4 ? 0 : N == 8 ? 1 : N == 16 ? 2 : 3;
     constexpr int G = 64 / N;
-    __shared__ double lds[4][G * N * (N + 1)];
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    __shared__ double lds[G * N * (N + 1)];
+    const uint32_t grp = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t *pre = B.pre_grp[C];
-    if (wv >= pre[B.njobs]) return;
-    const uint32_t j = find_job(pre, B.njobs, wv);
+    const uint32_t j = find_job(pre, B.njobs, grp);
     const DJob &J = B.jobs[j];
-    const uint32_t txi = (wv - pre[j]) * G + lane / N;
+    const uint32_t txi = (grp - pre[j]) * G + lane / N;
     const int16_t *coef;
     uint8_t *dst;
     int16_t *rout;
     locate_tx<N>(J, C, txi, coef, dst, rout);
-    idct_add_wave<N>(coef, dst, J.out.sy, rout, lds[threadIdx.x >> 6], lane);
+    idct_group<N>(coef, dst, J.out.sy, rout, lds, wave, lane);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -673,16 +715,16 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         if (n.inter) hipLaunchKernelGGL(k_mc, dim3(div_up(n.inter, 4)), dim3(256), 0, s, b);
         break;
     case STAGE_RES4:
-        if (n.grp[0]) hipLaunchKernelGGL(k_residual<4>, dim3(div_up(n.grp[0], 4)), dim3(256), 0, s, b);
+        if (n.grp[0]) hipLaunchKernelGGL(k_residual<4>, dim3(n.grp[0]), dim3(256), 0, s, b);
         break;
     case STAGE_RES8:
-        if (n.grp[1]) hipLaunchKernelGGL(k_residual<8>, dim3(div_up(n.grp[1], 4)), dim3(256), 0, s, b);
+        if (n.grp[1]) hipLaunchKernelGGL(k_residual<8>, dim3(n.grp[1]), dim3(256), 0, s, b);
         break;
     case STAGE_RES16:
-        if (n.grp[2]) hipLaunchKernelGGL(k_residual<16>, dim3(div_up(n.grp[2], 4)), dim3(256), 0, s, b);
+        if (n.grp[2]) hipLaunchKernelGGL(k_residual<16>, dim3(n.grp[2]), dim3(256), 0, s, b);
         break;
     case STAGE_RES32:
-        if (n.grp[3]) hipLaunchKernelGGL(k_residual<32>, dim3(div_up(n.grp[3], 4)), dim3(256), 0, s, b);
+        if (n.grp[3]) hipLaunchKernelGGL(k_residual<32>, dim3(n.grp[3]), dim3(256), 0, s, b);
         break;
     case STAGE_INTRA:
         if (n.intra) {
