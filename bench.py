@@ -105,6 +105,13 @@ def single_stream(P, args):
     frames = gen_frames(P, [(1, W, H, 20250, i) for i in range(F)])
     eng = P.Engine(0)
     preps = [eng.prepare(f) for f in frames]
+    tune = None
+    if not args.no_tune:  # launch shape for this workload, on a tuning stream
+        tsid = eng.open_stream()
+        eng.run([tsid], [preps[0]])
+        tune = eng.autotune([tsid], [preps[1]], iters=5).as_dict()
+        eng.close_stream(tsid)
+        log("single-stream tune:", tune)
     sid = eng.open_stream()
     order = [(args.warmup + k) % F for k in range(args.steps)]
     for k in range(args.warmup):
@@ -147,7 +154,7 @@ def single_stream(P, args):
     algo = sum(frames[i].algo_bytes() for i in order)
     eng.close()
     return {"clean_s": clean_s, "timed_s": timed_s, "stats": per_stage, "algo_bytes": algo,
-            "frames": frames}
+            "frames": frames, "tune": tune}
 
 
 def multi_stream(P, args):
@@ -165,6 +172,14 @@ def multi_stream(P, args):
     eng = P.Engine(0)
     preps = [eng.prepare(f) for f in frames]
     del frames
+    tune = None
+    if not args.no_tune:  # tune on streams opened for it, then drop them
+        tsids = [eng.open_stream() for _ in layout]
+        eng.run(tsids, [preps[s * F] for s in range(len(tsids))])
+        tune = eng.autotune(tsids, [preps[s * F + 1] for s in range(len(tsids))], iters=3).as_dict()
+        for t in tsids:
+            eng.close_stream(t)
+        log("multi-stream tune:", tune)
     sids = [eng.open_stream() for _ in layout]
     steps = args.msteps
 
@@ -190,7 +205,7 @@ def multi_stream(P, args):
              for n, (ms, c) in eng.stage_stats(reset=True).items() if c}
     eng.set_timing(False)
     eng.close()
-    return {"streams": len(layout), "steps": steps, "seconds": dt, "stages": stats,
+    return {"streams": len(layout), "steps": steps, "seconds": dt, "stages": stats, "tune": tune,
             "frames_per_s": len(layout) * steps / dt,
             "p1080_equiv_per_s": px / (1920 * 1080) / dt,
             "ms_per_batch": 1e3 * dt / steps}
@@ -223,6 +238,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=100)
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="only run the calibration copy kernel (for counter runs)")
     args = ap.parse_args()
@@ -284,6 +300,7 @@ def main():
         "timed_with_events_fps": round(args.steps / ss["timed_s"], 2),
         "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1),
         "stages": stats,
+        "tune": ss["tune"],
         "roofline": roof,
         "copy_peak_GBps": round(copy_gbps, 1),
         "multi_stream": None if not multi else {
@@ -293,6 +310,7 @@ def main():
             "p1080_equiv_frames_per_s": round(multi["p1080_equiv_per_s"], 1),
             "ms_per_batch": round(multi["ms_per_batch"], 4),
             "stages": multi["stages"],
+            "tune": multi["tune"],
             "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2)},
         "cpu_baseline": cpu,
     }

@@ -127,6 +127,16 @@ class ExportBuffer(C.Structure):
     ]
 
 
+class Tune(C.Structure):
+    """cvp9_tune_t (include/cvp9_hip/engine.h)."""
+    _fields_ = [("res_split", C.c_uint8 * 4), ("intra_lanes", C.c_uint16),
+                ("intra_workers", C.c_uint32)]
+
+    def as_dict(self):
+        return {"res_split": list(self.res_split), "intra_lanes": self.intra_lanes,
+                "intra_workers": self.intra_workers}
+
+
 class SynthParams(C.Structure):
     """cvp9_synth_params_t (include/cvp9_hip/synth.h)."""
     _fields_ = [
@@ -174,6 +184,11 @@ SIGNATURES = {
     "cvp9_engine_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint8), C.c_uint64,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "cvp9_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_get_tune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cvp9_engine_set_tune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cvp9_engine_compute_units": (C.c_int, [C.c_void_p]),
+    "cvp9_engine_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_void_p), C.c_int, C.c_void_p]),
     "cvp9_engine_stage_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32), C.c_int]),
     # testing.h
@@ -405,6 +420,23 @@ class Engine:
                                      out.size, C.byref(ww), C.byref(hh)), "cvp9_engine_read")
         assert (ww.value, hh.value) == (w, h)
         return out
+
+    def get_tune(self):
+        t = Tune()
+        check(lib().cvp9_engine_get_tune(self.e, C.byref(t)), "get_tune")
+        return t
+
+    def set_tune(self, res_split, intra_lanes, intra_workers):
+        t = Tune((C.c_uint8 * 4)(*res_split), intra_lanes, intra_workers)
+        check(lib().cvp9_engine_set_tune(self.e, C.byref(t)), "set_tune")
+
+    def autotune(self, streams, preps, iters=3):
+        n = len(streams)
+        s = (C.c_int * n)(*streams)
+        p = (C.c_void_p * n)(*[x.value if isinstance(x, C.c_void_p) else x for x in preps])
+        t = Tune()
+        check(lib().cvp9_engine_autotune(self.e, n, s, p, iters, C.byref(t)), "autotune")
+        return t
 
     def set_timing(self, on):
         check(lib().cvp9_engine_set_timing(self.e, 1 if on else 0), "set_timing")

@@ -50,7 +50,7 @@ struct Backend {
     uint8_t *pending_target = nullptr;   /* device address of the armed NV12 target */
     uint64_t pending_target_size = 0;
     uint32_t debug = 0;
-    uint32_t intra_wgs = 512;
+    Tune tune = cvp9_default_tune(256);
     bool dmabuf = false;
 };
 
@@ -158,7 +158,7 @@ cvp9_err_t hip_backend_init(void **ctx)
     b->device = dev;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-        b->intra_wgs = 2u * (uint32_t)prop.multiProcessorCount;
+        b->tune = cvp9_default_tune(prop.multiProcessorCount);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return CVP9_ERR_GPU;
@@ -238,7 +238,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     DJob job = b->st->make_job(L, s.stage_dev, out, b->pending_target);
     if (b->debug & CVP9_HIP_DEBUG_NO_DEBLOCK) job.lf_thr = 0;
     BatchBuilder bb;
-    bb.build(&job, 1, s.stage_host + batch_off, s.stage_dev + batch_off, b->intra_wgs, kSpinLimit);
+    bb.build(&job, 1, s.stage_host + batch_off, s.stage_dev + batch_off, b->tune, kSpinLimit);
 
     if ((e = hipMemcpyAsync(s.stage_dev, s.stage_host, batch_off + bb.used(), hipMemcpyHostToDevice,
                             b->stream)) != hipSuccess)

@@ -431,7 +431,7 @@ size_t BatchBuilder::bytes_for(uint32_t njobs)
 }
 
 void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_t *dev,
-                         uint32_t intra_wgs, uint32_t spin_limit)
+                         const Tune &tune, uint32_t spin_limit)
 {
     memcpy(host, jobs, (size_t)njobs * sizeof(DJob));
     const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
@@ -477,7 +477,7 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
     n_.intra = pre[P_INTRA * stride + njobs];
     n_.lf = pre[P_LF * stride + njobs];
     n_.nv12 = pre[P_NV12 * stride + njobs];
-    n_.intra_wgs = intra_wgs;
+    n_.tune = tune;
     used_ = tk_off + 256;
 }
 

@@ -95,7 +95,7 @@ public:
     static size_t bytes_for(uint32_t njobs);
     /* host/dev point to bytes_for(njobs) bytes each */
     void build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_t *dev,
-               uint32_t intra_wgs, uint32_t spin_limit);
+               const Tune &tune, uint32_t spin_limit);
     size_t used() const { return used_; }
     const DBatch &batch() const { return b_; }
     const StageCounts &counts() const { return n_; }

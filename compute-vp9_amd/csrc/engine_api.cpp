@@ -91,7 +91,8 @@ struct cvp9_engine {
     std::vector<StreamState *> streams;
     BatchSlot slots[kBatchSlots];
     int next = 0;
-    uint32_t intra_wgs = 512;
+    Tune tune = cvp9_default_tune(256);
+    int cus = 256;
     bool timing = false;
     std::vector<TimedLaunch> timed;
     std::vector<hipEvent_t> free_events;
@@ -141,7 +142,10 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
     hipSetDevice(device);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-        e->intra_wgs = 2u * (uint32_t)prop.multiProcessorCount;
+    {
+        e->cus = prop.multiProcessorCount;
+        e->tune = cvp9_default_tune(prop.multiProcessorCount);
+    }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return CVP9_ERR_GPU;
@@ -286,7 +290,7 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
         bs.cap = need;
     }
     BatchBuilder bb;
-    bb.build(jobs.data(), (uint32_t)n, bs.host, bs.dev, e->intra_wgs, kSpinLimit);
+    bb.build(jobs.data(), (uint32_t)n, bs.host, bs.dev, e->tune, kSpinLimit);
     if ((he = hipMemcpyAsync(bs.dev, bs.host, bb.used(), hipMemcpyHostToDevice, e->stream)) != hipSuccess)
         return to_err(he);
     for (int st = 0; st < STAGE_COUNT; st++) {
@@ -349,6 +353,91 @@ cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int id, uint8_t *dst, uint64_t cap
         hipMemcpy(&v, e->streams[id]->err_word(), 4, hipMemcpyDeviceToHost) != hipSuccess)
         return CVP9_ERR_GPU;
     return v ? CVP9_ERR_GPU : CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t)
+{
+    if (!e || !t) return CVP9_ERR_INVALID_DATA;
+    for (int i = 0; i < 4; i++) t->res_split[i] = e->tune.res_split[i];
+    t->intra_lanes = e->tune.intra_lanes;
+    t->intra_workers = e->tune.intra_workers;
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_set_tune(cvp9_engine_t *e, const cvp9_tune_t *t)
+{
+    if (!e || !t) return CVP9_ERR_INVALID_DATA;
+    for (int i = 0; i < 4; i++)
+        if (t->res_split[i] != 1 && t->res_split[i] != 2 && t->res_split[i] != 4)
+            return CVP9_ERR_INVALID_DATA;
+    if (t->intra_lanes != 64 && t->intra_lanes != 256) return CVP9_ERR_INVALID_DATA;
+    if (t->intra_workers < 1 || t->intra_workers > 65536) return CVP9_ERR_INVALID_DATA;
+    for (int i = 0; i < 4; i++) e->tune.res_split[i] = t->res_split[i];
+    e->tune.intra_lanes = t->intra_lanes;
+    e->tune.intra_workers = t->intra_workers;
+    return CVP9_OK;
+}
+
+int cvp9_engine_compute_units(cvp9_engine_t *e)
+{
+    return e ? e->cus : 0;
+}
+
+cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
+                                cvp9_prep_t *const *preps, int iters, cvp9_tune_t *best)
+{
+    if (!e || n <= 0 || !streams || !preps || iters < 1) return CVP9_ERR_INVALID_DATA;
+    const uint32_t cu = (uint32_t)(e->cus > 0 ? e->cus : 256);
+    /* each candidate run measures every stage at once; the transform split
+     * is uniform over sizes inside a run and chosen per size afterwards */
+    struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; };
+    const Cand cands[] = {
+        {1, 256, 2 * cu}, {2, 64, 8 * cu}, {4, 64, 16 * cu}, {1, 256, 4 * cu}, {2, 64, 4 * cu},
+    };
+    const bool was_timing = e->timing;
+    const Tune saved = e->tune;
+    double best_res[4] = {1e30, 1e30, 1e30, 1e30}, best_intra = 1e30;
+    Tune out = saved;
+    cvp9_err_t err = CVP9_OK;
+    cvp9_engine_stage_stats(e, nullptr, nullptr, 1);
+    for (const Cand &c : cands) {
+        for (int k = 0; k < 4; k++) e->tune.res_split[k] = c.split;
+        e->tune.intra_lanes = c.lanes;
+        e->tune.intra_workers = c.workers;
+        e->timing = false;
+        if ((err = cvp9_engine_run(e, n, streams, preps, nullptr)) != CVP9_OK) break; /* warm */
+        e->timing = true;
+        for (int i = 0; i < iters && err == CVP9_OK; i++)
+            err = cvp9_engine_run(e, n, streams, preps, nullptr);
+        if (err != CVP9_OK) break;
+        double ms[STAGE_COUNT];
+        uint32_t cnt[STAGE_COUNT];
+        if ((err = cvp9_engine_stage_stats(e, ms, cnt, 1)) != CVP9_OK) break;
+        for (int k = 0; k < 4; k++) {
+            const int st = STAGE_RES4 + k;
+            const double t = cnt[st] ? ms[st] / cnt[st] : 0.0;
+            if (cnt[st] && t < best_res[k]) {
+                best_res[k] = t;
+                out.res_split[k] = c.split;
+            }
+        }
+        if (cnt[STAGE_INTRA]) {
+            const double t = ms[STAGE_INTRA] / cnt[STAGE_INTRA];
+            if (t < best_intra) {
+                best_intra = t;
+                out.intra_lanes = c.lanes;
+                out.intra_workers = c.workers;
+            }
+        }
+    }
+    e->timing = was_timing;
+    e->tune = err == CVP9_OK ? out : saved;
+    if (err == CVP9_OK && best) {
+        for (int k = 0; k < 4; k++) best->res_split[k] = out.res_split[k];
+        best->intra_lanes = out.intra_lanes;
+        best->intra_workers = out.intra_workers;
+    }
+    return err;
 }
 
 cvp9_err_t cvp9_engine_set_timing(cvp9_engine_t *e, int on)

@@ -19,12 +19,22 @@ enum Stage {
     STAGE_COUNT
 };
 
+/* Launch shape knobs, chosen per workload class by the autotuner. */
+struct Tune {
+    uint8_t res_split[4];    /* waves sharing one transform group, N = 4,8,16,32: 1, 2 or 4 */
+    uint16_t intra_lanes;    /* threads per intra worker: 64 (one wave) or 256 */
+    uint32_t intra_workers;  /* persistent intra workers */
+};
+
 /* Total work items of each stage over the whole batch (the last element of
  * each prefix array in DBatch), plus launch parameters. */
 struct StageCounts {
     uint32_t fill, blocks, inter, grp[4], intra, lf, nv12;
-    uint32_t intra_wgs;   /* persistent workgroups for the intra stage */
+    Tune tune;
 };
+
+/* Default launch shape for a device with `cus` compute units. */
+Tune cvp9_default_tune(int cus);
 
 hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double *m16,
                                 const double *m32);

@@ -225,9 +225,9 @@ __global__ __launch_bounds__(256) void k_mc(DBatch B)
 /* ------------------------------------------------------------------------ */
 /* FP64 inverse DCT + residual add (vp9_reconstruct.c:13-60, :336-363).      */
 /*                                                                          */
-/* One workgroup (4 waves) transforms G = 64/N transforms of size N. In     */
+/* S waves (1, 2 or 4) transform G = 64/N transforms of size N together. In */
 /* every wave lane (t, j) stands for column j of transform t; wave g owns   */
-/* rows [g*R, g*R+R) in pass 1 and columns [g*R, g*R+R) in pass 2, R = N/4. */
+/* rows [g*R, g*R+R) in pass 1 and columns [g*R, g*R+R) in pass 2, R = N/S. */
 /*   pass 1: temp[i][j] = sum_k M[k][i] * in[k][j] for the wave's rows i,   */
 /*           k ascending; M[k][i] is wave-uniform (scalar loads).           */
 /*   temp is shared through LDS (row pitch N+1 doubles).                    */
@@ -246,11 +246,11 @@ __device__ __forceinline__ int round_clamp16(double v)
     return clampi(r, -32768, 32767);
 }
 
-template <int N>
+template <int N, int S>
 __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, uint32_t stride,
                                            int16_t *rout, double *lds, int wave, int lane)
 {
-    constexpr int R = N / 4;
+    constexpr int R = N / S;
     const int t = lane / N;
     const int j = lane % N;
     const int i0 = wave * R;
@@ -346,23 +346,30 @@ __device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
         dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
 }
 
-template <int N>
+template <int N, int S>
 __global__ __launch_bounds__(256) void k_residual(DBatch B)
 {
     constexpr int C = N == 4 ? 0 : N == 8 ? 1 : N == 16 ? 2 : 3;
     constexpr int G = 64 / N;
-    __shared__ double lds[G * N * (N + 1)];
-    const uint32_t grp = blockIdx.x;
+    constexpr int GPW = 4 / S; /* transform groups per workgroup */
+    __shared__ double lds[GPW][G * N * (N + 1)];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int gi = wave / S, sub = wave - gi * S;
+    const uint32_t grp = blockIdx.x * GPW + gi;
     const uint32_t *pre = B.pre_grp[C];
-    const uint32_t j = find_job(pre, B.njobs, grp);
-    const DJob &J = B.jobs[j];
-    const uint32_t txi = (grp - pre[j]) * G + lane / N;
-    const int16_t *coef;
-    uint8_t *dst;
-    int16_t *rout;
-    locate_tx<N>(J, C, txi, coef, dst, rout);
-    idct_group<N>(coef, dst, J.out.sy, rout, lds, wave, lane);
+    /* every wave reaches the barrier inside idct_group: a group past the
+     * end runs with no transform */
+    const int16_t *coef = nullptr;
+    uint8_t *dst = nullptr;
+    int16_t *rout = nullptr;
+    uint32_t stride = 0;
+    if (grp < pre[B.njobs]) {
+        const uint32_t j = find_job(pre, B.njobs, grp);
+        const DJob &J = B.jobs[j];
+        stride = J.out.sy;
+        locate_tx<N>(J, C, (grp - pre[j]) * G + lane / N, coef, dst, rout);
+    }
+    idct_group<N, S>(coef, dst, stride, rout, lds[gi], sub, lane);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -440,25 +447,47 @@ __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc
     return clampi(a + l - tl, 0, 255);
 }
 
+/* Synchronises the threads of one intra worker: a whole workgroup, or a
+ * single wave (whose LDS accesses execute in order; the fences keep the
+ * compiler from reordering them). */
+template <int W>
+__device__ __forceinline__ void worker_sync()
+{
+    if constexpr (W == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+struct IntraShared {
+    uint8_t nb[3][2][64];   /* above / left samples per plane */
+    uint8_t edge[3][2][64]; /* final bottom row / right column */
+    uint32_t sum[3];
+    uint32_t ticket;
+    uint8_t tl[3];
+};
+
+/* W threads per worker: 256 (one workgroup per block) or 64 (one wave per
+ * block, four independent workers per workgroup). */
+template <int W>
 __global__ __launch_bounds__(256) void k_intra(DBatch B)
 {
-    __shared__ uint32_t s_ticket;
-    __shared__ uint8_t s_nb[3][2][64];   /* above / left samples per plane */
-    __shared__ uint8_t s_tl[3];
-    __shared__ uint32_t s_sum[3];
-    __shared__ uint8_t s_edge[3][2][64]; /* final bottom row / right column */
-    const int tid = threadIdx.x;
+    __shared__ IntraShared shm[256 / W];
+    IntraShared &S = shm[threadIdx.x / W];
+    const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
 
-    /* The ticket is broadcast through LDS and made wave-uniform with
-     * readfirstlane, so the loop condition is a scalar branch. With an LDS
-     * value as the exit test the compiler treats the exit as divergent and
-     * restructures the loop so that thread 0's ticket fetch runs outside
-     * the loop holding the barriers: the other lanes then repeat one ticket
-     * forever. */
-    if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
-    __syncthreads();
-    uint32_t tk = __builtin_amdgcn_readfirstlane(s_ticket);
+    /* The ticket is made wave-uniform with readfirstlane, so the loop
+     * condition is a scalar branch. With an LDS value as the exit test the
+     * compiler treats the exit as divergent and restructures the loop so
+     * that thread 0's ticket fetch runs outside the loop holding the
+     * barriers: the other lanes then repeat one ticket forever. */
+    if (tid == 0) S.ticket = atomicAdd(B.ticket, 1u);
+    worker_sync<W>();
+    uint32_t tk = __builtin_amdgcn_readfirstlane(S.ticket);
     while (tk < total) {
         const uint32_t j = find_job(B.pre_intra, B.njobs, tk);
         const DJob &J = B.jobs[j];
@@ -468,9 +497,9 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
         const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
 
         /* 1. neighbours: 127 outside the frame, else the owner rule */
-        if (tid < 3) s_sum[tid] = 0;
+        if (tid < 3) S.sum[tid] = 0;
         const int nY = bw + bh + 1, nC = cbw + cbh + 1;
-        for (int i = tid; i < nY + 2 * nC; i += 256) {
+        for (int i = tid; i < nY + 2 * nC; i += W) {
             int p, e;
             if (i < nY) { p = 0; e = i; }
             else if (i < nY + nC) { p = 1; e = i - nY; }
@@ -478,41 +507,41 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
             const int px = p ? cbx : x, py = p ? cby : y;
             const int w = p ? cbw : bw, h = p ? cbh : bh;
             if (e < w) {
-                s_nb[p][0][e] = (uint8_t)(py > 0 ? nb_sample(J, p, px + e, py - 1, b, B.spin_limit) : 127);
+                S.nb[p][0][e] = (uint8_t)(py > 0 ? nb_sample(J, p, px + e, py - 1, b, B.spin_limit) : 127);
             } else if (e < w + h) {
                 const int r = e - w;
-                s_nb[p][1][r] = (uint8_t)(px > 0 ? nb_sample(J, p, px - 1, py + r, b, B.spin_limit) : 127);
+                S.nb[p][1][r] = (uint8_t)(px > 0 ? nb_sample(J, p, px - 1, py + r, b, B.spin_limit) : 127);
             } else {
-                s_tl[p] = (uint8_t)((px > 0 && py > 0) ? nb_sample(J, p, px - 1, py - 1, b, B.spin_limit)
+                S.tl[p] = (uint8_t)((px > 0 && py > 0) ? nb_sample(J, p, px - 1, py - 1, b, B.spin_limit)
                                                        : 127);
             }
         }
-        __syncthreads();
+        worker_sync<W>();
 
         /* 2. DC sums over w above + h left samples */
-        for (int i = tid; i < 3 * 128; i += 256) {
+        for (int i = tid; i < 3 * 128; i += W) {
             const int p = i >> 7, e = i & 127;
             if ((p == 0 ? k.ymode : k.uvmode) != 0) continue;
             const int w = p ? cbw : bw, h = p ? cbh : bh;
             uint32_t v = 0;
-            if (e < 64) { if (e < w) v = s_nb[p][0][e]; }
-            else if (e - 64 < h) v = s_nb[p][1][e - 64];
-            if (v) atomicAdd(&s_sum[p], v);
+            if (e < 64) { if (e < w) v = S.nb[p][0][e]; }
+            else if (e - 64 < h) v = S.nb[p][1][e - 64];
+            if (v) atomicAdd(&S.sum[p], v);
         }
-        __syncthreads();
+        worker_sync<W>();
 
         /* 3. prediction + residual, luma 4 samples per thread */
         {
-            const int dc = (int)(uint8_t)((s_sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
-            const int tl = s_tl[0], mode = k.ymode;
+            const int dc = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
+            const int tl = S.tl[0], mode = k.ymode;
             const int lt = k.tx + 2, T = 1 << lt;
             const bool res = !(k.flags & DB_SKIP);
             const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
             const int nq = (bw + 3) >> 2;
             uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
-            for (int i = tid; i < nq * bh; i += 256) {
+            for (int i = tid; i < nq * bh; i += W) {
                 const int r = i / nq, c0 = (i - r * nq) * 4;
-                const int l = s_nb[0][1][r];
+                const int l = S.nb[0][1][r];
                 const int16_t *rs = nullptr;
                 if (r < rrows && c0 < rcols)
                     rs = J.resid + k.coeff +
@@ -523,40 +552,34 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
                 for (int e = 0; e < 4; e++) {
                     const int c = c0 + e;
                     if (c < bw) {
-                        int v = intra_pred(mode, s_nb[0][0][c], l, tl, dc);
+                        int v = intra_pred(mode, S.nb[0][0][c], l, tl, dc);
                         if (rs) v = clampi(v + rs[e], 0, 255);
                         word |= (uint32_t)v << (8 * e);
-                        if (r == bh - 1) s_edge[0][0][c] = (uint8_t)v;
-                        if (c == bw - 1) s_edge[0][1][r] = (uint8_t)v;
+                        if (r == bh - 1) S.edge[0][0][c] = (uint8_t)v;
+                        if (c == bw - 1) S.edge[0][1][r] = (uint8_t)v;
                     }
                 }
-                uint8_t *d = Y + (size_t)r * J.out.sy + c0;
-                if (c0 + 3 < bw) {
-                    *(uint32_t *)d = word;
-                } else {
-                    for (int e = 0; c0 + e < bw; e++) d[e] = (uint8_t)(word >> (8 * e));
-                }
+                store_quad(Y + (size_t)r * J.out.sy + c0, word, bw - c0);
             }
             /* chroma: prediction only (the reference adds no chroma residual) */
-            for (int i = tid; i < 2 * cbw * cbh; i += 256) {
+            for (int i = tid; i < 2 * cbw * cbh; i += W) {
                 const int p = i < cbw * cbh ? 1 : 2;
                 const int q = i - (p - 1) * cbw * cbh;
                 const int r = q / cbw, c = q - r * cbw;
-                const int dcc = (int)(uint8_t)((s_sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
+                const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
                                                (uint32_t)(cbw + cbh));
-                const int v = intra_pred(k.uvmode, s_nb[p][0][c], s_nb[p][1][r], s_tl[p], dcc);
+                const int v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
                 uint8_t *pl = p == 1 ? J.out.u : J.out.v;
                 pl[(size_t)(cby + r) * J.out.suv + cbx + c] = (uint8_t)v;
-                if (r == cbh - 1) s_edge[p][0][c] = (uint8_t)v;
-                if (c == cbw - 1) s_edge[p][1][r] = (uint8_t)v;
+                if (r == cbh - 1) S.edge[p][0][c] = (uint8_t)v;
+                if (c == cbw - 1) S.edge[p][1][r] = (uint8_t)v;
             }
         }
-        __syncthreads();
+        worker_sync<W>();
 
         /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
          *    V bottom, V right */
-        if (tid < CVP9_EDGE_WORDS) {
-            const int t = tid;
+        for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
             int p, dir, wi, len;
             if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
             else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
@@ -564,16 +587,16 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
                 uint64_t d = (uint64_t)J.epoch << 32;
 #pragma unroll
                 for (int e = 0; e < 4; e++)
-                    if (4 * wi + e < len) d |= (uint64_t)s_edge[p][dir][4 * wi + e] << (8 * e);
+                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
                 __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
 
-        /* next ticket; the barrier also orders LDS reuse */
-        if (tid == 0) s_ticket = atomicAdd(B.ticket, 1u);
-        __syncthreads();
-        tk = __builtin_amdgcn_readfirstlane(s_ticket);
+        /* next ticket; the sync also orders LDS reuse */
+        if (tid == 0) S.ticket = atomicAdd(B.ticket, 1u);
+        worker_sync<W>();
+        tk = __builtin_amdgcn_readfirstlane(S.ticket);
     }
 }
 
@@ -702,6 +725,31 @@ hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double
 
 static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+Tune cvp9_default_tune(int cus)
+{
+    Tune t;
+    t.res_split[0] = 1;
+    t.res_split[1] = 1;
+    t.res_split[2] = 2;
+    t.res_split[3] = 4;
+    t.intra_lanes = 256;
+    t.intra_workers = 2u * (uint32_t)(cus > 0 ? cus : 256);
+    return t;
+}
+
+template <int N>
+static hipError_t launch_residual(const DBatch &b, const StageCounts &n, int c, hipStream_t s)
+{
+    const uint32_t g = n.grp[c];
+    if (!g) return hipSuccess;
+    switch (n.tune.res_split[c]) {
+    case 1: hipLaunchKernelGGL((k_residual<N, 1>), dim3(div_up(g, 4)), dim3(256), 0, s, b); break;
+    case 2: hipLaunchKernelGGL((k_residual<N, 2>), dim3(div_up(g, 2)), dim3(256), 0, s, b); break;
+    default: hipLaunchKernelGGL((k_residual<N, 4>), dim3(g), dim3(256), 0, s, b); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s)
 {
     switch (stage) {
@@ -715,21 +763,20 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         if (n.inter) hipLaunchKernelGGL(k_mc, dim3(div_up(n.inter, 4)), dim3(256), 0, s, b);
         break;
     case STAGE_RES4:
-        if (n.grp[0]) hipLaunchKernelGGL(k_residual<4>, dim3(n.grp[0]), dim3(256), 0, s, b);
-        break;
+        return launch_residual<4>(b, n, 0, s);
     case STAGE_RES8:
-        if (n.grp[1]) hipLaunchKernelGGL(k_residual<8>, dim3(n.grp[1]), dim3(256), 0, s, b);
-        break;
+        return launch_residual<8>(b, n, 1, s);
     case STAGE_RES16:
-        if (n.grp[2]) hipLaunchKernelGGL(k_residual<16>, dim3(n.grp[2]), dim3(256), 0, s, b);
-        break;
+        return launch_residual<16>(b, n, 2, s);
     case STAGE_RES32:
-        if (n.grp[3]) hipLaunchKernelGGL(k_residual<32>, dim3(n.grp[3]), dim3(256), 0, s, b);
-        break;
+        return launch_residual<32>(b, n, 3, s);
     case STAGE_INTRA:
         if (n.intra) {
-            const uint32_t g = n.intra < n.intra_wgs ? n.intra : n.intra_wgs;
-            hipLaunchKernelGGL(k_intra, dim3(g), dim3(256), 0, s, b);
+            const uint32_t w = n.intra < n.tune.intra_workers ? n.intra : n.tune.intra_workers;
+            if (n.tune.intra_lanes == 64)
+                hipLaunchKernelGGL(k_intra<64>, dim3(div_up(w, 4)), dim3(256), 0, s, b);
+            else
+                hipLaunchKernelGGL(k_intra<256>, dim3(w), dim3(256), 0, s, b);
         }
         break;
     case STAGE_LF:

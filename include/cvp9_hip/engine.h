@@ -55,6 +55,27 @@ cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e);
 cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int stream_id, uint8_t *dst, uint64_t cap,
                             uint32_t *width, uint32_t *height);
 
+/* Launch shape of the stage kernels: waves sharing one transform group for
+ * N = 4, 8, 16, 32 (1, 2 or 4), threads per intra worker (64 or 256) and
+ * the number of persistent intra workers. Output does not depend on it;
+ * cvp9_engine_autotune picks it for a workload class. */
+typedef struct {
+    uint8_t  res_split[4];
+    uint16_t intra_lanes;
+    uint32_t intra_workers;
+} cvp9_tune_t;
+cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t);
+cvp9_err_t cvp9_engine_set_tune(cvp9_engine_t *e, const cvp9_tune_t *t);
+int        cvp9_engine_compute_units(cvp9_engine_t *e);
+
+/* Small autotuner: times candidate launch shapes on a representative batch
+ * (frame preps[i] of tuning stream streams[i]; those streams' reference
+ * pools advance, so use streams opened for tuning) `iters` times each,
+ * keeps the fastest shape per stage, installs it and returns it in *best.
+ * Synchronises. */
+cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
+                                cvp9_prep_t *const *preps, int iters, cvp9_tune_t *best);
+
 /* Per-stage timing with HIP events on the engine's stream. When on, every
  * stage launch is bracketed by events; cvp9_engine_stage_stats returns, for
  * each of the CVP9_ENGINE_STAGES stages, the summed milliseconds and the

@@ -39,3 +39,33 @@ def test_engine_timing_counts(pkg, gpu_ready):
         st = eng.stage_stats()
         assert st["owner"][1] == 3 and st["mc"][1] == 2 and st["deblock"][1] == 3
         assert all(ms >= 0 for ms, _ in st.values())
+
+
+TUNES = [([1, 1, 1, 1], 256, 64), ([2, 2, 2, 2], 64, 512), ([4, 4, 4, 4], 64, 37),
+         ([1, 2, 4, 4], 256, 5), ([4, 1, 2, 1], 64, 2048)]
+
+
+@pytest.mark.parametrize("split,lanes,workers", TUNES)
+def test_every_launch_shape_bit_exact(pkg, orc, gpu_ready, split, lanes, workers):
+    """Launch shapes (autotuner knobs) never change the output."""
+    frames = [pkg.SynthFrame(pkg.synth_params(2, 264, 200, 900 + lanes), i) for i in range(3)]
+    want = orc.run_oracle(frames)
+    print(f"[case] tune split={split} lanes={lanes} workers={workers}", flush=True)
+    with pkg.Engine(0) as eng:
+        eng.set_tune(split, lanes, workers)
+        sid = eng.open_stream()
+        for i, f in enumerate(frames):
+            eng.run([sid], [eng.prepare(f)])
+            eng.sync()
+            assert np.array_equal(eng.read(sid, 264, 200), want[i]), f"frame {i}"
+
+
+def test_autotune_installs_a_valid_shape(pkg, gpu_ready):
+    with pkg.Engine(0) as eng:
+        sids = [eng.open_stream() for _ in range(3)]
+        p = pkg.synth_params(1, 320, 184, 4)
+        preps = [eng.prepare(pkg.SynthFrame(p, 1)) for _ in sids]
+        t = eng.autotune(sids, preps, iters=2)
+        assert all(s in (1, 2, 4) for s in t.res_split)
+        assert t.intra_lanes in (64, 256) and t.intra_workers > 0
+        assert eng.get_tune().as_dict() == t.as_dict()
