@@ -191,6 +191,22 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p), C.c_int, C.c_void_p]),
     "cvp9_engine_stage_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32), C.c_int]),
+    "cvp9_engine_stream_reset": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_hip_stream": (C.c_void_p, [C.c_void_p]),
+    # server.h
+    "cvp9_server_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "cvp9_server_destroy": (None, [C.c_void_p]),
+    "cvp9_server_stream_cost": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+    "cvp9_server_open": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
+    "cvp9_server_close": (None, [C.c_void_p, C.c_int]),
+    "cvp9_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
+    "cvp9_server_pump": (C.c_int, [C.c_void_p]),
+    "cvp9_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.c_int]),
+    "cvp9_server_pending": (C.c_uint32, [C.c_void_p]),
+    "cvp9_server_read_nv12": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.POINTER(C.c_uint8),
+                                        C.c_uint64]),
+    "cvp9_server_drain": (C.c_int, [C.c_void_p]),
     # testing.h
     "cvp9_hip_version": (C.c_char_p, []),
     "cvp9_hip_device_count": (C.c_int, []),
@@ -457,3 +473,76 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+
+# ---- continuous-batching server (include/cvp9_hip/server.h) ----------------------
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_uint32),
+                ("max_inflight", C.c_uint32)]
+
+
+def stream_cost(w, h):
+    return int(lib().cvp9_server_stream_cost(w, h))
+
+
+class Server:
+    """Mirror of the cvp9_server_* entry points."""
+
+    def __init__(self, engine, hbm_budget, max_batch=4096, max_inflight=2):
+        s = C.c_void_p()
+        cfg = ServerConfig(hbm_budget, max_batch, max_inflight)
+        check(lib().cvp9_server_create(engine.e, C.byref(cfg), C.byref(s)), "server_create")
+        self.s = s
+        self._tags = (C.c_uint64 * 4096)()
+        self._ns = (C.c_uint64 * 4096)()
+
+    def open(self, w, h):
+        """Stream handle, or None when the HBM budget is full."""
+        sid = C.c_int()
+        rc = lib().cvp9_server_open(self.s, w, h, C.byref(sid))
+        if rc == CVP9_ERR_AGAIN:
+            return None
+        check(rc, "server_open")
+        return sid.value
+
+    def close(self, sid):
+        lib().cvp9_server_close(self.s, sid)
+
+    def submit(self, sid, prep, tag):
+        check(lib().cvp9_server_submit(self.s, sid, prep, tag), "server_submit")
+
+    def pump(self):
+        r = lib().cvp9_server_pump(self.s)
+        if r < 0:
+            raise Cvp9Error(r, "server_pump")
+        return r
+
+    def poll(self):
+        n = lib().cvp9_server_poll(self.s, self._tags, self._ns, 4096)
+        if n < 0:
+            raise Cvp9Error(n, "server_poll")
+        return [(self._tags[i], self._ns[i]) for i in range(n)]
+
+    def pending(self):
+        return int(lib().cvp9_server_pending(self.s))
+
+    def read_nv12(self, sid, frame, w, h):
+        out = np.zeros(i420_size(w, h), np.uint8)
+        check(lib().cvp9_server_read_nv12(self.s, sid, frame,
+                                          out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size),
+              "server_read_nv12")
+        return out
+
+    def drain(self):
+        check(lib().cvp9_server_drain(self.s), "server_drain")
+
+    def close_server(self):
+        if self.s:
+            lib().cvp9_server_destroy(self.s)
+            self.s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close_server()

@@ -330,6 +330,15 @@ void StreamState::update_refs(bool key, FrameBuf *out)
     }
 }
 
+hipError_t StreamState::reset(hipStream_t s)
+{
+    for (int i = 0; i < 8; i++) {
+        release(refs_[i]);
+        refs_[i] = nullptr;
+    }
+    return err_ ? hipMemsetAsync(err_, 0, 256, s) : hipSuccess;
+}
+
 hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
 {
     hipError_t e;

@@ -64,6 +64,9 @@ public:
     /* reference-pool rule of the CPU path: KEY -> all 8 slots, else slot 0
      * (src/decoder/vp9_decoder.c:224-231) */
     void update_refs(bool key, FrameBuf *out);
+    /* back to the state of a new stream (empty slots, no error), keeping
+     * device memory; work already queued on s is unaffected */
+    hipError_t reset(hipStream_t s);
     /* owner map, edge records and residual scratch large enough for L;
      * may synchronize s */
     hipError_t ensure_scratch(const PrepLayout &L, hipStream_t s);

@@ -202,6 +202,18 @@ void cvp9_engine_stream_close(cvp9_engine_t *e, int id)
     e->streams[id] = nullptr;
 }
 
+cvp9_err_t cvp9_engine_stream_reset(cvp9_engine_t *e, int id)
+{
+    if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    return to_err(e->streams[id]->reset(e->stream));
+}
+
+void *cvp9_engine_hip_stream(cvp9_engine_t *e)
+{
+    return e ? (void *)e->stream : nullptr;
+}
+
 uint64_t cvp9_engine_stream_bytes(cvp9_engine_t *e, int id)
 {
     if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return 0;

@@ -28,6 +28,11 @@ void       cvp9_engine_destroy(cvp9_engine_t *e);
 /* A decode stream: an empty 8-slot reference pool on the device. */
 cvp9_err_t cvp9_engine_stream_open(cvp9_engine_t *e, int *stream_id);
 void       cvp9_engine_stream_close(cvp9_engine_t *e, int stream_id);
+/* Empties a stream's reference pool (as a new stream) and clears its error
+ * state, keeping its device memory for reuse. */
+cvp9_err_t cvp9_engine_stream_reset(cvp9_engine_t *e, int stream_id);
+/* The engine's HIP stream (hipStream_t), for callers recording events. */
+void      *cvp9_engine_hip_stream(cvp9_engine_t *e);
 /* HBM held by a stream (frame pool, scratch). */
 uint64_t   cvp9_engine_stream_bytes(cvp9_engine_t *e, int stream_id);
 
