@@ -211,6 +211,92 @@ def multi_stream(P, args):
             "ms_per_batch": 1e3 * dt / steps}
 
 
+def serving(P, args):
+    """Serving loop: Poisson arrivals, HBM-budget admission, 30 fps pacing,
+    continuous batching; submit -> NV12-ready latency per frame."""
+    rng = np.random.default_rng(4040)
+    n = args.sstreams
+    res = [MIX[i] for i in rng.integers(0, 4, n)]
+    lengths = rng.integers(30, 301, n)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    pool_n = args.pool
+    log(f"serving: {n} streams, {int(lengths.sum())} frames, pools of {pool_n} frames")
+    specs = [(4, w, h, 9000 + r, i) for r, (w, h) in enumerate(MIX) for i in range(pool_n)]
+    frames = gen_frames(P, specs)
+    eng = P.Engine(0)
+    if args.serve_tune:
+        eng.set_tune(*args.serve_tune)
+    pool = {}
+    for r, (w, h) in enumerate(MIX):
+        pool[(w, h)] = [eng.prepare(frames[r * pool_n + i]) for i in range(pool_n)]
+    del frames
+    budget = int(args.budget_gib * (1 << 30))
+    srv = P.Server(eng, budget, max_batch=4096, max_inflight=2)
+    period_ns = int(1e9 / 30)
+    handle, start_ns, next_k, done = {}, {}, [0] * n, [0] * n
+    submit_ns = {}
+    lat = []
+    waiting = []
+    arrived = 0
+    finished = 0
+    waited = 0
+    max_active = 0
+    batches = 0
+    t0 = time.monotonic_ns()
+    while finished < n:
+        now = time.monotonic_ns() - t0
+        while arrived < n and arrivals[arrived] * 1e9 <= now:
+            waiting.append(arrived)
+            arrived += 1
+        while waiting:
+            s = waiting[0]
+            h = srv.open(*res[s])
+            if h is None:
+                waited += 1
+                break
+            waiting.pop(0)
+            handle[s] = h
+            start_ns[s] = now
+        max_active = max(max_active, len(handle))
+        for s, h in handle.items():
+            while next_k[s] < lengths[s] and start_ns[s] + next_k[s] * period_ns <= now:
+                k = next_k[s]
+                pl = pool[res[s]]
+                prep = pl[0] if k == 0 else pl[1 + (k - 1 + s) % (pool_n - 1)]
+                tag = s * 1000 + k
+                submit_ns[tag] = time.monotonic_ns()
+                srv.submit(h, prep, tag)
+                next_k[s] += 1
+        if srv.pump() > 0:
+            batches += 1
+        for tag, dns in srv.poll():
+            tag = int(tag)
+            lat.append((dns - submit_ns.pop(tag)) / 1e6)
+            s = tag // 1000
+            done[s] += 1
+            if done[s] == lengths[s]:
+                srv.close(handle.pop(s))
+                finished += 1
+        if not srv.pending():
+            time.sleep(0.0002)
+    wall = (time.monotonic_ns() - t0) / 1e9
+    srv.drain()
+    srv.close_server()
+    eng.close()
+    lat = np.array(lat)
+    px = sum(int(lengths[s]) * res[s][0] * res[s][1] for s in range(n))
+    return {"streams": n, "frames": int(lat.size), "seconds": round(wall, 3),
+            "frames_per_s": round(lat.size / wall, 1),
+            "p1080_equiv_frames_per_s": round(px / (1920 * 1080) / wall, 1),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "latency_ms_max": round(float(lat.max()), 3),
+            "max_concurrent_streams": max_active, "admission_waits": waited,
+            "batches": batches, "hbm_budget_GiB": args.budget_gib,
+            "workload": f"{n} streams, Poisson {args.rate}/s, 25% each 360p/480p/720p/1080p, "
+                        f"30-300 frames at 30 fps, {pool_n}-frame pool per resolution"}
+
+
 def cpu_baseline(P, frames, n):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -239,6 +325,13 @@ def main():
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--sstreams", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--pool", type=int, default=24)
+    ap.add_argument("--budget-gib", type=float, default=4.0)
+    ap.add_argument("--serve-tune", type=int, nargs=6, default=None,
+                    help="res_split x4, intra_lanes, intra_workers for the serving run")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="only run the calibration copy kernel (for counter runs)")
     args = ap.parse_args()
@@ -280,6 +373,12 @@ def main():
     multi = None if args.no_multi else multi_stream(P, args)
     if multi:
         log("multi:", json.dumps(multi))
+    if args.serve_tune:
+        a = args.serve_tune
+        args.serve_tune = (a[:4], a[4], a[5])
+    serve = None if args.no_serve else serving(P, args)
+    if serve:
+        log("serving:", json.dumps(serve))
     cpu = None if args.no_cpu else cpu_baseline(P, ss["frames"], args.cpu_frames)
 
     line = {
@@ -312,6 +411,7 @@ def main():
             "stages": multi["stages"],
             "tune": multi["tune"],
             "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2)},
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
