@@ -236,6 +236,7 @@ def serving(P, args):
     handle, start_ns, next_k, done = {}, {}, [0] * n, [0] * n
     submit_ns = {}
     lat = []
+    wait_ms = []
     waiting = []
     arrived = 0
     finished = 0
@@ -252,11 +253,13 @@ def serving(P, args):
             s = waiting[0]
             h = srv.open(*res[s])
             if h is None:
-                waited += 1
                 break
             waiting.pop(0)
             handle[s] = h
             start_ns[s] = now
+            wait_ms.append((now - arrivals[s] * 1e9) / 1e6)
+            if wait_ms[-1] > 1.0:
+                waited += 1
         max_active = max(max_active, len(handle))
         for s, h in handle.items():
             while next_k[s] < lengths[s] and start_ns[s] + next_k[s] * period_ns <= now:
@@ -291,7 +294,9 @@ def serving(P, args):
             "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
             "latency_ms_max": round(float(lat.max()), 3),
-            "max_concurrent_streams": max_active, "admission_waits": waited,
+            "max_concurrent_streams": max_active, "streams_that_waited": waited,
+            "admission_wait_ms_p50": round(float(np.percentile(wait_ms, 50)), 3),
+            "admission_wait_ms_p99": round(float(np.percentile(wait_ms, 99)), 3),
             "batches": batches, "hbm_budget_GiB": args.budget_gib,
             "workload": f"{n} streams, Poisson {args.rate}/s, 25% each 360p/480p/720p/1080p, "
                         f"30-300 frames at 30 fps, {pool_n}-frame pool per resolution"}
@@ -329,7 +334,7 @@ def main():
     ap.add_argument("--sstreams", type=int, default=256)
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--pool", type=int, default=24)
-    ap.add_argument("--budget-gib", type=float, default=4.0)
+    ap.add_argument("--budget-gib", type=float, default=3.0)
     ap.add_argument("--serve-tune", type=int, nargs=6, default=None,
                     help="res_split x4, intra_lanes, intra_workers for the serving run")
     ap.add_argument("--copy-probe-only", action="store_true",
