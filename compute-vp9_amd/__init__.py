@@ -192,6 +192,11 @@ SIGNATURES = {
     "cvp9_engine_stage_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32), C.c_int]),
     "cvp9_engine_stream_reset": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_capture_begin": (C.c_int, [C.c_void_p]),
+    "cvp9_engine_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "cvp9_engine_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cvp9_engine_graph_free": (None, [C.c_void_p, C.c_void_p]),
     "cvp9_engine_hip_stream": (C.c_void_p, [C.c_void_p]),
     # server.h
     "cvp9_server_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -384,7 +389,7 @@ class HipBackend:
 
 # ---- multi-stream engine (include/cvp9_hip/engine.h) ------------------------------
 STAGE_NAMES = ["fill", "owner", "mc", "residual4", "residual8", "residual16", "residual32",
-               "intra", "deblock", "nv12"]
+               "intra", "deblock", "nv12", "fused"]
 
 
 class Engine:
@@ -458,10 +463,28 @@ class Engine:
         check(lib().cvp9_engine_set_timing(self.e, 1 if on else 0), "set_timing")
 
     def stage_stats(self, reset=True):
-        ms = (C.c_double * 10)()
-        n = (C.c_uint32 * 10)()
+        k = len(STAGE_NAMES)
+        ms = (C.c_double * k)()
+        n = (C.c_uint32 * k)()
         check(lib().cvp9_engine_stage_stats(self.e, ms, n, 1 if reset else 0), "stage_stats")
-        return {STAGE_NAMES[i]: (ms[i], n[i]) for i in range(10)}
+        return {STAGE_NAMES[i]: (ms[i], n[i]) for i in range(k)}
+
+    def set_fused(self, on):
+        check(lib().cvp9_engine_set_fused(self.e, 1 if on else 0), "set_fused")
+
+    def capture_begin(self):
+        check(lib().cvp9_engine_capture_begin(self.e), "capture_begin")
+
+    def capture_end(self):
+        g = C.c_void_p()
+        check(lib().cvp9_engine_capture_end(self.e, C.byref(g)), "capture_end")
+        return g
+
+    def graph_launch(self, g):
+        check(lib().cvp9_engine_graph_launch(self.e, g), "graph_launch")
+
+    def graph_free(self, g):
+        lib().cvp9_engine_graph_free(self.e, g)
 
     def close(self):
         if self.e:

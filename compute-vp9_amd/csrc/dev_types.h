@@ -114,7 +114,8 @@ struct DBatch {
     const uint32_t *pre_lf;        /* deblocking windows */
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
-    uint32_t       *ticket;        /* intra ticket counter */
+    uint32_t       *ticket;        /* intra ticket counter (zeroed by the descriptor copy) */
+    uint32_t       *bar;           /* grid-barrier counter of the fused kernel (zeroed likewise) */
     uint32_t        spin_limit;    /* bound on every dependency wait */
 };
 

@@ -278,7 +278,14 @@ StreamState::~StreamState()
     hipFree(err_);
 }
 
-hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out)
+bool StreamState::scratch_fits(const PrepLayout &L) const
+{
+    const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
+    return err_ && cells <= owner_cap_ && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
+           (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_;
+}
+
+hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free)
 {
     uint32_t sy, suv;
     const size_t need = frame_bytes(w, h, &sy, &suv);
@@ -287,7 +294,7 @@ hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out)
         if (f->refcnt == 0 && f->cap >= need && (!pick || f->cap < pick->cap)) pick = f;
     if (!pick) {
         /* drop idle buffers of other sizes before allocating */
-        for (size_t i = 0; i < all_.size();) {
+        for (size_t i = 0; may_free && i < all_.size();) {
             if (all_[i]->refcnt == 0) {
                 hipFree(all_[i]->dev);
                 delete all_[i];
@@ -478,6 +485,8 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
     b_.pre_nv12 = dpre + P_NV12 * stride;
     const uint64_t tk_off = pre_off + align256(12ull * stride * 4);
     b_.ticket = (uint32_t *)(dev + tk_off);
+    b_.bar = (uint32_t *)(dev + tk_off + 128);
+    memset(host + tk_off, 0, 256); /* ticket and barrier start at zero */
     b_.spin_limit = spin_limit;
     n_.fill = pre[P_FILL * stride + njobs];
     n_.blocks = pre[P_BLK * stride + njobs];
@@ -492,7 +501,7 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
 
 hipError_t BatchBuilder::launch(const DBatch &b, const StageCounts &n, hipStream_t s)
 {
-    for (int st = 0; st < STAGE_COUNT; st++) {
+    for (int st = 0; st < STAGE_FUSED; st++) {
         hipError_t e = cvp9_launch_stage(st, b, n, s);
         if (e != hipSuccess) return e;
     }

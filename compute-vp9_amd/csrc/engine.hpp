@@ -59,7 +59,9 @@ public:
     StreamState &operator=(const StreamState &) = delete;
 
     /* frame for the next output; hipSuccess or an allocation error */
-    hipError_t acquire(uint32_t w, uint32_t h, FrameBuf **out);
+    hipError_t acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free = true);
+    /* scratch already large enough for L (no allocation needed) */
+    bool scratch_fits(const PrepLayout &L) const;
     void release(FrameBuf *f);
     /* reference-pool rule of the CPU path: KEY -> all 8 slots, else slot 0
      * (src/decoder/vp9_decoder.c:224-231) */

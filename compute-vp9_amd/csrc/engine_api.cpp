@@ -85,6 +85,13 @@ uint64_t prep_algo_bytes(const vp9_parsed_frame_t *pf)
 
 } // namespace
 
+struct cvp9_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<uint8_t *, uint8_t *>> desc; /* pinned source, device copy */
+    uint32_t batches = 0;
+};
+
 struct cvp9_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -99,6 +106,9 @@ struct cvp9_engine {
     double stage_ms[STAGE_COUNT] = {0};
     uint32_t stage_n[STAGE_COUNT] = {0};
     std::vector<uint32_t *> err_words;
+    bool fused = false;
+    uint32_t fused_wgs = 0;
+    cvp9_graph *capture = nullptr; /* recording batches into a graph */
 
     hipEvent_t take_event()
     {
@@ -150,6 +160,7 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
         delete e;
         return CVP9_ERR_GPU;
     }
+    e->fused_wgs = cvp9_fused_max_wgs(device);
     for (BatchSlot &s : e->slots) {
         if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
             cvp9_engine_destroy(e);
@@ -277,36 +288,69 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             if (streams[k] == id) return CVP9_ERR_INVALID_DATA;
     }
     hipError_t he;
+    if (e->capture) /* nothing may synchronise or memset while recording */
+        for (int i = 0; i < n; i++)
+            if (!e->streams[streams[i]]->scratch_fits(preps[i]->L)) return CVP9_ERR_UNSUPPORTED;
     for (int i = 0; i < n; i++) {
         StreamState *s = e->streams[streams[i]];
         const PrepLayout &L = preps[i]->L;
-        if ((he = s->ensure_scratch(L, e->stream)) != hipSuccess) return to_err(he);
+        if (!e->capture && (he = s->ensure_scratch(L, e->stream)) != hipSuccess) return to_err(he);
         FrameBuf *out = nullptr;
-        if ((he = s->acquire(L.W, L.H, &out)) != hipSuccess) return to_err(he);
+        if ((he = s->acquire(L.W, L.H, &out, !e->capture)) != hipSuccess) return to_err(he);
         jobs[i] = s->make_job(L, preps[i]->dev, out, nv12 ? nv12[i] : nullptr);
         s->update_refs(L.key != 0, out);
     }
 
-    BatchSlot &bs = e->slots[e->next];
-    e->next = (e->next + 1) % kBatchSlots;
-    if (bs.used && hipEventSynchronize(bs.done) != hipSuccess) return CVP9_ERR_GPU;
     const size_t need = BatchBuilder::bytes_for((uint32_t)n);
-    if (bs.cap < need) {
-        if (bs.host) hipHostFree(bs.host);
-        if (bs.dev) hipFree(bs.dev);
-        bs.host = bs.dev = nullptr;
-        bs.cap = 0;
-        if ((he = hipHostMalloc((void **)&bs.host, need, hipHostMallocDefault)) != hipSuccess ||
-            (he = hipMalloc((void **)&bs.dev, need)) != hipSuccess)
+    uint8_t *dhost = nullptr, *ddev = nullptr;
+    BatchSlot *bs = nullptr;
+    if (e->capture) {
+        /* a recorded batch owns its descriptor for the graph's lifetime */
+        if ((he = hipHostMalloc((void **)&dhost, need, hipHostMallocDefault)) != hipSuccess) return to_err(he);
+        if ((he = hipMalloc((void **)&ddev, need)) != hipSuccess) {
+            hipHostFree(dhost);
             return to_err(he);
-        bs.cap = need;
+        }
+        e->capture->desc.emplace_back(dhost, ddev);
+        e->capture->batches++;
+    } else {
+        bs = &e->slots[e->next];
+        e->next = (e->next + 1) % kBatchSlots;
+        if (bs->used && hipEventSynchronize(bs->done) != hipSuccess) return CVP9_ERR_GPU;
+        if (bs->cap < need) {
+            if (bs->host) hipHostFree(bs->host);
+            if (bs->dev) hipFree(bs->dev);
+            bs->host = bs->dev = nullptr;
+            bs->cap = 0;
+            if ((he = hipHostMalloc((void **)&bs->host, need, hipHostMallocDefault)) != hipSuccess ||
+                (he = hipMalloc((void **)&bs->dev, need)) != hipSuccess)
+                return to_err(he);
+            bs->cap = need;
+        }
+        dhost = bs->host;
+        ddev = bs->dev;
     }
     BatchBuilder bb;
-    bb.build(jobs.data(), (uint32_t)n, bs.host, bs.dev, e->tune, kSpinLimit);
-    if ((he = hipMemcpyAsync(bs.dev, bs.host, bb.used(), hipMemcpyHostToDevice, e->stream)) != hipSuccess)
+    bb.build(jobs.data(), (uint32_t)n, dhost, ddev, e->tune, kSpinLimit);
+    if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, e->stream)) != hipSuccess)
         return to_err(he);
-    for (int st = 0; st < STAGE_COUNT; st++) {
-        const bool timed = e->timing && stage_has_work(st, bb.counts());
+    const bool timing = e->timing && !e->capture;
+    if (e->fused) {
+        TimedLaunch t{STAGE_FUSED, nullptr, nullptr};
+        if (timing) {
+            t.a = e->take_event();
+            t.b = e->take_event();
+            if (!t.a || !t.b) return CVP9_ERR_GPU;
+            hipEventRecord(t.a, e->stream);
+        }
+        if ((he = cvp9_launch_fused(bb.batch(), e->fused_wgs, e->stream)) != hipSuccess) return to_err(he);
+        if (timing) {
+            hipEventRecord(t.b, e->stream);
+            e->timed.push_back(t);
+        }
+    }
+    for (int st = 0; !e->fused && st < STAGE_FUSED; st++) {
+        const bool timed = timing && stage_has_work(st, bb.counts());
         TimedLaunch t{st, nullptr, nullptr};
         if (timed) {
             t.a = e->take_event();
@@ -321,9 +365,78 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             e->timed.push_back(t);
         }
     }
-    if ((he = hipEventRecord(bs.done, e->stream)) != hipSuccess) return to_err(he);
-    bs.used = true;
+    if (bs) {
+        if ((he = hipEventRecord(bs->done, e->stream)) != hipSuccess) return to_err(he);
+        bs->used = true;
+    }
     return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_set_fused(cvp9_engine_t *e, int on)
+{
+    if (!e) return CVP9_ERR_INVALID_DATA;
+    if (on && e->fused_wgs == 0) return CVP9_ERR_UNSUPPORTED;
+    e->fused = on != 0;
+    return CVP9_OK;
+}
+
+static void free_graph(cvp9_graph *g)
+{
+    if (!g) return;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    for (auto &d : g->desc) {
+        hipHostFree(d.first);
+        hipFree(d.second);
+    }
+    delete g;
+}
+
+cvp9_err_t cvp9_engine_capture_begin(cvp9_engine_t *e)
+{
+    if (!e || e->capture) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    e->capture = new (std::nothrow) cvp9_graph();
+    if (!e->capture) return CVP9_ERR_NOMEM;
+    if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        free_graph(e->capture);
+        e->capture = nullptr;
+        return CVP9_ERR_GPU;
+    }
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_capture_end(cvp9_engine_t *e, cvp9_graph_t **out)
+{
+    if (!e || !e->capture || !out) return CVP9_ERR_INVALID_DATA;
+    cvp9_graph *g = e->capture;
+    e->capture = nullptr;
+    *out = nullptr;
+    if (hipStreamEndCapture(e->stream, &g->graph) != hipSuccess || !g->graph ||
+        hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        free_graph(g);
+        return CVP9_ERR_GPU;
+    }
+    *out = g;
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_graph_launch(cvp9_engine_t *e, cvp9_graph_t *g)
+{
+    if (!e || !g || !g->exec) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(e->device);
+    return hipGraphLaunch(g->exec, e->stream) == hipSuccess ? CVP9_OK : CVP9_ERR_GPU;
+}
+
+void cvp9_engine_graph_free(cvp9_engine_t *e, cvp9_graph_t *g)
+{
+    if (!g) return;
+    if (e) {
+        hipSetDevice(e->device);
+        hipStreamSynchronize(e->stream);
+    }
+    free_graph(g);
 }
 
 cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e)

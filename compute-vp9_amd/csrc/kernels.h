@@ -16,6 +16,7 @@ enum Stage {
     STAGE_INTRA,
     STAGE_LF,
     STAGE_NV12,
+    STAGE_FUSED, /* the fused kernel (all of the above in one launch) */
     STAGE_COUNT
 };
 
@@ -40,5 +41,10 @@ hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double
                                 const double *m32);
 hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t s);
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s);
+
+/* Fused persistent kernel: every stage of the batch in one launch of `wgs`
+ * co-resident workgroups (at most cvp9_fused_max_wgs()). */
+hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s);
+uint32_t cvp9_fused_max_wgs(int device);
 
 #endif

@@ -56,9 +56,8 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p)
 /* 128 fill of the output frame and NONE fill of the owner map, for jobs    */
 /* whose blocks do not cover the frame (vp9_frame.c:27-30 initial value).   */
 
-__global__ __launch_bounds__(256) void k_fill(DBatch B)
+__device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w)
 {
-    const uint32_t w = blockIdx.x;
     const uint32_t j = find_job(B.pre_fill, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t chunk = w - B.pre_fill[j];
@@ -84,16 +83,17 @@ __global__ __launch_bounds__(256) void k_fill(DBatch B)
     for (uint32_t i = threadIdx.x; i < (uint32_t)(len >> 4); i += 256) p[i] = v;
 }
 
+__global__ __launch_bounds__(256) void k_fill(DBatch B)
+{
+    fill_item(B, blockIdx.x);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
 /* Also resets the intra ticket counter for this batch.                     */
 
-__global__ __launch_bounds__(256) void k_owner(DBatch B)
+__device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_t lane)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t lane = threadIdx.x & 63;
-    if (wv >= B.pre_blocks[B.njobs]) return;
     const uint32_t j = find_job(B.pre_blocks, B.njobs, wv);
     const DJob &J = B.jobs[j];
     const uint32_t b = wv - B.pre_blocks[j];
@@ -105,6 +105,13 @@ __global__ __launch_bounds__(256) void k_owner(DBatch B)
         const uint32_t r = i / ncx, c = i - r * ncx;
         J.owner[(cy0 + r) * J.ow + cx0 + c] = b;
     }
+}
+
+__global__ __launch_bounds__(256) void k_owner(DBatch B)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv < B.pre_blocks[B.njobs]) owner_item(B, wv, threadIdx.x & 63);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -163,11 +170,8 @@ __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
     }
 }
 
-__global__ __launch_bounds__(256) void k_mc(DBatch B)
+__device__ __forceinline__ void mc_item(const DBatch &B, uint32_t wv, int lane)
 {
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wv >= B.pre_inter[B.njobs]) return;
     const uint32_t j = find_job(B.pre_inter, B.njobs, wv);
     const DJob &J = B.jobs[j];
     const uint32_t l = wv - B.pre_inter[j];
@@ -220,6 +224,12 @@ __global__ __launch_bounds__(256) void k_mc(DBatch B)
             store_quad(pl + (size_t)(cby + cr) * J.out.suv + cbx + 4 * cq, v, cbw - 4 * cq);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_mc(DBatch B)
+{
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv < B.pre_inter[B.njobs]) mc_item(B, wv, threadIdx.x & 63);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -347,15 +357,22 @@ __device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
 }
 
 template <int N, int S>
-__global__ __launch_bounds__(256) void k_residual(DBatch B)
+struct ResShape {
+    static constexpr int C = N == 4 ? 0 : N == 8 ? 1 : N == 16 ? 2 : 3;
+    static constexpr int G = 64 / N;
+    static constexpr int GPW = 4 / S;                         /* groups per workgroup */
+    static constexpr int LDS_DOUBLES = GPW * G * N * (N + 1); /* temp of all groups */
+};
+
+/* one workgroup unit: GPW transform groups of size N, S waves per group */
+template <int N, int S>
+__device__ __forceinline__ void residual_unit(const DBatch &B, uint32_t unit, double *lds)
 {
-    constexpr int C = N == 4 ? 0 : N == 8 ? 1 : N == 16 ? 2 : 3;
-    constexpr int G = 64 / N;
-    constexpr int GPW = 4 / S; /* transform groups per workgroup */
-    __shared__ double lds[GPW][G * N * (N + 1)];
+    using RS = ResShape<N, S>;
+    constexpr int C = RS::C, G = RS::G, GPW = RS::GPW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int gi = wave / S, sub = wave - gi * S;
-    const uint32_t grp = blockIdx.x * GPW + gi;
+    const uint32_t grp = unit * GPW + gi;
     const uint32_t *pre = B.pre_grp[C];
     /* every wave reaches the barrier inside idct_group: a group past the
      * end runs with no transform */
@@ -369,7 +386,14 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
         stride = J.out.sy;
         locate_tx<N>(J, C, (grp - pre[j]) * G + lane / N, coef, dst, rout);
     }
-    idct_group<N, S>(coef, dst, stride, rout, lds[gi], sub, lane);
+    idct_group<N, S>(coef, dst, stride, rout, lds + gi * (G * N * (N + 1)), sub, lane);
+}
+
+template <int N, int S>
+__global__ __launch_bounds__(256) void k_residual(DBatch B)
+{
+    __shared__ double lds[ResShape<N, S>::LDS_DOUBLES];
+    residual_unit<N, S>(B, blockIdx.x, lds);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -473,9 +497,8 @@ struct IntraShared {
 /* W threads per worker: 256 (one workgroup per block) or 64 (one wave per
  * block, four independent workers per workgroup). */
 template <int W>
-__global__ __launch_bounds__(256) void k_intra(DBatch B)
+__device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
 {
-    __shared__ IntraShared shm[256 / W];
     IntraShared &S = shm[threadIdx.x / W];
     const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
@@ -600,6 +623,13 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
     }
 }
 
+template <int W>
+__global__ __launch_bounds__(256) void k_intra(DBatch B)
+{
+    __shared__ IntraShared shm[256 / W];
+    intra_worker<W>(B, shm);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Deblocking (vp9_reconstruct.c:134-191): windows [64i-2, 64i+62) in both  */
 /* axes. Each window holds every sample its own edges read or write, and    */
@@ -618,10 +648,8 @@ __device__ __forceinline__ void lf_edge(uint8_t &p1, uint8_t &p0, uint8_t &q0, u
     }
 }
 
-__global__ __launch_bounds__(256) void k_lf(DBatch B)
+__device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*tile)[65])
 {
-    __shared__ uint8_t tile[64][65];
-    const uint32_t w = blockIdx.x;
     const uint32_t j = find_job(B.pre_lf, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t l = w - B.pre_lf[j];
@@ -664,13 +692,18 @@ __global__ __launch_bounds__(256) void k_lf(DBatch B)
     }
 }
 
+__global__ __launch_bounds__(256) void k_lf(DBatch B)
+{
+    __shared__ uint8_t tile[64][65];
+    lf_item(B, blockIdx.x, tile);
+}
+
 /* ------------------------------------------------------------------------ */
 /* I420 -> NV12 into the job's render target, pitch W (vp9_nv12_pack and    */
 /* vaapi_entrypoint.c:131-153 layout). One workgroup per 64x64 luma tile.    */
 
-__global__ __launch_bounds__(256) void k_nv12(DBatch B)
+__device__ __forceinline__ void nv12_item(const DBatch &B, uint32_t w)
 {
-    const uint32_t w = blockIdx.x;
     const uint32_t j = find_job(B.pre_nv12, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t l = w - B.pre_nv12[j];
@@ -690,6 +723,116 @@ __global__ __launch_bounds__(256) void k_nv12(DBatch B)
             uv[(size_t)r * 2 * cw + 2 * c + 1] = J.out.v[(size_t)r * J.out.suv + c];
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_nv12(DBatch B)
+{
+    nv12_item(B, blockIdx.x);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fused persistent reconstruction: every stage of a batch in one launch.   */
+/* All workgroups are co-resident (the grid is sized from the occupancy     */
+/* query); each phase spreads its work items over the grid and ends with a  */
+/* grid barrier: a counter in the batch descriptor (zeroed by the           */
+/* descriptor copy that precedes every launch or graph replay), agent-scope */
+/* release before arriving and acquire after leaving. Waits are bounded: a  */
+/* timeout sets error bit 4 of the first job's stream and falls through.    */
+
+__device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(B.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(B.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > B.spin_limit) {
+                atomicOr(B.jobs[0].err, 4u);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+/* transform split used inside the fused kernel, per size 4, 8, 16, 32 */
+constexpr int FS4 = 1, FS8 = 1, FS16 = 2, FS32 = 4;
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t FUSED_LDS =
+    cmax(8 * cmax(cmax(ResShape<4, FS4>::LDS_DOUBLES, ResShape<8, FS8>::LDS_DOUBLES),
+                  cmax(ResShape<16, FS16>::LDS_DOUBLES, ResShape<32, FS32>::LDS_DOUBLES)),
+         cmax(4 * sizeof(IntraShared), 64 * 65));
+
+template <int N, int S>
+__device__ __forceinline__ void residual_phase(const DBatch &B, double *lds)
+{
+    using RS = ResShape<N, S>;
+    const uint32_t units = (B.pre_grp[RS::C][B.njobs] + RS::GPW - 1) / RS::GPW;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        residual_unit<N, S>(B, u, lds);
+        __syncthreads(); /* the next unit reuses the LDS */
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fused(DBatch B)
+{
+    __shared__ __align__(16) uint8_t smem[FUSED_LDS];
+    const uint32_t nwg = gridDim.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t phase = 0;
+
+    const uint32_t nfill = B.pre_fill[B.njobs];
+    if (nfill) {
+        for (uint32_t w = blockIdx.x; w < nfill; w += nwg) fill_item(B, w);
+        grid_barrier(B, ++phase * nwg);
+    }
+    /* owner map and motion compensation (independent) */
+    const uint32_t nblk = B.pre_blocks[B.njobs], nmc = B.pre_inter[B.njobs];
+    for (uint32_t wv = blockIdx.x * 4 + wave; wv < nblk; wv += nwg * 4) owner_item(B, wv, lane);
+    for (uint32_t wv = blockIdx.x * 4 + wave; wv < nmc; wv += nwg * 4) mc_item(B, wv, lane);
+    grid_barrier(B, ++phase * nwg);
+    /* transforms: inter ones add to the prediction, intra ones go aside */
+    double *lds = (double *)smem;
+    residual_phase<4, FS4>(B, lds);
+    residual_phase<8, FS8>(B, lds);
+    residual_phase<16, FS16>(B, lds);
+    residual_phase<32, FS32>(B, lds);
+    grid_barrier(B, ++phase * nwg);
+    /* intra blocks in list order, one-wave workers */
+    if (B.pre_intra[B.njobs]) {
+        intra_worker<64>(B, (IntraShared *)smem);
+        grid_barrier(B, ++phase * nwg);
+    }
+    /* deblocking windows */
+    const uint32_t nlf = B.pre_lf[B.njobs];
+    for (uint32_t w = blockIdx.x; w < nlf; w += nwg) {
+        lf_item(B, w, (uint8_t(*)[65])smem);
+        __syncthreads();
+    }
+    /* NV12 */
+    const uint32_t nnv = B.pre_nv12[B.njobs];
+    if (nnv) {
+        grid_barrier(B, ++phase * nwg);
+        for (uint32_t w = blockIdx.x; w < nnv; w += nwg) nv12_item(B, w);
+    }
+}
+
+hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fused, dim3(wgs), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+uint32_t cvp9_fused_max_wgs(int device)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * cus);
 }
 
 /* ------------------------------------------------------------------------ */

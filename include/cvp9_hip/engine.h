@@ -20,6 +20,7 @@ extern "C" {
 
 typedef struct cvp9_engine cvp9_engine_t;
 typedef struct cvp9_prep   cvp9_prep_t;
+typedef struct cvp9_graph  cvp9_graph_t;
 
 /* Engine on HIP device `device` with its own stream. */
 cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out);
@@ -81,13 +82,32 @@ int        cvp9_engine_compute_units(cvp9_engine_t *e);
 cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
                                 cvp9_prep_t *const *preps, int iters, cvp9_tune_t *best);
 
+/* Fused mode: cvp9_engine_run launches one persistent kernel per batch
+ * (every stage, separated by grid barriers, all workgroups co-resident)
+ * instead of one kernel per stage. CVP9_ERR_UNSUPPORTED if the device
+ * cannot hold the fused kernel's grid. Output is identical. */
+cvp9_err_t cvp9_engine_set_fused(cvp9_engine_t *e, int on);
+
+/* Graph capture: batches enqueued by cvp9_engine_run between begin and end
+ * are recorded (descriptor copy + kernels), not executed, into a HIP graph
+ * that cvp9_engine_graph_launch replays. The host-side reference pools
+ * advance at record time. A replay reproduces the recorded frames exactly
+ * when the recorded sequence starts with a KEY frame on every stream and
+ * nothing else ran on those streams since. Streams must have run once at
+ * the same sizes first (no allocation while recording: otherwise
+ * CVP9_ERR_UNSUPPORTED). */
+cvp9_err_t cvp9_engine_capture_begin(cvp9_engine_t *e);
+cvp9_err_t cvp9_engine_capture_end(cvp9_engine_t *e, cvp9_graph_t **g);
+cvp9_err_t cvp9_engine_graph_launch(cvp9_engine_t *e, cvp9_graph_t *g);
+void       cvp9_engine_graph_free(cvp9_engine_t *e, cvp9_graph_t *g);
+
 /* Per-stage timing with HIP events on the engine's stream. When on, every
  * stage launch is bracketed by events; cvp9_engine_stage_stats returns, for
  * each of the CVP9_ENGINE_STAGES stages, the summed milliseconds and the
  * number of launches since the last reset (it synchronises). Stage order:
  * fill, owner, mc, residual4, residual8, residual16, residual32, intra,
- * deblock, nv12. */
-#define CVP9_ENGINE_STAGES 10
+ * deblock, nv12, fused. */
+#define CVP9_ENGINE_STAGES 11
 cvp9_err_t cvp9_engine_set_timing(cvp9_engine_t *e, int on);
 cvp9_err_t cvp9_engine_stage_stats(cvp9_engine_t *e, double *ms, uint32_t *launches,
                                    int reset);
