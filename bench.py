@@ -302,6 +302,61 @@ def serving(P, args):
                         f"30-300 frames at 30 fps, {pool_n}-frame pool per resolution"}
 
 
+def config2(P, args):
+    """3840x2160 preset 2, 60 frames: plain per-stage kernels, the fused
+    persistent kernel, and the fused kernel with the sequence captured as
+    one HIP graph. Inputs prepared in HBM; frames/s of each."""
+    W, H, F = 3840, 2160, args.c2frames
+    log(f"config 2: generating {F} frames {W}x{H} preset 2")
+    frames = gen_frames(P, [(2, W, H, 2222, i) for i in range(F)])
+    eng = P.Engine(0)
+    preps = [eng.prepare(f) for f in frames]
+    del frames
+    tune = None
+    if not args.no_tune:
+        tsid = eng.open_stream()
+        eng.run([tsid], [preps[0]])
+        tune = eng.autotune([tsid], [preps[1]], iters=3).as_dict()
+        eng.close_stream(tsid)
+    sid = eng.open_stream()
+    out = {"workload": f"{W}x{H} single stream, {F}-frame preset-2 sequence "
+                       "(32x32 transforms at p=0.5, MV +-256 px, level per frame)",
+           "tune": tune}
+
+    def timed(run):
+        run()                      # warm: sizes scratch and frame pool
+        eng.sync()
+        t0 = time.perf_counter()
+        run()
+        eng.sync()
+        return time.perf_counter() - t0
+
+    def seq():
+        for p in preps:
+            eng.run([sid], [p])
+
+    dt = timed(seq)
+    out["plain_fps"] = round(F / dt, 2)
+    eng.set_fused(True)
+    dt = timed(seq)
+    out["fused_fps"] = round(F / dt, 2)
+    eng.capture_begin()
+    seq()
+    g = eng.capture_end()
+    dt = timed(lambda: eng.graph_launch(g))
+    out["fused_graph_fps"] = round(F / dt, 2)
+    eng.graph_free(g)
+    eng.set_fused(False)
+    # per-stage view of the plain path (HIP events)
+    eng.stage_stats(reset=True)
+    eng.set_timing(True)
+    seq()
+    out["plain_stages_us"] = {n: round(1e3 * ms / c, 1)
+                              for n, (ms, c) in eng.stage_stats(reset=True).items() if c}
+    eng.close()
+    return out
+
+
 def cpu_baseline(P, frames, n):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -331,6 +386,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c2frames", type=int, default=60)
     ap.add_argument("--sstreams", type=int, default=256)
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--pool", type=int, default=24)
@@ -378,6 +435,9 @@ def main():
     multi = None if args.no_multi else multi_stream(P, args)
     if multi:
         log("multi:", json.dumps(multi))
+    c2 = None if args.no_c2 else config2(P, args)
+    if c2:
+        log("config 2:", json.dumps(c2))
     if args.serve_tune:
         a = args.serve_tune
         args.serve_tune = (a[:4], a[4], a[5])
@@ -416,6 +476,7 @@ def main():
             "stages": multi["stages"],
             "tune": multi["tune"],
             "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2)},
+        "config2_2160p": c2,
         "serving": serve,
         "cpu_baseline": cpu,
     }
