@@ -217,6 +217,7 @@ SIGNATURES = {
     "cvp9_hip_device_count": (C.c_int, []),
     "cvp9_hip_idct_table": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "cvp9_hip_plan_check": (C.c_int, [PF_P]),
+    "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "hip_backend_set_debug": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cvp9_hip_copy_probe": (C.c_double, [C.c_uint64, C.c_int]),
 }

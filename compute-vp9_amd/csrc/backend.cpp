@@ -13,10 +13,11 @@
 #include <hsa/hsa_ext_amd.h>
 #include <sys/mman.h>
 #include <unistd.h>
-#include <string.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <new>
+#include <vector>
 
 #include "cvp9_hip/hip_backend.h"
 #include "cvp9_hip/testing.h"
@@ -405,6 +406,26 @@ cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf)
 {
     PrepLayout L;
     return plan_measure(pf, L);
+}
+
+int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap)
+{
+    PrepLayout L;
+    cvp9_err_t err = plan_measure(pf, L);
+    if (err != CVP9_OK) return err;
+    if (L.n_intra > cap || (L.n_intra && !order)) return CVP9_ERR_INVALID_DATA;
+    std::vector<uint8_t> img(L.bytes);
+    plan_write(pf, L, img.data());
+    /* device list index -> index in pf->blocks (blocks clamped away are not listed) */
+    std::vector<uint32_t> src;
+    src.reserve(L.n_blocks);
+    for (uint32_t i = 0; i < pf->num_blocks; i++) {
+        const vp9_macroblock_info_t &b = pf->blocks[i];
+        if (b.x < pf->hdr.width && b.y < pf->hdr.height) src.push_back(i);
+    }
+    const uint32_t *intra = (const uint32_t *)(img.data() + L.off_intra);
+    for (uint32_t i = 0; i < L.n_intra; i++) order[i] = src[intra[i]];
+    return (int)L.n_intra;
 }
 
 double cvp9_hip_copy_probe(uint64_t bytes, int iters)

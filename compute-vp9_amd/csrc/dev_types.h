@@ -110,11 +110,11 @@ struct DBatch {
     const uint32_t *pre_blocks;    /* owner-map scatter: one wave per block */
     const uint32_t *pre_inter;     /* motion compensation: one wave / strip */
     const uint32_t *pre_grp[4];    /* residual: groups of 64/N transforms */
-    const uint32_t *pre_intra;     /* intra: one ticket per block */
+    const uint32_t *pre_intra;     /* intra blocks per job */
     const uint32_t *pre_lf;        /* deblocking windows */
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
-    uint32_t       *ticket;        /* intra ticket counter (zeroed by the descriptor copy) */
+    uint32_t       *jticket;       /* per-job intra ticket counters (zeroed by the descriptor copy) */
     uint32_t       *bar;           /* grid-barrier counter of the fused kernel (zeroed likewise) */
     uint32_t        spin_limit;    /* bound on every dependency wait */
 };

@@ -6,7 +6,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 namespace cvp9 {
 
@@ -170,6 +172,59 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
     return CVP9_OK;
 }
 
+/* Orders the intra list by dependency level, then list index. An intra
+ * block depends on the earlier intra blocks owning its above row, left
+ * column and above-left sample (for chroma too: a chroma neighbour lies in
+ * the same 4x4 luma cells). Level 0 has no such block; otherwise the level
+ * is one more than the highest level among them. Intra workers take blocks
+ * in this order, so every block a worker waits for was handed out earlier,
+ * and the blocks of one level can all run at once. Those neighbour samples
+ * lie on the owner's bottom row or right column, so only those cells record
+ * the owner's level (plus one) in the scratch map, in list order: a cell
+ * still at zero when block b reads it is not owned by an earlier intra
+ * block. */
+static uint32_t order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint32_t W, uint32_t H)
+{
+    if (ni < 2) return ni ? 1 : 0;
+    const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
+    thread_local std::vector<uint32_t> map; /* level + 1 per 4x4 cell, zero between frames */
+    thread_local std::vector<uint32_t> lev, cnt, tmp;
+    if (map.size() < (size_t)ow * oh) map.assign((size_t)ow * oh, 0);
+    lev.resize(ni);
+    uint32_t depth = 0;
+    for (uint32_t i = 0; i < ni; i++) {
+        const DBlock &k = blocks[intra[i]];
+        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
+        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
+        uint32_t m = 0;
+        if (cy0 > 0) {
+            const uint32_t *row = &map[(size_t)(cy0 - 1) * ow];
+            for (uint32_t cx = cx0 ? cx0 - 1 : 0; cx <= cx1; cx++) m = std::max(m, row[cx]);
+        }
+        if (cx0 > 0)
+            for (uint32_t cy = cy0; cy <= cy1; cy++) m = std::max(m, map[(size_t)cy * ow + cx0 - 1]);
+        lev[i] = m; /* level of this block */
+        depth = std::max(depth, m + 1);
+        for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = m + 1;
+        for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = m + 1;
+    }
+    /* clear what this frame wrote, then a stable counting sort by level */
+    for (uint32_t i = 0; i < ni; i++) {
+        const DBlock &k = blocks[intra[i]];
+        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
+        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
+        for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = 0;
+        for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = 0;
+    }
+    cnt.assign(depth + 1, 0);
+    for (uint32_t i = 0; i < ni; i++) cnt[lev[i] + 1]++;
+    for (uint32_t l = 0; l < depth; l++) cnt[l + 1] += cnt[l];
+    tmp.resize(ni);
+    for (uint32_t i = 0; i < ni; i++) tmp[cnt[lev[i]]++] = intra[i];
+    memcpy(intra, tmp.data(), (size_t)ni * 4);
+    return depth;
+}
+
 void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
 {
     const int W = (int)L.W, H = (int)L.H;
@@ -236,6 +291,7 @@ void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
     }
     for (int k = 0; k < 4; k++) cpre[k][cn[k]] = ctx[k];
     mc_pre[nn] = mc_acc;
+    order_intra(blocks, intra, ni, L.W, L.H);
     if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
 }
 
@@ -455,6 +511,7 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
     const uint32_t stride = njobs + 1;
     enum { P_FILL, P_BLK, P_INTER, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
     for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
+    memset(pre + P_N * stride, 0, (size_t)stride * 4); /* row P_N: intra tickets */
     for (uint32_t i = 0; i < njobs; i++) {
         const DJob &J = jobs[i];
         uint32_t c[P_N];
@@ -483,10 +540,10 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
     b_.pre_intra = dpre + P_INTRA * stride;
     b_.pre_lf = dpre + P_LF * stride;
     b_.pre_nv12 = dpre + P_NV12 * stride;
+    b_.jticket = (uint32_t *)(dev + pre_off) + P_N * stride;
     const uint64_t tk_off = pre_off + align256(12ull * stride * 4);
-    b_.ticket = (uint32_t *)(dev + tk_off);
-    b_.bar = (uint32_t *)(dev + tk_off + 128);
-    memset(host + tk_off, 0, 256); /* ticket and barrier start at zero */
+    b_.bar = (uint32_t *)(dev + tk_off);
+    memset(host + tk_off, 0, 256); /* the barrier counter starts at zero */
     b_.spin_limit = spin_limit;
     n_.fill = pre[P_FILL * stride + njobs];
     n_.blocks = pre[P_BLK * stride + njobs];

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_fill(DBatch B)
 
 /* ------------------------------------------------------------------------ */
 /* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
-/* Also resets the intra ticket counter for this batch.                     */
+/* Also resets the per-job intra ticket counters of this batch.             */
 
 __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_t lane)
 {
@@ -109,7 +109,8 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
 
 __global__ __launch_bounds__(256) void k_owner(DBatch B)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < B.njobs; i += 256) B.jticket[i] = 0;
     const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wv < B.pre_blocks[B.njobs]) owner_item(B, wv, threadIdx.x & 63);
 }
@@ -406,15 +407,21 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
 /* computed by the residual stage (they do not depend on neighbours), so    */
 /* the ordered part is prediction + residual add only.                      */
 /*                                                                          */
-/* Intra blocks are taken by workgroups in list order through an atomic     */
-/* ticket. Each publishes its final bottom row and right column of Y, U, V  */
+/* Intra blocks of a job are taken in the order of the job's intra list    */
+/* (dependency level, then list index) through the job's atomic ticket. A   */
+/* worker starts on the job that holds its share of the batch's             */
+/* intra blocks and moves on to the next job when that one's tickets run    */
+/* out, so the chains of all frames advance together. Each worker publishes */
+/* its block's final bottom row and right column of Y, U, V                 */
 /* as 64-bit words (4 samples + the frame epoch), written and read with     */
 /* device-scope atomics: a word is valid exactly when its epoch matches, so */
 /* no fences or flags are needed. A neighbour sample of block b owned by an */
 /* earlier intra block o always lies on o's bottom row or right column:     */
 /* o covers the sample and not the sample's neighbour inside b, and blocks  */
 /* are disjoint rectangles. A block waits (bounded) only on earlier intra   */
-/* blocks, which running workgroups already hold, so every wait ends.        */
+/* blocks of its own job at lower dependency levels; the host orders each  */
+/* job's intra list by level, so their tickets were handed out earlier, to  */
+/* workers that are running, and every wait ends.                           */
 
 __device__ __forceinline__ int nb_sample(const DJob &J, int p, int px, int py, uint32_t b,
                                          uint32_t spin_limit)
@@ -502,19 +509,33 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
     IntraShared &S = shm[threadIdx.x / W];
     const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
+    const uint32_t nwk = gridDim.x * (256 / W);
+    const uint32_t wk = blockIdx.x * (256 / W) + threadIdx.x / W;
 
-    /* The ticket is made wave-uniform with readfirstlane, so the loop
-     * condition is a scalar branch. With an LDS value as the exit test the
-     * compiler treats the exit as divergent and restructures the loop so
-     * that thread 0's ticket fetch runs outside the loop holding the
-     * barriers: the other lanes then repeat one ticket forever. */
-    if (tid == 0) S.ticket = atomicAdd(B.ticket, 1u);
+    /* Job, ticket and the count of jobs left to visit are made wave-uniform
+     * with readfirstlane, so every loop test is a scalar branch. With an
+     * LDS value as the exit test the compiler treats the exit as divergent
+     * and restructures the loop so that thread 0's ticket fetch runs outside
+     * the loop holding the barriers: the other lanes then repeat one ticket
+     * forever. */
+    uint32_t j = __builtin_amdgcn_readfirstlane(
+        find_job(B.pre_intra, B.njobs, (uint32_t)((uint64_t)wk * total / nwk)));
+    uint32_t left = B.njobs;
+    if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
     worker_sync<W>();
     uint32_t tk = __builtin_amdgcn_readfirstlane(S.ticket);
-    while (tk < total) {
-        const uint32_t j = find_job(B.pre_intra, B.njobs, tk);
+    for (;;) {
         const DJob &J = B.jobs[j];
-        const uint32_t b = J.intra[tk - B.pre_intra[j]];
+        if (tk >= J.n_intra) {
+            /* this job's blocks are all taken: try the next one */
+            if (--left == 0) break;
+            j = j + 1 == B.njobs ? 0 : j + 1;
+            if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
+            worker_sync<W>();
+            tk = __builtin_amdgcn_readfirstlane(S.ticket);
+            continue;
+        }
+        const uint32_t b = J.intra[tk];
         const DBlock k = J.blocks[b];
         const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
         const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
@@ -617,7 +638,7 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
         }
 
         /* next ticket; the sync also orders LDS reuse */
-        if (tid == 0) S.ticket = atomicAdd(B.ticket, 1u);
+        if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
         worker_sync<W>();
         tk = __builtin_amdgcn_readfirstlane(S.ticket);
     }

@@ -30,6 +30,13 @@ int cvp9_hip_idct_table(int n, double *out);
  * HIP path takes the frame, else the code hip_decode_frame would return. */
 cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf);
 
+/* The order in which intra workers take pf's intra blocks: indices into
+ * pf->blocks, by dependency level (no earlier intra block among the
+ * above/left/above-left neighbours: level 0), then list index. Returns the
+ * number of intra blocks written to order (at most cap), or a negative
+ * cvp9_err_t. Needs no device. */
+int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap);
+
 /* Copies `bytes` (multiple of 16) device to device `iters` times with a
  * plain streaming kernel on the current device; returns the achieved GB/s
  * (read + write bytes / time), or a negative value on error. Used to

@@ -209,3 +209,79 @@ def test_generator_coefficient_density(pkg):
         nz = np.argwhere(blk != 0)
         assert len(nz) <= 6 and all(r + c <= 2 for r, c in nz)
         break
+
+
+# ---- intra work order (what keeps the persistent intra workers deadlock-free) ----
+
+def intra_dependencies(b, w, h):
+    """For each intra block i (index into b): the earlier intra blocks owning
+    a neighbour sample its prediction reads, luma and chroma, sample by
+    sample (vp9_reconstruct.c:224-247, :272-311)."""
+    owner = -np.ones(((h + 3) // 4, (w + 3) // 4), np.int64)
+    for i, k in enumerate(b):
+        x, y = int(k["x"]), int(k["y"])
+        if x >= w or y >= h:
+            continue
+        bw, bh = min(int(k["width"]), w - x), min(int(k["height"]), h - y)
+        owner[y // 4:(y + bh + 3) // 4, x // 4:(x + bw + 3) // 4] = i
+    deps = {}
+    for i, k in enumerate(b):
+        x, y = int(k["x"]), int(k["y"])
+        if not k["is_intra"] or x >= w or y >= h:
+            continue
+        bw, bh = min(int(k["width"]), w - x), min(int(k["height"]), h - y)
+        samples = []
+        for sh, (px, py, pw, ph) in ((2, (x, y, bw, bh)),
+                                     (1, (x >> 1, y >> 1, (bw + 1) >> 1, (bh + 1) >> 1))):
+            if py > 0:
+                samples += [(px + e, py - 1, sh) for e in range(pw)]
+            if px > 0:
+                samples += [(px - 1, py + r, sh) for r in range(ph)]
+            if px > 0 and py > 0:
+                samples.append((px - 1, py - 1, sh))
+        d = set()
+        for sx, sy, sh in samples:
+            o = int(owner[sy >> sh, sx >> sh])
+            if 0 <= o < i and b[o]["is_intra"]:
+                d.add(o)
+        deps[i] = d
+    return deps
+
+
+def check_order(pkg, frame, b):
+    w, h = frame.pf.hdr.width, frame.pf.hdr.height
+    deps = intra_dependencies(b, w, h)
+    buf = np.zeros(max(len(deps), 1), np.uint32)
+    n = pkg.lib().cvp9_hip_intra_order(frame.ptr, buf.ctypes.data_as(pkg.C.POINTER(pkg.C.c_uint32)),
+                                       buf.size)
+    assert n == len(deps)
+    order = buf[:n].tolist()
+    assert sorted(order) == sorted(deps), "not a permutation of the intra blocks"
+    pos = {blk: p for p, blk in enumerate(order)}
+    for blk, d in deps.items():
+        assert all(pos[o] < pos[blk] for o in d), f"block {blk} handed out before a dependency"
+    # level 0 (no intra dependency) comes first, in list order
+    free = sorted(blk for blk, d in deps.items() if not d)
+    assert order[:len(free)] == free
+    return n
+
+
+@pytest.mark.parametrize("preset,w,h", [(1, 320, 184), (2, 643, 362), (0, 136, 72)])
+def test_intra_order_respects_dependencies(pkg, preset, w, h):
+    p = pkg.synth_params(preset, w, h, 21)
+    for idx in range(3):
+        f = pkg.SynthFrame(p, idx)
+        check_order(pkg, f, f.blocks())
+
+
+def test_intra_order_shuffled_lists(pkg):
+    rng = np.random.default_rng(8)
+    p = pkg.synth_params(1, 264, 136, 4)
+    for idx in range(3):
+        f = pkg.SynthFrame(p, idx)
+        b = f.blocks()
+        keep = rng.random(b.size) >= 0.1
+        b = b[keep][rng.permutation(int(keep.sum()))]
+        h = f.pf.hdr
+        g = pkg.OwnedFrame(h.width, h.height, b, f.coeffs(), frame_type=h.frame_type)
+        assert check_order(pkg, g, b) > 0
