@@ -34,7 +34,8 @@ METRIC = "recon frames/s @1080p (1 vs 64 streams), p50/p99 frame latency, HBM GB
 MIX = [(640, 360), (854, 480), (1280, 720), (1920, 1080)]
 
 
-KERNEL_OF = {"intra": "k_intra", "mc": "k_mc", "deblock": "k_lf", "owner": "k_owner",
+KERNEL_OF = {"intra": "k_intra", "neighbours": "k_nbr", "mc": "k_mc", "deblock": "k_lf",
+             "owner": "k_owner",
              "fill": "k_fill", "nv12": "k_nv12", "residual4": "k_residual<4>",
              "residual8": "k_residual<8>", "residual16": "k_residual<16>",
              "residual32": "k_residual<32>"}
@@ -88,6 +89,9 @@ def stage_bytes(P, frame):
         sel = inter & (b["tx_size"] == k)
         out[f"residual{n}"] = int(4 * area[sel].sum())
     out["intra"] = int((bw * bh + 2 * cbw * cbh)[intra].sum() + 4 * area[intra].sum())
+    # neighbour samples gathered and the neighbour record written per intra block
+    ncell = (bw + 3) // 4 + (bh + 3) // 4 + 1
+    out["neighbours"] = int((bw + bh + 1 + 2 * (cbw + cbh + 1) + 8 * ncell + 40)[intra].sum())
     out["deblock"] = 2 * W * H if frame.pf.hdr.filter_level else 0
     out["fill"] = 0
     out["owner"] = int((32 + ((bw + 3) // 4) * ((bh + 3) // 4) * 4)[ok].sum())

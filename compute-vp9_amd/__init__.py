@@ -390,7 +390,7 @@ class HipBackend:
 
 # ---- multi-stream engine (include/cvp9_hip/engine.h) ------------------------------
 STAGE_NAMES = ["fill", "owner", "mc", "residual4", "residual8", "residual16", "residual32",
-               "intra", "deblock", "nv12", "fused"]
+               "neighbours", "intra", "deblock", "nv12", "fused"]
 
 
 class Engine:

@@ -13,13 +13,15 @@
  *   Prepared frame ("prep"): the parsed frame staged once into HBM:
  *     DBlock[nblk]   24-byte block records (clamped, normalised modes)
  *     inter[]        local indices of inter blocks          (motion comp.)
- *     intra[]        local indices of intra blocks, list order
+ *     intra[]        local indices of intra blocks, by dependency level
+ *                    then list order
  *     cls_blk[c][]   inter non-skip blocks with transform size class c
  *     cls_pre[c][]   exclusive prefix of their transform counts
  *     coeffs[]       the parser's int16 coefficients, verbatim
  *
  *   Job: one frame of one stream in a batch: output frame, the 8 reference
- *   slots, the prep, and per-stream scratch (owner map, ready flags).
+ *   slots, the prep, and per-stream scratch (owner map, intra edge and
+ *   neighbour records, intra residuals).
  *   A batch is an array of jobs plus per-stage prefix sums of work items, so
  *   every stage is one launch over all jobs of the batch.
  */
@@ -39,6 +41,25 @@
  *   V bottom [48,56) V right [56,64)   (intra blocks are at most 64x64) */
 #define CVP9_EDGE_WORDS 64
 #define CVP9_MAX_INTRA_DIM 64
+
+/* Neighbour record of an intra block (one per entry of the intra list),
+ * written by the neighbour stage before the ordered intra stage. Cells are
+ * the 4x4 luma cells next to the block: above row (ceil(bw/4) cells), left
+ * column (ceil(bh/4)), then the above-left cell; one 64-bit word each, then
+ * a mask with bit c set when cell c belongs to an earlier intra block.
+ *   resolved cell: bytes 0-3 luma, 4-5 U, 6-7 V samples of the cell on the
+ *     block's side (above: left to right; left: top to bottom; above-left:
+ *     the single samples in bytes 3, 5, 7)
+ *   intra cell: owner block index (bits 0-31), edge side (bit 32: 0 bottom
+ *     row, 1 right column), luma edge word (bits 33-36), index of the
+ *     cell's first chroma sample on the owner's chroma edge (bits 40-44)
+ * followed by the block's local index and a copy of its DBlock, so the
+ * ordered stage reads everything it needs about a block in one round. */
+#define CVP9_NB_CELLS 33
+#define CVP9_NB_MASK  33
+#define CVP9_NB_INDEX 34
+#define CVP9_NB_BLOCK 35   /* 3 words */
+#define CVP9_NB_WORDS 38
 
 /* Motion compensation splits an inter block into strips of rows; a strip
  * is one wave, one quad of 4 luma samples per lane. */
@@ -73,6 +94,10 @@ struct DBlock {          /* 24 bytes */
     uint32_t nx;         /* transforms per row (clamped width / T) */
 };
 
+#ifdef __cplusplus
+static_assert(sizeof(DBlock) == 24, "DBlock is copied as three 64-bit words");
+#endif
+
 struct DFrame {          /* 40 bytes */
     uint8_t *y, *u, *v;
     uint32_t w, h;       /* luma size */
@@ -91,6 +116,7 @@ struct DJob {
     const uint32_t *cls_pre[4];
     uint32_t       *owner;         /* ow x oh cells, local block index */
     uint64_t       *edge;          /* per local block: CVP9_EDGE_WORDS edge words */
+    uint64_t       *nbr;           /* per intra-list entry: CVP9_NB_WORDS words */
     int16_t        *resid;         /* intra residuals, indexed like coeffs */
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
     uint32_t       *err;           /* error word (spin timeout) */

@@ -330,6 +330,7 @@ StreamState::~StreamState()
     }
     hipFree(owner_);
     hipFree(edge_);
+    hipFree(nbr_);
     hipFree(resid_);
     hipFree(err_);
 }
@@ -338,7 +339,7 @@ bool StreamState::scratch_fits(const PrepLayout &L) const
 {
     const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
     return err_ && cells <= owner_cap_ && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
-           (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_;
+           (L.n_intra ? L.n_intra : 1) <= nbr_cap_ && (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_;
 }
 
 hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free)
@@ -434,6 +435,19 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
         if ((e = hipMemsetAsync(edge_, 0, bytes, s)) != hipSuccess) return e;
         edge_cap_ = cap;
     }
+    const size_t ni = L.n_intra ? L.n_intra : 1;
+    if (ni > nbr_cap_) {
+        size_t cap = nbr_cap_ ? nbr_cap_ : 1024;
+        while (cap < ni) cap *= 2;
+        if (nbr_) {
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            hipFree(nbr_);
+            nbr_ = nullptr;
+        }
+        /* rewritten by the neighbour stage before every read */
+        if ((e = hipMalloc(&nbr_, cap * CVP9_NB_WORDS * sizeof(uint64_t))) != hipSuccess) return e;
+        nbr_cap_ = cap;
+    }
     const size_t nc = L.num_coeffs ? L.num_coeffs : 1;
     if (nc > resid_cap_) {
         size_t cap = resid_cap_ ? resid_cap_ : 65536;
@@ -470,6 +484,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     }
     j.owner = owner_;
     j.edge = edge_;
+    j.nbr = nbr_;
     j.resid = resid_;
     j.nv12 = nv12;
     j.err = err_;
@@ -490,7 +505,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
 
 size_t StreamState::resident_bytes() const
 {
-    size_t b = owner_cap_ * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 + resid_cap_ * 2 + (err_ ? 256 : 0);
+    size_t b = owner_cap_ * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 + nbr_cap_ * CVP9_NB_WORDS * 8 +
+               resid_cap_ * 2 + (err_ ? 256 : 0);
     for (const FrameBuf *f : all_) b += f->cap;
     return b;
 }

@@ -69,7 +69,8 @@ public:
     /* back to the state of a new stream (empty slots, no error), keeping
      * device memory; work already queued on s is unaffected */
     hipError_t reset(hipStream_t s);
-    /* owner map, edge records and residual scratch large enough for L;
+    /* owner map, edge and neighbour records and residual scratch large
+     * enough for L;
      * may synchronize s */
     hipError_t ensure_scratch(const PrepLayout &L, hipStream_t s);
     /* job descriptor for one frame of this stream */
@@ -87,6 +88,8 @@ private:
     size_t owner_cap_ = 0;
     uint64_t *edge_ = nullptr;
     size_t edge_cap_ = 0;
+    uint64_t *nbr_ = nullptr;
+    size_t nbr_cap_ = 0;      /* intra blocks */
     int16_t *resid_ = nullptr;
     size_t resid_cap_ = 0;
     uint32_t *err_ = nullptr;

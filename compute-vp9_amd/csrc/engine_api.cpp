@@ -10,6 +10,8 @@
 #include "cvp9_hip/engine.h"
 #include "engine.hpp"
 
+static_assert(CVP9_ENGINE_STAGES == STAGE_COUNT, "stage list of the ABI and the kernels differ");
+
 using namespace cvp9;
 
 struct cvp9_prep {
@@ -45,6 +47,7 @@ bool stage_has_work(int st, const StageCounts &n)
     case STAGE_RES8: return n.grp[1] != 0;
     case STAGE_RES16: return n.grp[2] != 0;
     case STAGE_RES32: return n.grp[3] != 0;
+    case STAGE_NBR:
     case STAGE_INTRA: return n.intra != 0;
     case STAGE_LF: return n.lf != 0;
     case STAGE_NV12: return n.nv12 != 0;

@@ -13,6 +13,7 @@ enum Stage {
     STAGE_RES8,
     STAGE_RES16,
     STAGE_RES32,
+    STAGE_NBR,   /* neighbour records of intra blocks */
     STAGE_INTRA,
     STAGE_LF,
     STAGE_NV12,

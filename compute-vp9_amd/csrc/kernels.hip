@@ -423,41 +423,112 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
 /* job's intra list by level, so their tickets were handed out earlier, to  */
 /* workers that are running, and every wait ends.                           */
 
-__device__ __forceinline__ int nb_sample(const DJob &J, int p, int px, int py, uint32_t b,
-                                         uint32_t spin_limit)
+/* Neighbour stage: one wave per intra block, one lane per neighbour cell.
+ * A cell owned by no block, by a later block or lying in an inter block is
+ * final before the intra stage starts (inter blocks are complete after the
+ * residual stage), so its samples are resolved here, off the ordered path;
+ * a cell of an earlier intra block becomes a pointer into that block's edge
+ * record. A sample of block b owned by an earlier intra block o always lies
+ * on o's bottom row or right column: o covers the sample and not the
+ * sample's neighbour inside b, and blocks are disjoint rectangles. The
+ * chroma samples next to b lie in the same 4x4 luma cells as the luma ones
+ * (x, y are multiples of 4). */
+__device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t lane)
 {
-    const int sh = p == 0 ? 2 : 1; /* plane sample -> 4x4 luma owner cell */
-    const uint32_t o = J.owner[(uint32_t)(py >> sh) * J.ow + (uint32_t)(px >> sh)];
-    if (o == CVP9_OWNER_NONE || o >= b) return 128;
-    const DBlock k = J.blocks[o];
-    if (!(k.flags & DB_INTRA)) {
-        const uint8_t *pl = p == 0 ? J.out.y : (p == 1 ? J.out.u : J.out.v);
-        const uint32_t st = p == 0 ? J.out.sy : J.out.suv;
-        return pl[(size_t)py * st + px];
+    const uint32_t j = find_job(B.pre_intra, B.njobs, wv);
+    const DJob &J = B.jobs[j];
+    const uint32_t tk = wv - B.pre_intra[j];
+    const uint32_t b = J.intra[tk];
+    const DBlock k = J.blocks[b];
+    const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
+    const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
+    const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
+    int kind, q = 0;
+    bool present;
+    if ((int)lane < ncx) { kind = 0; q = lane; present = y > 0; }
+    else if ((int)lane < ncx + ncy) { kind = 1; q = lane - ncx; present = x > 0; }
+    else { kind = 2; present = (int)lane == ncx + ncy && x > 0 && y > 0; }
+    uint64_t val = 0;
+    bool dep = false;
+    if (present) {
+        const int cx = kind == 0 ? (x >> 2) + q : (x >> 2) - 1;
+        const int cy = kind == 1 ? (y >> 2) + q : (y >> 2) - 1;
+        const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
+        if (o == CVP9_OWNER_NONE || o >= b) {
+            val = 0x8080808080808080ull; /* still the initial 128 */
+        } else {
+            const DBlock n = J.blocks[o];
+            if (!(n.flags & DB_INTRA)) {
+                const DFrame &F = J.out;
+                if (kind == 0) {
+                    /* rows are 64-byte aligned and x + 4q < W: the dword and
+                     * the chroma pair stay inside their rows */
+                    val = *(const uint32_t *)(F.y + (size_t)(y - 1) * F.sy + x + 4 * q);
+                    const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
+                    val |= (uint64_t)*(const uint16_t *)(F.u + co) << 32;
+                    val |= (uint64_t)*(const uint16_t *)(F.v + co) << 48;
+                } else if (kind == 1) {
+                    for (int e = 0; e < 4 && 4 * q + e < bh; e++)
+                        val |= (uint64_t)F.y[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
+                    for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
+                        const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
+                        val |= (uint64_t)F.u[co] << (32 + 8 * e);
+                        val |= (uint64_t)F.v[co] << (48 + 8 * e);
+                    }
+                } else {
+                    const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
+                    val = (uint64_t)F.y[(size_t)(y - 1) * F.sy + x - 1] << 24 |
+                          (uint64_t)F.u[co] << 40 | (uint64_t)F.v[co] << 56;
+                }
+            } else {
+                const int ox = n.x, oy = n.y;
+                const bool bottom_row = y - 1 == oy + n.bh - 1, right_col = x - 1 == ox + n.bw - 1;
+                int side, lw, c;
+                if (kind == 0 && bottom_row) {
+                    side = 0; lw = (x + 4 * q - ox) >> 2; c = cbx + 2 * q - (ox >> 1);
+                } else if (kind == 1 && right_col) {
+                    side = 1; lw = (y + 4 * q - oy) >> 2; c = cby + 2 * q - (oy >> 1);
+                } else if (kind == 2 && bottom_row) {
+                    side = 0; lw = (x - 4 - ox) >> 2; c = cbx - 2 - (ox >> 1);
+                } else if (kind == 2 && right_col) {
+                    side = 1; lw = (y - 4 - oy) >> 2; c = cby - 2 - (oy >> 1);
+                } else {
+                    atomicOr(J.err, 2u); /* not on an edge: impossible for disjoint blocks */
+                    side = 0; lw = 0; c = 0;
+                }
+                val = (uint64_t)o | (uint64_t)side << 32 | (uint64_t)(lw & 15) << 33 |
+                      (uint64_t)(c & 31) << 40;
+                dep = true;
+            }
+        }
     }
-    int ox, oy, ow, oh;
-    if (p == 0) {
-        ox = k.x; oy = k.y; ow = k.bw; oh = k.bh;
-    } else {
-        ox = k.x >> 1; oy = k.y >> 1; ow = (k.bw + 1) >> 1; oh = (k.bh + 1) >> 1;
-    }
-    const int bottom = p == 0 ? 0 : (p == 1 ? 32 : 48);
-    int base, idx;
-    if (py == oy + oh - 1) {
-        base = bottom;
-        idx = px - ox;
-    } else if (px == ox + ow - 1) {
-        base = bottom + (p == 0 ? 16 : 8);
-        idx = py - oy;
-    } else {
-        atomicOr(J.err, 2u); /* not on an edge: impossible for disjoint blocks */
-        return 128;
-    }
-    const uint64_t *w = J.edge + (size_t)o * CVP9_EDGE_WORDS + base + (idx >> 2);
+    const uint64_t mask = wave_ballot(dep);
+    uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+    if ((int)lane <= ncx + ncy) rec[lane] = val;
+    if (lane == CVP9_NB_MASK) rec[CVP9_NB_MASK] = mask;
+    if (lane == CVP9_NB_INDEX) rec[CVP9_NB_INDEX] = b;
+    if (lane >= CVP9_NB_BLOCK && lane < CVP9_NB_WORDS)
+        rec[lane] = ((const uint64_t *)(J.blocks + b))[lane - CVP9_NB_BLOCK];
+}
+
+__global__ __launch_bounds__(256) void k_nbr(DBatch B)
+{
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv < B.pre_intra[B.njobs]) nbr_item(B, wv, threadIdx.x & 63);
+}
+
+/* Edge word of an earlier intra block, waited for (bounded) until it
+ * carries this frame's epoch. The poll backs off (64, 512, then 2048
+ * clocks between reads), so thousands of waiting lanes do not crowd the
+ * memory system that the blocks they wait for are using. */
+__device__ __forceinline__ uint64_t edge_word(const DJob &J, const uint64_t *w, uint32_t spin_limit)
+{
     uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
     while ((uint32_t)(v >> 32) != J.epoch) {
-        __builtin_amdgcn_s_sleep(1);
+        if (spins < 4) __builtin_amdgcn_s_sleep(1);
+        else if (spins < 16) __builtin_amdgcn_s_sleep(8);
+        else __builtin_amdgcn_s_sleep(32);
         if (++spins > spin_limit) {
             atomicOr(J.err, 1u);
             break;
@@ -467,7 +538,22 @@ __device__ __forceinline__ int nb_sample(const DJob &J, int p, int px, int py, u
             break;
         v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    return (int)((v >> (8 * (idx & 3))) & 0xff);
+    return v;
+}
+
+/* The samples of an intra cell in the resolved layout (bytes 0-3 luma,
+ * 4-5 U, 6-7 V), read from the owner's edge record. */
+__device__ __forceinline__ uint64_t cell_from_edges(const DJob &J, uint64_t d, uint32_t spin_limit)
+{
+    const uint32_t o = (uint32_t)d;
+    const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
+    const uint32_t c = (uint32_t)(d >> 40) & 31;
+    const uint64_t *rec = J.edge + (size_t)o * CVP9_EDGE_WORDS;
+    const uint64_t wy = edge_word(J, rec + (side ? 16 : 0) + lw, spin_limit);
+    const uint64_t wu = edge_word(J, rec + (side ? 40 : 32) + (c >> 2), spin_limit);
+    const uint64_t wv = edge_word(J, rec + (side ? 56 : 48) + (c >> 2), spin_limit);
+    const uint32_t sh = 8 * (c & 3);
+    return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
 }
 
 __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc)
@@ -480,7 +566,10 @@ __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc
 
 /* Synchronises the threads of one intra worker: a whole workgroup, or a
  * single wave (whose LDS accesses execute in order; the fences keep the
- * compiler from reordering them). */
+ * compiler from reordering them). Only LDS is ordered: threads of a worker
+ * share nothing through global memory (other blocks read a block's samples
+ * only through its edge words, each validated by its epoch), so the
+ * workgroup barrier does not wait for the global stores in flight. */
 template <int W>
 __device__ __forceinline__ void worker_sync()
 {
@@ -489,7 +578,9 @@ __device__ __forceinline__ void worker_sync()
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     } else {
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     }
 }
 
@@ -535,29 +626,76 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
             tk = __builtin_amdgcn_readfirstlane(S.ticket);
             continue;
         }
-        const uint32_t b = J.intra[tk];
-        const DBlock k = J.blocks[b];
+        /* the next ticket of this job, fetched while this block runs; the
+         * tickets a worker holds are its current one and a later one of the
+         * same job, so the lowest unfinished ticket of a job always runs.
+         * (atomicInc: the compiler's wave aggregation of atomicAdd would wait
+         * for the result right here; tickets never reach the wrap limit.) */
+        uint32_t next = 0;
+        if (tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
+        /* block index, block record, dependency mask and this thread's cell
+         * all come from the neighbour record, in one round of loads */
+        const uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+        const uint64_t cell = tid < CVP9_NB_CELLS ? rec[tid] : 0;
+        const uint64_t dmask = rec[CVP9_NB_MASK];
+        const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
+        DBlock k;
+        {
+            const uint64_t k0 = rec[CVP9_NB_BLOCK], k1 = rec[CVP9_NB_BLOCK + 1], k2 = rec[CVP9_NB_BLOCK + 2];
+            uint64_t kw[3] = {k0, k1, k2};
+            memcpy(&k, kw, sizeof(k));
+        }
         const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
         const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
 
-        /* 1. neighbours: 127 outside the frame, else the owner rule */
+        /* residual geometry; this thread's first luma quad of residual is
+         * loaded now, so it arrives while the neighbours are gathered. A
+         * quad with residual lies inside one transform row (c0 is a multiple
+         * of 4, transforms are at least 4 wide). */
+        const int lt = k.tx + 2, T = 1 << lt;
+        const bool res = !(k.flags & DB_SKIP);
+        const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
+        const int nq = (bw + 3) >> 2;
+        auto resid_quad = [&](int i) -> const int16_t * {
+            const int r = i / nq, c0 = (i - r * nq) * 4;
+            if (r >= rrows || c0 >= rcols) return nullptr;
+            return J.resid + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
+                   ((r & (T - 1)) << lt) + (c0 & (T - 1));
+        };
+        int16_t rq0[4] = {0, 0, 0, 0};
+        const int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
+        if (rs0) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) rq0[e] = rs0[e];
+        }
+
+        /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
+         *    the neighbour stage; cells of earlier intra blocks from their
+         *    edge records. ncx + ncy + 1 <= 33 cells: one per thread. */
         if (tid < 3) S.sum[tid] = 0;
-        const int nY = bw + bh + 1, nC = cbw + cbh + 1;
-        for (int i = tid; i < nY + 2 * nC; i += W) {
-            int p, e;
-            if (i < nY) { p = 0; e = i; }
-            else if (i < nY + nC) { p = 1; e = i - nY; }
-            else { p = 2; e = i - nY - nC; }
-            const int px = p ? cbx : x, py = p ? cby : y;
-            const int w = p ? cbw : bw, h = p ? cbh : bh;
-            if (e < w) {
-                S.nb[p][0][e] = (uint8_t)(py > 0 ? nb_sample(J, p, px + e, py - 1, b, B.spin_limit) : 127);
-            } else if (e < w + h) {
-                const int r = e - w;
-                S.nb[p][1][r] = (uint8_t)(px > 0 ? nb_sample(J, p, px - 1, py + r, b, B.spin_limit) : 127);
+        const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
+        if (tid <= ncx + ncy) {
+            const int c = tid;
+            const int kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
+            const bool present = kind == 0 ? y > 0 : (kind == 1 ? x > 0 : (x > 0 && y > 0));
+            uint64_t v = 0x7f7f7f7f7f7f7f7full;
+            if (present) {
+                v = cell;
+                if ((dmask >> c) & 1) v = cell_from_edges(J, v, B.spin_limit);
+            }
+            if (kind == 2) {
+                S.tl[0] = (uint8_t)(v >> 24);
+                S.tl[1] = (uint8_t)(v >> 40);
+                S.tl[2] = (uint8_t)(v >> 56);
             } else {
-                S.tl[p] = (uint8_t)((px > 0 && py > 0) ? nb_sample(J, p, px - 1, py - 1, b, B.spin_limit)
-                                                       : 127);
+                const int q = kind == 0 ? c : c - ncx;
+#pragma unroll
+                for (int e = 0; e < 4; e++) S.nb[0][kind][4 * q + e] = (uint8_t)(v >> (8 * e));
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    S.nb[1][kind][2 * q + e] = (uint8_t)(v >> (32 + 8 * e));
+                    S.nb[2][kind][2 * q + e] = (uint8_t)(v >> (48 + 8 * e));
+                }
             }
         }
         worker_sync<W>();
@@ -578,26 +716,27 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
         {
             const int dc = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
             const int tl = S.tl[0], mode = k.ymode;
-            const int lt = k.tx + 2, T = 1 << lt;
-            const bool res = !(k.flags & DB_SKIP);
-            const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
-            const int nq = (bw + 3) >> 2;
             uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
             for (int i = tid; i < nq * bh; i += W) {
                 const int r = i / nq, c0 = (i - r * nq) * 4;
                 const int l = S.nb[0][1][r];
-                const int16_t *rs = nullptr;
-                if (r < rrows && c0 < rcols)
-                    rs = J.resid + k.coeff +
-                         (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
-                         ((r & (T - 1)) << lt) + (c0 & (T - 1));
+                int16_t rq[4] = {rq0[0], rq0[1], rq0[2], rq0[3]};
+                bool has = rs0 != nullptr;
+                if (i != tid) {
+                    const int16_t *rs = resid_quad(i);
+                    has = rs != nullptr;
+                    if (has) {
+#pragma unroll
+                        for (int e = 0; e < 4; e++) rq[e] = rs[e];
+                    }
+                }
                 uint32_t word = 0;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const int c = c0 + e;
                     if (c < bw) {
                         int v = intra_pred(mode, S.nb[0][0][c], l, tl, dc);
-                        if (rs) v = clampi(v + rs[e], 0, 255);
+                        if (has) v = clampi(v + rq[e], 0, 255);
                         word |= (uint32_t)v << (8 * e);
                         if (r == bh - 1) S.edge[0][0][c] = (uint8_t)v;
                         if (c == bw - 1) S.edge[0][1][r] = (uint8_t)v;
@@ -638,7 +777,7 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
         }
 
         /* next ticket; the sync also orders LDS reuse */
-        if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
+        if (tid == 0) S.ticket = next;
         worker_sync<W>();
         tk = __builtin_amdgcn_readfirstlane(S.ticket);
     }
@@ -822,8 +961,11 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     residual_phase<16, FS16>(B, lds);
     residual_phase<32, FS32>(B, lds);
     grid_barrier(B, ++phase * nwg);
-    /* intra blocks in list order, one-wave workers */
-    if (B.pre_intra[B.njobs]) {
+    /* neighbour records, then intra blocks in ticket order, one-wave workers */
+    const uint32_t nint = B.pre_intra[B.njobs];
+    if (nint) {
+        for (uint32_t wv = blockIdx.x * 4 + wave; wv < nint; wv += nwg * 4) nbr_item(B, wv, lane);
+        grid_barrier(B, ++phase * nwg);
         intra_worker<64>(B, (IntraShared *)smem);
         grid_barrier(B, ++phase * nwg);
     }
@@ -934,6 +1076,9 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         return launch_residual<16>(b, n, 2, s);
     case STAGE_RES32:
         return launch_residual<32>(b, n, 3, s);
+    case STAGE_NBR:
+        if (n.intra) hipLaunchKernelGGL(k_nbr, dim3(div_up(n.intra, 4)), dim3(256), 0, s, b);
+        break;
     case STAGE_INTRA:
         if (n.intra) {
             const uint32_t w = n.intra < n.tune.intra_workers ? n.intra : n.tune.intra_workers;
