@@ -668,6 +668,15 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
 #pragma unroll
             for (int e = 0; e < 4; e++) rq0[e] = rs0[e];
         }
+        /* residual at one luma sample (0 where the block has none) */
+        auto resid_at = [&](int r, int c) -> int {
+            if (r >= rrows || c >= rcols) return 0;
+            return J.resid[k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c >> lt)) << (2 * lt)) +
+                           ((r & (T - 1)) << lt) + (c & (T - 1))];
+        };
+        /* and this thread's first luma edge sample: bottom row, then right
+         * column */
+        const int eres0 = tid < bw ? resid_at(bh - 1, tid) : (tid < bw + bh ? resid_at(tid - bw, bw - 1) : 0);
 
         /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
          *    the neighbour stage; cells of earlier intra blocks from their
@@ -712,9 +721,57 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
         }
         worker_sync<W>();
 
-        /* 3. prediction + residual, luma 4 samples per thread */
+        /* 3. the edge samples first (bottom row and right column of Y, U
+         *    and V), published at once, so blocks waiting on this one go on
+         *    while its interior is written */
+        const int dc0 = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
+        const int ne = bw + bh + 2 * (cbw + cbh);
+        for (int i = tid; i < ne; i += W) {
+            int p, dir, e;
+            if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
+            else {
+                const int q = i - bw - bh;
+                p = q < cbw + cbh ? 1 : 2;
+                const int u = q - (p - 1) * (cbw + cbh);
+                dir = u >= cbw;
+                e = dir ? u - cbw : u;
+            }
+            int v;
+            if (p == 0) {
+                const int r = dir ? e : bh - 1, c = dir ? bw - 1 : e;
+                v = intra_pred(k.ymode, S.nb[0][0][c], S.nb[0][1][r], S.tl[0], dc0);
+                const int rv = i == tid ? eres0 : resid_at(r, c);
+                v = clampi(v + rv, 0, 255);
+            } else {
+                const int r = dir ? e : cbh - 1, c = dir ? cbw - 1 : e;
+                const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
+                                               (uint32_t)(cbw + cbh));
+                v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
+            }
+            S.edge[p][dir][e] = (uint8_t)v;
+        }
+        worker_sync<W>();
+
+        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
+         *    V bottom, V right */
+        for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
+            int p, dir, wi, len;
+            if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
+            else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
+            if (4 * wi < len) {
+                uint64_t d = (uint64_t)J.epoch << 32;
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
+                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+
+        /* 5. the whole block: prediction + residual, luma 4 samples per
+         *    thread; chroma prediction only (the reference adds no chroma
+         *    residual) */
         {
-            const int dc = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
             const int tl = S.tl[0], mode = k.ymode;
             uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
             for (int i = tid; i < nq * bh; i += W) {
@@ -735,16 +792,13 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
                 for (int e = 0; e < 4; e++) {
                     const int c = c0 + e;
                     if (c < bw) {
-                        int v = intra_pred(mode, S.nb[0][0][c], l, tl, dc);
+                        int v = intra_pred(mode, S.nb[0][0][c], l, tl, dc0);
                         if (has) v = clampi(v + rq[e], 0, 255);
                         word |= (uint32_t)v << (8 * e);
-                        if (r == bh - 1) S.edge[0][0][c] = (uint8_t)v;
-                        if (c == bw - 1) S.edge[0][1][r] = (uint8_t)v;
                     }
                 }
                 store_quad(Y + (size_t)r * J.out.sy + c0, word, bw - c0);
             }
-            /* chroma: prediction only (the reference adds no chroma residual) */
             for (int i = tid; i < 2 * cbw * cbh; i += W) {
                 const int p = i < cbw * cbh ? 1 : 2;
                 const int q = i - (p - 1) * cbw * cbh;
@@ -754,25 +808,6 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
                 const int v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
                 uint8_t *pl = p == 1 ? J.out.u : J.out.v;
                 pl[(size_t)(cby + r) * J.out.suv + cbx + c] = (uint8_t)v;
-                if (r == cbh - 1) S.edge[p][0][c] = (uint8_t)v;
-                if (c == cbw - 1) S.edge[p][1][r] = (uint8_t)v;
-            }
-        }
-        worker_sync<W>();
-
-        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
-         *    V bottom, V right */
-        for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
-            int p, dir, wi, len;
-            if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
-            else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
-            if (4 * wi < len) {
-                uint64_t d = (uint64_t)J.epoch << 32;
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
-                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
             }
         }
 
