@@ -239,7 +239,8 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     DJob job = b->st->make_job(L, s.stage_dev, out, b->pending_target);
     if (b->debug & CVP9_HIP_DEBUG_NO_DEBLOCK) job.lf_thr = 0;
     BatchBuilder bb;
-    bb.build(&job, 1, s.stage_host + batch_off, s.stage_dev + batch_off, b->tune, kSpinLimit);
+    const PrepLayout *lay = &L;
+    bb.build(&job, &lay, 1, s.stage_host + batch_off, s.stage_dev + batch_off, b->tune, kSpinLimit);
 
     if ((e = hipMemcpyAsync(s.stage_dev, s.stage_host, batch_off + bb.used(), hipMemcpyHostToDevice,
                             b->stream)) != hipSuccess)

@@ -33,6 +33,7 @@
 #define CVP9_CELL 4                 /* owner-map granularity in luma px */
 #define CVP9_OWNER_NONE 0xffffffffu
 #define CVP9_MAX_BATCH 4096
+#define CVP9_MAX_HEAD_LEVELS 64     /* leading intra levels launched one by one */
 
 /* Edge record of an intra block: its final bottom row and right column of
  * Y, U and V, 4 samples per 64-bit word with the frame epoch in the upper 32
@@ -140,8 +141,14 @@ struct DBatch {
     const uint32_t *pre_lf;        /* deblocking windows */
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
-    uint32_t       *jticket;       /* per-job intra ticket counters (zeroed by the descriptor copy) */
+    uint32_t       *jticket;       /* per-job intra ticket counters, set by the descriptor copy
+                                      to where the persistent workers start */
     uint32_t       *bar;           /* grid-barrier counter of the fused kernel (zeroed likewise) */
+    /* leading levels of the intra stage, per dependency level l (rows of
+     * njobs + 1): exclusive prefix over jobs of the blocks on level l, and
+     * each job's intra-list position where level l starts */
+    const uint32_t *lvl_pre;
+    const uint32_t *lvl_start;
     uint32_t        spin_limit;    /* bound on every dependency wait */
 };
 

@@ -183,9 +183,15 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
  * the owner's level (plus one) in the scratch map, in list order: a cell
  * still at zero when block b reads it is not owned by an earlier intra
  * block. */
-static uint32_t order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint32_t W, uint32_t H)
+static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint32_t W, uint32_t H,
+                        std::vector<uint32_t> &levels)
 {
-    if (ni < 2) return ni ? 1 : 0;
+    levels.clear();
+    if (ni == 0) return;
+    if (ni == 1) {
+        levels = {0, 1};
+        return;
+    }
     const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
     thread_local std::vector<uint32_t> map; /* level + 1 per 4x4 cell, zero between frames */
     thread_local std::vector<uint32_t> lev, cnt, tmp;
@@ -219,13 +225,13 @@ static uint32_t order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, 
     cnt.assign(depth + 1, 0);
     for (uint32_t i = 0; i < ni; i++) cnt[lev[i] + 1]++;
     for (uint32_t l = 0; l < depth; l++) cnt[l + 1] += cnt[l];
+    levels.assign(cnt.begin(), cnt.end());
     tmp.resize(ni);
     for (uint32_t i = 0; i < ni; i++) tmp[cnt[lev[i]]++] = intra[i];
     memcpy(intra, tmp.data(), (size_t)ni * 4);
-    return depth;
 }
 
-void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
+void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
 {
     const int W = (int)L.W, H = (int)L.H;
     DBlock *blocks = (DBlock *)(dst + L.off_blocks);
@@ -291,7 +297,7 @@ void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst)
     }
     for (int k = 0; k < 4; k++) cpre[k][cn[k]] = ctx[k];
     mc_pre[nn] = mc_acc;
-    order_intra(blocks, intra, ni, L.W, L.H);
+    order_intra(blocks, intra, ni, L.W, L.H, L.levels);
     if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
 }
 
@@ -515,11 +521,12 @@ size_t StreamState::resident_bytes() const
 
 size_t BatchBuilder::bytes_for(uint32_t njobs)
 {
-    return align256((uint64_t)njobs * sizeof(DJob)) + align256(12ull * (njobs + 1) * 4) + 256;
+    return align256((uint64_t)njobs * sizeof(DJob)) + align256(12ull * (njobs + 1) * 4) + 256 +
+           2ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
 }
 
-void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_t *dev,
-                         const Tune &tune, uint32_t spin_limit)
+void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uint32_t njobs,
+                         uint8_t *host, uint8_t *dev, const Tune &tune, uint32_t spin_limit)
 {
     memcpy(host, jobs, (size_t)njobs * sizeof(DJob));
     const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
@@ -527,7 +534,7 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
     const uint32_t stride = njobs + 1;
     enum { P_FILL, P_BLK, P_INTER, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
     for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
-    memset(pre + P_N * stride, 0, (size_t)stride * 4); /* row P_N: intra tickets */
+
     for (uint32_t i = 0; i < njobs; i++) {
         const DJob &J = jobs[i];
         uint32_t c[P_N];
@@ -570,6 +577,52 @@ void BatchBuilder::build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_
     n_.nv12 = pre[P_NV12 * stride + njobs];
     n_.tune = tune;
     used_ = tk_off + 256;
+
+    /* Leading levels at least tune.intra_level_min blocks wide (over the
+     * batch) are launched one by one; the persistent workers take the rest,
+     * each job's tickets starting where its first remaining level starts. */
+    uint32_t *tick = pre + P_N * stride; /* row P_N: per-job intra tickets */
+    uint32_t head = 0;
+    level_n_.clear();
+    if (tune.intra_level_min > 0) {
+        for (; head < CVP9_MAX_HEAD_LEVELS; head++) {
+            uint32_t c = 0;
+            for (uint32_t i = 0; i < njobs; i++) {
+                const std::vector<uint32_t> &lv = layouts[i]->levels;
+                if (head + 1 < lv.size()) c += lv[head + 1] - lv[head];
+            }
+            if (c == 0 || c < tune.intra_level_min) break;
+            level_n_.push_back(c);
+        }
+    }
+    uint32_t tail = 0;
+    for (uint32_t i = 0; i < njobs; i++) {
+        const std::vector<uint32_t> &lv = layouts[i]->levels;
+        tick[i] = lv.empty() ? 0 : lv[std::min<size_t>(head, lv.size() - 1)];
+        tail += jobs[i].n_intra - tick[i];
+    }
+    n_.levels = head;
+    n_.level_n = head ? level_n_.data() : nullptr;
+    n_.intra_tail = tail;
+    b_.lvl_pre = b_.lvl_start = nullptr;
+    if (head) {
+        uint32_t *lp = (uint32_t *)(host + used_), *ls = lp + (size_t)head * stride;
+        for (uint32_t l = 0; l < head; l++) {
+            uint32_t acc = 0;
+            for (uint32_t i = 0; i < njobs; i++) {
+                const std::vector<uint32_t> &lv = layouts[i]->levels;
+                const bool has = l + 1 < lv.size();
+                lp[(size_t)l * stride + i] = acc;
+                ls[(size_t)l * stride + i] = has ? lv[l] : 0;
+                acc += has ? lv[l + 1] - lv[l] : 0;
+            }
+            lp[(size_t)l * stride + njobs] = acc;
+            ls[(size_t)l * stride + njobs] = 0;
+        }
+        b_.lvl_pre = (const uint32_t *)(dev + used_);
+        b_.lvl_start = b_.lvl_pre + (size_t)head * stride;
+        used_ += 2ull * head * stride * 4;
+    }
 }
 
 hipError_t BatchBuilder::launch(const DBatch &b, const StageCounts &n, hipStream_t s)

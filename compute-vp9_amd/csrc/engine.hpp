@@ -31,14 +31,17 @@ struct PrepLayout {
     uint64_t off_blocks = 0, off_inter = 0, off_intra = 0, off_mc_pre = 0;
     uint64_t off_cls_blk[4] = {0, 0, 0, 0}, off_cls_pre[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
+    /* written by plan_write: intra-list position where each dependency
+     * level starts, then the list's end (empty without intra blocks) */
+    std::vector<uint32_t> levels;
 };
 
 /* Validates pf and fills L. Returns CVP9_OK, CVP9_ERR_UNSUPPORTED (layout
  * this backend does not take: the caller falls back to the CPU path) or
  * CVP9_ERR_INVALID_DATA (coefficient references outside the pool). */
 cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L);
-/* Writes the device image of pf into dst (L.bytes bytes). */
-void plan_write(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst);
+/* Writes the device image of pf into dst (L.bytes bytes) and L.levels. */
+void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst);
 
 /* Device frame buffer: planes Y, U, V back to back, 64-byte row pitch. */
 struct FrameBuf {
@@ -100,10 +103,12 @@ private:
  * buffer mirrored at a device address, and launches the stage sequence. */
 class BatchBuilder {
 public:
+    /* descriptor bytes for njobs jobs */
     static size_t bytes_for(uint32_t njobs);
-    /* host/dev point to bytes_for(njobs) bytes each */
-    void build(const DJob *jobs, uint32_t njobs, uint8_t *host, uint8_t *dev,
-               const Tune &tune, uint32_t spin_limit);
+    /* host/dev point to bytes_for(njobs) bytes each; layouts[i] is the plan
+     * of jobs[i] */
+    void build(const DJob *jobs, const PrepLayout *const *layouts, uint32_t njobs, uint8_t *host,
+               uint8_t *dev, const Tune &tune, uint32_t spin_limit);
     size_t used() const { return used_; }
     const DBatch &batch() const { return b_; }
     const StageCounts &counts() const { return n_; }
@@ -113,6 +118,7 @@ private:
     DBatch b_{};
     StageCounts n_{};
     size_t used_ = 0;
+    std::vector<uint32_t> level_n_;
 };
 
 /* Once per device: upload the IDCT matrices (computed with the C library's

@@ -334,7 +334,11 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
         ddev = bs->dev;
     }
     BatchBuilder bb;
-    bb.build(jobs.data(), (uint32_t)n, dhost, ddev, e->tune, kSpinLimit);
+    std::vector<const PrepLayout *> lays((size_t)n);
+    for (int i = 0; i < n; i++) lays[i] = &preps[i]->L;
+    Tune tune = e->tune;
+    if (e->fused) tune.intra_level_min = 0; /* the fused kernel runs persistent workers only */
+    bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, kSpinLimit);
     if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, e->stream)) != hipSuccess)
         return to_err(he);
     const bool timing = e->timing && !e->capture;
@@ -489,6 +493,7 @@ cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t)
     for (int i = 0; i < 4; i++) t->res_split[i] = e->tune.res_split[i];
     t->intra_lanes = e->tune.intra_lanes;
     t->intra_workers = e->tune.intra_workers;
+    t->intra_level_min = e->tune.intra_level_min;
     return CVP9_OK;
 }
 
@@ -503,6 +508,7 @@ cvp9_err_t cvp9_engine_set_tune(cvp9_engine_t *e, const cvp9_tune_t *t)
     for (int i = 0; i < 4; i++) e->tune.res_split[i] = t->res_split[i];
     e->tune.intra_lanes = t->intra_lanes;
     e->tune.intra_workers = t->intra_workers;
+    e->tune.intra_level_min = t->intra_level_min;
     return CVP9_OK;
 }
 
@@ -518,9 +524,11 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
     const uint32_t cu = (uint32_t)(e->cus > 0 ? e->cus : 256);
     /* each candidate run measures every stage at once; the transform split
      * is uniform over sizes inside a run and chosen per size afterwards */
-    struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; };
+    struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; uint32_t level_min; };
     const Cand cands[] = {
-        {1, 256, 2 * cu}, {2, 64, 8 * cu}, {4, 64, 16 * cu}, {1, 256, 4 * cu}, {2, 64, 4 * cu},
+        {1, 256, 2 * cu, 0}, {2, 64, 8 * cu, 0}, {4, 64, 16 * cu, 0}, {1, 256, 4 * cu, 0},
+        {2, 64, 4 * cu, 0},  {1, 64, 2 * cu, 2 * cu}, {2, 256, 2 * cu, 2 * cu},
+        {1, 256, 2 * cu, 8 * cu},
     };
     const bool was_timing = e->timing;
     const Tune saved = e->tune;
@@ -532,6 +540,7 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
         for (int k = 0; k < 4; k++) e->tune.res_split[k] = c.split;
         e->tune.intra_lanes = c.lanes;
         e->tune.intra_workers = c.workers;
+        e->tune.intra_level_min = c.level_min;
         e->timing = false;
         if ((err = cvp9_engine_run(e, n, streams, preps, nullptr)) != CVP9_OK) break; /* warm */
         e->timing = true;
@@ -555,6 +564,7 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
                 best_intra = t;
                 out.intra_lanes = c.lanes;
                 out.intra_workers = c.workers;
+                out.intra_level_min = c.level_min;
             }
         }
     }
@@ -564,6 +574,7 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
         for (int k = 0; k < 4; k++) best->res_split[k] = out.res_split[k];
         best->intra_lanes = out.intra_lanes;
         best->intra_workers = out.intra_workers;
+        best->intra_level_min = out.intra_level_min;
     }
     return err;
 }

@@ -26,12 +26,19 @@ struct Tune {
     uint8_t res_split[4];    /* waves sharing one transform group, N = 4,8,16,32: 1, 2 or 4 */
     uint16_t intra_lanes;    /* threads per intra worker: 64 (one wave) or 256 */
     uint32_t intra_workers;  /* persistent intra workers */
+    uint32_t intra_level_min; /* leading dependency levels holding at least this many
+                                 blocks over the batch run as one launch per level (up
+                                 to CVP9_MAX_HEAD_LEVELS), the rest on persistent
+                                 workers; 0: persistent workers only */
 };
 
 /* Total work items of each stage over the whole batch (the last element of
  * each prefix array in DBatch), plus launch parameters. */
 struct StageCounts {
     uint32_t fill, blocks, inter, grp[4], intra, lf, nv12;
+    uint32_t levels;          /* leading levels launched one by one */
+    const uint32_t *level_n;  /* host array: intra blocks per leading level over the batch */
+    uint32_t intra_tail;      /* intra blocks left to the persistent workers */
     Tune tune;
 };
 

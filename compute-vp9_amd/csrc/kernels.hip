@@ -90,7 +90,6 @@ __global__ __launch_bounds__(256) void k_fill(DBatch B)
 
 /* ------------------------------------------------------------------------ */
 /* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
-/* Also resets the per-job intra ticket counters of this batch.             */
 
 __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_t lane)
 {
@@ -109,8 +108,6 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
 
 __global__ __launch_bounds__(256) void k_owner(DBatch B)
 {
-    if (blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < B.njobs; i += 256) B.jticket[i] = 0;
     const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wv < B.pre_blocks[B.njobs]) owner_item(B, wv, threadIdx.x & 63);
 }
@@ -592,8 +589,246 @@ struct IntraShared {
     uint8_t tl[3];
 };
 
-/* W threads per worker: 256 (one workgroup per block) or 64 (one wave per
- * block, four independent workers per workgroup). */
+/* One intra block: entry tk of job J's intra list, W threads (tid) sharing
+ * S. EARLY publishes the edge record before writing the block (for workers
+ * that others wait on); otherwise the edges are collected while the block
+ * is written and published at the end. */
+template <int W, bool EARLY>
+__device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint32_t tk,
+                                            IntraShared &S, int tid)
+{
+    /* block index, block record, dependency mask and this thread's cell
+     * all come from the neighbour record, in one round of loads */
+    const uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+    const uint64_t cell = tid < CVP9_NB_CELLS ? rec[tid] : 0;
+    const uint64_t dmask = rec[CVP9_NB_MASK];
+    const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
+    DBlock k;
+    {
+        const uint64_t k0 = rec[CVP9_NB_BLOCK], k1 = rec[CVP9_NB_BLOCK + 1], k2 = rec[CVP9_NB_BLOCK + 2];
+        uint64_t kw[3] = {k0, k1, k2};
+        memcpy(&k, kw, sizeof(k));
+    }
+    const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
+    const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
+
+    /* residual geometry; this thread's first luma quad of residual is
+     * loaded now, so it arrives while the neighbours are gathered. A
+     * quad with residual lies inside one transform row (c0 is a multiple
+     * of 4, transforms are at least 4 wide). */
+    const int lt = k.tx + 2, T = 1 << lt;
+    const bool res = !(k.flags & DB_SKIP);
+    const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
+    const int nq = (bw + 3) >> 2;
+    auto resid_quad = [&](int i) -> const int16_t * {
+        const int r = i / nq, c0 = (i - r * nq) * 4;
+        if (r >= rrows || c0 >= rcols) return nullptr;
+        return J.resid + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
+               ((r & (T - 1)) << lt) + (c0 & (T - 1));
+    };
+    int16_t rq0[4] = {0, 0, 0, 0};
+    const int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
+    if (rs0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) rq0[e] = rs0[e];
+    }
+    /* residual at one luma sample (0 where the block has none) */
+    auto resid_at = [&](int r, int c) -> int {
+        if (r >= rrows || c >= rcols) return 0;
+        return J.resid[k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c >> lt)) << (2 * lt)) +
+                       ((r & (T - 1)) << lt) + (c & (T - 1))];
+    };
+    /* and this thread's first luma edge sample: bottom row, then right
+     * column */
+    const int eres0 = !EARLY ? 0
+                             : tid < bw ? resid_at(bh - 1, tid)
+                                        : (tid < bw + bh ? resid_at(tid - bw, bw - 1) : 0);
+
+    /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
+     *    the neighbour stage; cells of earlier intra blocks from their
+     *    edge records. ncx + ncy + 1 <= 33 cells: one per thread. */
+    if (tid < 3) S.sum[tid] = 0;
+    const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
+    if (tid <= ncx + ncy) {
+        const int c = tid;
+        const int kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
+        const bool present = kind == 0 ? y > 0 : (kind == 1 ? x > 0 : (x > 0 && y > 0));
+        uint64_t v = 0x7f7f7f7f7f7f7f7full;
+        if (present) {
+            v = cell;
+            if ((dmask >> c) & 1) v = cell_from_edges(J, v, B.spin_limit);
+        }
+        if (kind == 2) {
+            S.tl[0] = (uint8_t)(v >> 24);
+            S.tl[1] = (uint8_t)(v >> 40);
+            S.tl[2] = (uint8_t)(v >> 56);
+        } else {
+            const int q = kind == 0 ? c : c - ncx;
+#pragma unroll
+            for (int e = 0; e < 4; e++) S.nb[0][kind][4 * q + e] = (uint8_t)(v >> (8 * e));
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                S.nb[1][kind][2 * q + e] = (uint8_t)(v >> (32 + 8 * e));
+                S.nb[2][kind][2 * q + e] = (uint8_t)(v >> (48 + 8 * e));
+            }
+        }
+    }
+    worker_sync<W>();
+
+    /* 2. DC sums over w above + h left samples */
+    for (int i = tid; i < 3 * 128; i += W) {
+        const int p = i >> 7, e = i & 127;
+        if ((p == 0 ? k.ymode : k.uvmode) != 0) continue;
+        const int w = p ? cbw : bw, h = p ? cbh : bh;
+        uint32_t v = 0;
+        if (e < 64) { if (e < w) v = S.nb[p][0][e]; }
+        else if (e - 64 < h) v = S.nb[p][1][e - 64];
+        if (v) atomicAdd(&S.sum[p], v);
+    }
+    worker_sync<W>();
+
+    const int dc0 = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
+    if constexpr (EARLY) {
+        /* 3. with EARLY, the edge samples first (bottom row and right
+         *    column of Y, U and V), published at once, so blocks waiting on
+         *    this one go on while its interior is written */
+        const int ne = bw + bh + 2 * (cbw + cbh);
+        for (int i = tid; i < ne; i += W) {
+            int p, dir, e;
+            if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
+            else {
+                const int q = i - bw - bh;
+                p = q < cbw + cbh ? 1 : 2;
+                const int u = q - (p - 1) * (cbw + cbh);
+                dir = u >= cbw;
+                e = dir ? u - cbw : u;
+            }
+            int v;
+            if (p == 0) {
+                const int r = dir ? e : bh - 1, c = dir ? bw - 1 : e;
+                v = intra_pred(k.ymode, S.nb[0][0][c], S.nb[0][1][r], S.tl[0], dc0);
+                const int rv = i == tid ? eres0 : resid_at(r, c);
+                v = clampi(v + rv, 0, 255);
+            } else {
+                const int r = dir ? e : cbh - 1, c = dir ? cbw - 1 : e;
+                const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
+                                               (uint32_t)(cbw + cbh));
+                v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
+            }
+            S.edge[p][dir][e] = (uint8_t)v;
+        }
+        worker_sync<W>();
+
+        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
+         *    V bottom, V right */
+        for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
+            int p, dir, wi, len;
+            if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
+            else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
+            if (4 * wi < len) {
+                uint64_t d = (uint64_t)J.epoch << 32;
+    #pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
+                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+
+    /* 5. the whole block: prediction + residual, 4 luma samples per item,
+     *    four items' residuals loaded before any is used (so a 64-thread
+     *    worker on a large block waits once per four items, not per item);
+     *    chroma prediction only (the reference adds no chroma residual), 4
+     *    samples per item. Without EARLY the edge samples are collected
+     *    here and published after. */
+    {
+        const int tl = S.tl[0], mode = k.ymode;
+        uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
+        const int nitems = nq * bh;
+        for (int i0 = tid; i0 < nitems; i0 += 4 * W) {
+            int16_t rq[4][4];
+            bool has[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * W;
+                const bool first = i == tid; /* loaded before the gather */
+                const int16_t *rs = first ? rs0 : (i < nitems ? resid_quad(i) : nullptr);
+                has[u] = rs != nullptr;
+#pragma unroll
+                for (int e = 0; e < 4; e++) rq[u][e] = first ? rq0[e] : (rs ? rs[e] : (int16_t)0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * W;
+                if (i >= nitems) break;
+                const int r = i / nq, c0 = (i - r * nq) * 4;
+                const int l = S.nb[0][1][r];
+                uint32_t word = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int c = c0 + e;
+                    if (c < bw) {
+                        int v = intra_pred(mode, S.nb[0][0][c], l, tl, dc0);
+                        if (has[u]) v = clampi(v + rq[u][e], 0, 255);
+                        word |= (uint32_t)v << (8 * e);
+                        if constexpr (!EARLY) {
+                            if (r == bh - 1) S.edge[0][0][c] = (uint8_t)v;
+                            if (c == bw - 1) S.edge[0][1][r] = (uint8_t)v;
+                        }
+                    }
+                }
+                store_quad(Y + (size_t)r * J.out.sy + c0, word, bw - c0);
+            }
+        }
+        const int nqc = (cbw + 3) >> 2;
+        for (int i = tid; i < 2 * nqc * cbh; i += W) {
+            const int p = i < nqc * cbh ? 1 : 2;
+            const int q = i - (p - 1) * nqc * cbh;
+            const int r = q / nqc, c0 = (q - r * nqc) * 4;
+            const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
+                                           (uint32_t)(cbw + cbh));
+            const int l = S.nb[p][1][r];
+            uint32_t word = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int c = c0 + e;
+                if (c < cbw) {
+                    const int v = intra_pred(k.uvmode, S.nb[p][0][c], l, S.tl[p], dcc);
+                    word |= (uint32_t)v << (8 * e);
+                    if constexpr (!EARLY) {
+                        if (r == cbh - 1) S.edge[p][0][c] = (uint8_t)v;
+                        if (c == cbw - 1) S.edge[p][1][r] = (uint8_t)v;
+                    }
+                }
+            }
+            uint8_t *pl = p == 1 ? J.out.u : J.out.v;
+            store_quad(pl + (size_t)(cby + r) * J.out.suv + cbx + c0, word, cbw - c0);
+        }
+    }
+    if constexpr (!EARLY) {
+        worker_sync<W>();
+
+        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
+         *    V bottom, V right */
+        for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
+            int p, dir, wi, len;
+            if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
+            else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
+            if (4 * wi < len) {
+                uint64_t d = (uint64_t)J.epoch << 32;
+    #pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
+                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
+/* Persistent workers, W threads each: 256 (one workgroup per block) or 64
+ * (one wave per block, four independent workers per workgroup). */
 template <int W>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
 {
@@ -633,183 +868,7 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
          * for the result right here; tickets never reach the wrap limit.) */
         uint32_t next = 0;
         if (tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
-        /* block index, block record, dependency mask and this thread's cell
-         * all come from the neighbour record, in one round of loads */
-        const uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
-        const uint64_t cell = tid < CVP9_NB_CELLS ? rec[tid] : 0;
-        const uint64_t dmask = rec[CVP9_NB_MASK];
-        const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
-        DBlock k;
-        {
-            const uint64_t k0 = rec[CVP9_NB_BLOCK], k1 = rec[CVP9_NB_BLOCK + 1], k2 = rec[CVP9_NB_BLOCK + 2];
-            uint64_t kw[3] = {k0, k1, k2};
-            memcpy(&k, kw, sizeof(k));
-        }
-        const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
-        const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
-
-        /* residual geometry; this thread's first luma quad of residual is
-         * loaded now, so it arrives while the neighbours are gathered. A
-         * quad with residual lies inside one transform row (c0 is a multiple
-         * of 4, transforms are at least 4 wide). */
-        const int lt = k.tx + 2, T = 1 << lt;
-        const bool res = !(k.flags & DB_SKIP);
-        const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
-        const int nq = (bw + 3) >> 2;
-        auto resid_quad = [&](int i) -> const int16_t * {
-            const int r = i / nq, c0 = (i - r * nq) * 4;
-            if (r >= rrows || c0 >= rcols) return nullptr;
-            return J.resid + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
-                   ((r & (T - 1)) << lt) + (c0 & (T - 1));
-        };
-        int16_t rq0[4] = {0, 0, 0, 0};
-        const int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
-        if (rs0) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) rq0[e] = rs0[e];
-        }
-        /* residual at one luma sample (0 where the block has none) */
-        auto resid_at = [&](int r, int c) -> int {
-            if (r >= rrows || c >= rcols) return 0;
-            return J.resid[k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c >> lt)) << (2 * lt)) +
-                           ((r & (T - 1)) << lt) + (c & (T - 1))];
-        };
-        /* and this thread's first luma edge sample: bottom row, then right
-         * column */
-        const int eres0 = tid < bw ? resid_at(bh - 1, tid) : (tid < bw + bh ? resid_at(tid - bw, bw - 1) : 0);
-
-        /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
-         *    the neighbour stage; cells of earlier intra blocks from their
-         *    edge records. ncx + ncy + 1 <= 33 cells: one per thread. */
-        if (tid < 3) S.sum[tid] = 0;
-        const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
-        if (tid <= ncx + ncy) {
-            const int c = tid;
-            const int kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
-            const bool present = kind == 0 ? y > 0 : (kind == 1 ? x > 0 : (x > 0 && y > 0));
-            uint64_t v = 0x7f7f7f7f7f7f7f7full;
-            if (present) {
-                v = cell;
-                if ((dmask >> c) & 1) v = cell_from_edges(J, v, B.spin_limit);
-            }
-            if (kind == 2) {
-                S.tl[0] = (uint8_t)(v >> 24);
-                S.tl[1] = (uint8_t)(v >> 40);
-                S.tl[2] = (uint8_t)(v >> 56);
-            } else {
-                const int q = kind == 0 ? c : c - ncx;
-#pragma unroll
-                for (int e = 0; e < 4; e++) S.nb[0][kind][4 * q + e] = (uint8_t)(v >> (8 * e));
-#pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    S.nb[1][kind][2 * q + e] = (uint8_t)(v >> (32 + 8 * e));
-                    S.nb[2][kind][2 * q + e] = (uint8_t)(v >> (48 + 8 * e));
-                }
-            }
-        }
-        worker_sync<W>();
-
-        /* 2. DC sums over w above + h left samples */
-        for (int i = tid; i < 3 * 128; i += W) {
-            const int p = i >> 7, e = i & 127;
-            if ((p == 0 ? k.ymode : k.uvmode) != 0) continue;
-            const int w = p ? cbw : bw, h = p ? cbh : bh;
-            uint32_t v = 0;
-            if (e < 64) { if (e < w) v = S.nb[p][0][e]; }
-            else if (e - 64 < h) v = S.nb[p][1][e - 64];
-            if (v) atomicAdd(&S.sum[p], v);
-        }
-        worker_sync<W>();
-
-        /* 3. the edge samples first (bottom row and right column of Y, U
-         *    and V), published at once, so blocks waiting on this one go on
-         *    while its interior is written */
-        const int dc0 = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
-        const int ne = bw + bh + 2 * (cbw + cbh);
-        for (int i = tid; i < ne; i += W) {
-            int p, dir, e;
-            if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
-            else {
-                const int q = i - bw - bh;
-                p = q < cbw + cbh ? 1 : 2;
-                const int u = q - (p - 1) * (cbw + cbh);
-                dir = u >= cbw;
-                e = dir ? u - cbw : u;
-            }
-            int v;
-            if (p == 0) {
-                const int r = dir ? e : bh - 1, c = dir ? bw - 1 : e;
-                v = intra_pred(k.ymode, S.nb[0][0][c], S.nb[0][1][r], S.tl[0], dc0);
-                const int rv = i == tid ? eres0 : resid_at(r, c);
-                v = clampi(v + rv, 0, 255);
-            } else {
-                const int r = dir ? e : cbh - 1, c = dir ? cbw - 1 : e;
-                const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
-                                               (uint32_t)(cbw + cbh));
-                v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
-            }
-            S.edge[p][dir][e] = (uint8_t)v;
-        }
-        worker_sync<W>();
-
-        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
-         *    V bottom, V right */
-        for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
-            int p, dir, wi, len;
-            if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
-            else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
-            if (4 * wi < len) {
-                uint64_t d = (uint64_t)J.epoch << 32;
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
-                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-
-        /* 5. the whole block: prediction + residual, luma 4 samples per
-         *    thread; chroma prediction only (the reference adds no chroma
-         *    residual) */
-        {
-            const int tl = S.tl[0], mode = k.ymode;
-            uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
-            for (int i = tid; i < nq * bh; i += W) {
-                const int r = i / nq, c0 = (i - r * nq) * 4;
-                const int l = S.nb[0][1][r];
-                int16_t rq[4] = {rq0[0], rq0[1], rq0[2], rq0[3]};
-                bool has = rs0 != nullptr;
-                if (i != tid) {
-                    const int16_t *rs = resid_quad(i);
-                    has = rs != nullptr;
-                    if (has) {
-#pragma unroll
-                        for (int e = 0; e < 4; e++) rq[e] = rs[e];
-                    }
-                }
-                uint32_t word = 0;
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int c = c0 + e;
-                    if (c < bw) {
-                        int v = intra_pred(mode, S.nb[0][0][c], l, tl, dc0);
-                        if (has) v = clampi(v + rq[e], 0, 255);
-                        word |= (uint32_t)v << (8 * e);
-                    }
-                }
-                store_quad(Y + (size_t)r * J.out.sy + c0, word, bw - c0);
-            }
-            for (int i = tid; i < 2 * cbw * cbh; i += W) {
-                const int p = i < cbw * cbh ? 1 : 2;
-                const int q = i - (p - 1) * cbw * cbh;
-                const int r = q / cbw, c = q - r * cbw;
-                const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
-                                               (uint32_t)(cbw + cbh));
-                const int v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
-                uint8_t *pl = p == 1 ? J.out.u : J.out.v;
-                pl[(size_t)(cby + r) * J.out.suv + cbx + c] = (uint8_t)v;
-            }
-        }
+        intra_block<W, true>(B, J, tk, S, tid);
 
         /* next ticket; the sync also orders LDS reuse */
         if (tid == 0) S.ticket = next;
@@ -823,6 +882,23 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
 {
     __shared__ IntraShared shm[256 / W];
     intra_worker<W>(B, shm);
+}
+
+/* Level-synchronous alternative: one launch per dependency level, one
+ * worker per intra block of that level across all jobs. Every block a
+ * block depends on lies on a lower level, finished by an earlier launch, so
+ * no worker waits (its edge words already carry the frame's epoch). */
+template <int W>
+__global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level)
+{
+    __shared__ IntraShared shm[256 / W];
+    const uint32_t *pre = B.lvl_pre + (size_t)level * (B.njobs + 1);
+    const uint32_t *start = B.lvl_start + (size_t)level * (B.njobs + 1);
+    const uint32_t g = blockIdx.x * (256 / W) + threadIdx.x / W;
+    if (g >= pre[B.njobs]) return; /* uniform per worker */
+    const uint32_t j = find_job(pre, B.njobs, g);
+    intra_block<W, false>(B, B.jobs[j], start[j] + (g - pre[j]), shm[threadIdx.x / W],
+                          threadIdx.x % W);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1075,6 +1151,7 @@ Tune cvp9_default_tune(int cus)
     t.res_split[3] = 4;
     t.intra_lanes = 256;
     t.intra_workers = 2u * (uint32_t)(cus > 0 ? cus : 256);
+    t.intra_level_min = 0;
     return t;
 }
 
@@ -1115,8 +1192,15 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         if (n.intra) hipLaunchKernelGGL(k_nbr, dim3(div_up(n.intra, 4)), dim3(256), 0, s, b);
         break;
     case STAGE_INTRA:
-        if (n.intra) {
-            const uint32_t w = n.intra < n.tune.intra_workers ? n.intra : n.tune.intra_workers;
+        for (uint32_t l = 0; l < n.levels; l++) {
+            const uint32_t c = n.level_n[l];
+            if (n.tune.intra_lanes == 64)
+                hipLaunchKernelGGL(k_intra_level<64>, dim3(div_up(c, 4)), dim3(256), 0, s, b, l);
+            else
+                hipLaunchKernelGGL(k_intra_level<256>, dim3(c), dim3(256), 0, s, b, l);
+        }
+        if (n.intra_tail) {
+            const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;
             if (n.tune.intra_lanes == 64)
                 hipLaunchKernelGGL(k_intra<64>, dim3(div_up(w, 4)), dim3(256), 0, s, b);
             else

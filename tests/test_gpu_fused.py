@@ -1,6 +1,7 @@
-"""GPU: the fused persistent kernel, graph replay, and block lists in
-arbitrary order with holes (intra neighbours owned by later blocks or by no
-block read the initial 128), all bit-exact against the oracle."""
+"""GPU: the fused persistent kernel, the level-synchronous intra stage,
+graph replay, and block lists in arbitrary order with holes (intra
+neighbours owned by later blocks or by no block read the initial 128), all
+bit-exact against the oracle."""
 import numpy as np
 import pytest
 
@@ -17,9 +18,18 @@ def shuffled_with_holes(pkg, frame, rng, drop=0.1):
                           filter_level=h.filter_level, sharpness=h.sharpness_level)
 
 
-def run_engine(pkg, frames, fused, w, h):
+def set_mode(eng, mode):
+    """plain: per-stage kernels with persistent intra workers; fused: one
+    persistent kernel; levels: per-stage kernels, one intra launch per
+    dependency level (the first 64; persistent workers after)."""
+    eng.set_fused(mode == "fused")
+    if mode == "levels":
+        eng.set_tune([1, 1, 2, 4], 64, 512, 1)
+
+
+def run_engine(pkg, frames, mode, w, h):
     with pkg.Engine(0) as eng:
-        eng.set_fused(fused)
+        set_mode(eng, mode)
         sid = eng.open_stream()
         out = []
         for f in frames:
@@ -34,32 +44,33 @@ def test_fused_bit_exact(pkg, orc, gpu_ready, preset, w, h):
     frames = [pkg.SynthFrame(pkg.synth_params(preset, w, h, 60 + preset), i) for i in range(4)]
     want = orc.run_oracle(frames)
     print(f"[case] fused preset{preset} {w}x{h}", flush=True)
-    got = run_engine(pkg, frames, True, w, h)
+    got = run_engine(pkg, frames, "fused", w, h)
     for i, (g, x) in enumerate(zip(got, want)):
         assert np.array_equal(g, x), f"frame {i}"
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_shuffled_blocks_with_holes(pkg, orc, gpu_ready, fused):
+@pytest.mark.parametrize("mode", ["plain", "fused", "levels"])
+def test_shuffled_blocks_with_holes(pkg, orc, gpu_ready, mode):
     rng = np.random.default_rng(11)
     w, h = 200, 136
     base = [pkg.SynthFrame(pkg.synth_params(1, w, h, 71), i) for i in range(4)]
     frames = [shuffled_with_holes(pkg, f, rng) for f in base]
     want = orc.run_oracle(frames)
-    print(f"[case] shuffled holes fused={fused}", flush=True)
-    got = run_engine(pkg, frames, fused, w, h)
+    print(f"[case] shuffled holes mode={mode}", flush=True)
+    got = run_engine(pkg, frames, mode, w, h)
     for i, (g, x) in enumerate(zip(got, want)):
         assert np.array_equal(g, x), f"frame {i}"
 
 
-def test_fused_multi_stream_batch(pkg, orc, gpu_ready):
+@pytest.mark.parametrize("mode", ["fused", "levels"])
+def test_multi_stream_batch(pkg, orc, gpu_ready, mode):
     sizes = [(136, 72), (320, 200), (64, 64)]
     seqs = [[pkg.SynthFrame(pkg.synth_params(1, w, h, 80 + s), i) for i in range(3)]
             for s, (w, h) in enumerate(sizes)]
     want = [orc.run_oracle(q) for q in seqs]
-    print("[case] fused multi-stream", flush=True)
+    print(f"[case] multi-stream mode={mode}", flush=True)
     with pkg.Engine(0) as eng:
-        eng.set_fused(True)
+        set_mode(eng, mode)
         sids = [eng.open_stream() for _ in sizes]
         preps = [[eng.prepare(f) for f in q] for q in seqs]
         for i in range(3):
@@ -69,16 +80,16 @@ def test_fused_multi_stream_batch(pkg, orc, gpu_ready):
                 assert np.array_equal(eng.read(sids[s], w, h), want[s][i]), f"stream {s} frame {i}"
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_graph_replay_bit_exact(pkg, orc, gpu_ready, fused):
+@pytest.mark.parametrize("mode", ["plain", "fused", "levels"])
+def test_graph_replay_bit_exact(pkg, orc, gpu_ready, mode):
     sizes = [(264, 136), (136, 72)]
     n = 4
     seqs = [[pkg.SynthFrame(pkg.synth_params(2, w, h, 90 + s), i) for i in range(n)]
             for s, (w, h) in enumerate(sizes)]
     want = [orc.run_oracle(q)[-1] for q in seqs]
-    print(f"[case] graph replay fused={fused}", flush=True)
+    print(f"[case] graph replay mode={mode}", flush=True)
     with pkg.Engine(0) as eng:
-        eng.set_fused(fused)
+        set_mode(eng, mode)
         sids = [eng.open_stream() for _ in sizes]
         preps = [[eng.prepare(f) for f in q] for q in seqs]
         for i in range(n):  # sizes the scratch and the frame pool
