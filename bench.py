@@ -43,14 +43,26 @@ SINGLE_WORKLOAD = ("1920x1080 single stream, 100-frame preset-1 sequence, "
                    "plain per-stage kernels")
 
 
-def measured_traffic(workload, stage):
-    """HBM-side bytes per launch of a stage's kernel from the committed
-    rocprofv3 counter summary (profiles/traffic.json), or None."""
+def kernel_of(stage, tune):
+    """Kernel that runs a stage under a launch shape, named as rocprofv3
+    names it (templates on threads per intra worker, transform size and
+    waves per transform group)."""
+    if stage == "intra" and tune:
+        return f"k_intra<{tune['intra_lanes']}>"
+    if stage.startswith("residual") and tune:
+        n = int(stage[8:])
+        return f"k_residual<{n}, {tune['res_split'][(4, 8, 16, 32).index(n)]}>"
+    return KERNEL_OF.get(stage, stage)
+
+
+def measured_traffic(workload, kernel):
+    """HBM-side bytes per launch of a kernel from the committed rocprofv3
+    counter summary (profiles/traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             entry = json.load(f)[workload]
-        return entry["kernels"][KERNEL_OF[stage]]["traffic_bytes"], entry["profile"]
+        return entry["kernels"][kernel]["traffic_bytes"], entry["profile"]
     except (OSError, KeyError, ValueError):
         return None, None
 
@@ -116,6 +128,8 @@ def single_stream(P, args):
         tune = eng.autotune([tsid], [preps[1]], iters=5).as_dict()
         eng.close_stream(tsid)
         log("single-stream tune:", tune)
+    else:
+        tune = eng.get_tune().as_dict()
     sid = eng.open_stream()
     order = [(args.warmup + k) % F for k in range(args.steps)]
     for k in range(args.warmup):
@@ -157,8 +171,10 @@ def single_stream(P, args):
             per_stage[name]["fp64_dense_TOPs"] = round(ops / (ms * 1e-3) / 1e12, 2)
     algo = sum(frames[i].algo_bytes() for i in order)
     eng.close()
+    lv = [P.intra_levels(f) for f in frames]
     return {"clean_s": clean_s, "timed_s": timed_s, "stats": per_stage, "algo_bytes": algo,
-            "frames": frames, "tune": tune}
+            "frames": frames, "tune": tune, "depth": [max(len(x) - 1, 0) for x in lv],
+            "width0": [int(x[1] - x[0]) if len(x) > 1 else 0 for x in lv]}
 
 
 def multi_stream(P, args):
@@ -424,13 +440,26 @@ def main():
     stats = ss["stats"]
     dom = max(stats, key=lambda k: stats[k]["ms_total"])
     d = stats[dom]
-    traffic, tsrc = measured_traffic(SINGLE_WORKLOAD, dom)
-    roof = {"kernel": KERNEL_OF.get(dom, dom), "bound": "hbm", "achieved": d["algo_GBps"],
+    kern = kernel_of(dom, ss["tune"])
+    traffic, tsrc = measured_traffic(SINGLE_WORKLOAD, kern)
+    roof = {"kernel": kern, "bound": "hbm", "achieved": d["algo_GBps"],
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(d["algo_GBps"] / HBM_PEAK_GBS, 4) if d["algo_GBps"] else None,
             "traffic": traffic, "traffic_source": tsrc,
             "algo_bytes_per_launch": d.get("algo_bytes_per_launch"),
             "us_per_launch": d["us_per_launch"]}
+    if dom == "intra":
+        # the ordered stage is bounded by its dependency chain, not by HBM:
+        # levels per frame and the time per level
+        order = [(args.warmup + k) % args.frames for k in range(args.steps)]
+        depth = [ss["depth"][i] for i in order]
+        lm = (ss["tune"] or {}).get("intra_level_min", 0)
+        heads = sum(1 for i in order if lm and ss["width0"][i] >= lm)
+        roof["dependency_chain"] = {
+            "levels_per_frame": round(float(np.mean(depth)), 2),
+            "max_levels": int(max(depth)),
+            "us_per_level": round(d["us_per_launch"] / float(np.mean(depth)), 3),
+            "frames_with_level_launches": heads}
     if "fp64_dense_TOPs" in d:
         roof["fp64_dense_TOPs"] = d["fp64_dense_TOPs"]
         roof["fp64_peak_no_fma_TOPs"] = FP64_PEAK_TOPS / 2

@@ -218,6 +218,7 @@ SIGNATURES = {
     "cvp9_hip_idct_table": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "cvp9_hip_plan_check": (C.c_int, [PF_P]),
     "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
+    "cvp9_hip_intra_levels": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "hip_backend_set_debug": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cvp9_hip_copy_probe": (C.c_double, [C.c_uint64, C.c_int]),
 }
@@ -322,6 +323,18 @@ class OwnedFrame:
     @property
     def pf(self):
         return self._f
+
+
+def intra_levels(frame):
+    """Order positions where each intra dependency level of a parsed frame
+    starts, plus the intra block count (numpy uint32, depth + 1 entries)."""
+    n = lib().cvp9_hip_intra_levels(frame.ptr, None, 0)
+    check(min(n, 0), "intra_levels")
+    buf = np.zeros(n + 1, np.uint32)
+    if n:
+        check(min(lib().cvp9_hip_intra_levels(frame.ptr, buf.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                               buf.size), 0), "intra_levels")
+    return buf if n else np.zeros(0, np.uint32)
 
 
 def frame_info_to_i420(info):

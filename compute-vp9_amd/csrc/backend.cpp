@@ -429,6 +429,22 @@ int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t
     return (int)L.n_intra;
 }
 
+int cvp9_hip_intra_levels(const vp9_parsed_frame_t *pf, uint32_t *starts, uint32_t cap)
+{
+    PrepLayout L;
+    cvp9_err_t err = plan_measure(pf, L);
+    if (err != CVP9_OK) return err;
+    std::vector<uint8_t> img(L.bytes);
+    plan_write(pf, L, img.data());
+    if (L.levels.empty()) return 0;
+    const uint32_t depth = (uint32_t)L.levels.size() - 1;
+    if (starts) {
+        if (cap < depth + 1) return CVP9_ERR_INVALID_DATA;
+        memcpy(starts, L.levels.data(), (size_t)(depth + 1) * 4);
+    }
+    return (int)depth;
+}
+
 double cvp9_hip_copy_probe(uint64_t bytes, int iters)
 {
     if (bytes < 16 || (bytes & 15) || iters < 1) return -1.0;

@@ -150,6 +150,8 @@ struct DBatch {
     const uint32_t *lvl_pre;
     const uint32_t *lvl_start;
     uint32_t        spin_limit;    /* bound on every dependency wait */
+    uint32_t        fused_lanes;   /* fused kernel: threads per intra worker (64 or 256) */
+    uint32_t        fused_workers; /* fused kernel: persistent intra workers */
 };
 
 #endif

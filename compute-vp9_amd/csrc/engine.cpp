@@ -568,6 +568,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     b_.bar = (uint32_t *)(dev + tk_off);
     memset(host + tk_off, 0, 256); /* the barrier counter starts at zero */
     b_.spin_limit = spin_limit;
+    b_.fused_lanes = tune.intra_lanes;
+    b_.fused_workers = tune.intra_workers;
     n_.fill = pre[P_FILL * stride + njobs];
     n_.blocks = pre[P_BLK * stride + njobs];
     n_.inter = pre[P_INTER * stride + njobs];

@@ -828,15 +828,15 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
 }
 
 /* Persistent workers, W threads each: 256 (one workgroup per block) or 64
- * (one wave per block, four independent workers per workgroup). */
+ * (one wave per block, four independent workers per workgroup); this is
+ * worker wk of nwk. */
 template <int W>
-__device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm)
+__device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
+                                             uint32_t nwk)
 {
     IntraShared &S = shm[threadIdx.x / W];
     const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
-    const uint32_t nwk = gridDim.x * (256 / W);
-    const uint32_t wk = blockIdx.x * (256 / W) + threadIdx.x / W;
 
     /* Job, ticket and the count of jobs left to visit are made wave-uniform
      * with readfirstlane, so every loop test is a scalar branch. With an
@@ -881,7 +881,7 @@ template <int W>
 __global__ __launch_bounds__(256) void k_intra(DBatch B)
 {
     __shared__ IntraShared shm[256 / W];
-    intra_worker<W>(B, shm);
+    intra_worker<W>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W, gridDim.x * (256 / W));
 }
 
 /* Level-synchronous alternative: one launch per dependency level, one
@@ -1072,12 +1072,22 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     residual_phase<16, FS16>(B, lds);
     residual_phase<32, FS32>(B, lds);
     grid_barrier(B, ++phase * nwg);
-    /* neighbour records, then intra blocks in ticket order, one-wave workers */
+    /* neighbour records, then intra blocks in ticket order on the tuned
+     * number and width of persistent workers (the other workgroups go on
+     * to the barrier: more workers than the chains can feed only poll) */
     const uint32_t nint = B.pre_intra[B.njobs];
     if (nint) {
         for (uint32_t wv = blockIdx.x * 4 + wave; wv < nint; wv += nwg * 4) nbr_item(B, wv, lane);
         grid_barrier(B, ++phase * nwg);
-        intra_worker<64>(B, (IntraShared *)smem);
+        const uint32_t nwk = B.fused_workers < 1 ? 1 : B.fused_workers;
+        if (B.fused_lanes == 64) {
+            const uint32_t wgs = (nwk + 3) / 4 < nwg ? (nwk + 3) / 4 : nwg;
+            if (blockIdx.x < wgs)
+                intra_worker<64>(B, (IntraShared *)smem, blockIdx.x * 4 + wave, wgs * 4);
+        } else {
+            const uint32_t wgs = nwk < nwg ? nwk : nwg;
+            if (blockIdx.x < wgs) intra_worker<256>(B, (IntraShared *)smem, blockIdx.x, wgs);
+        }
         grid_barrier(B, ++phase * nwg);
     }
     /* deblocking windows */

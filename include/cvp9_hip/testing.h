@@ -37,6 +37,12 @@ cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf);
  * cvp9_err_t. Needs no device. */
 int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap);
 
+/* Dependency levels of pf's intra blocks (the order above groups them by
+ * level): returns the number of levels, and with starts != NULL writes the
+ * order position where each level starts followed by the number of intra
+ * blocks (depth + 1 entries, at most cap), or a negative cvp9_err_t. */
+int cvp9_hip_intra_levels(const vp9_parsed_frame_t *pf, uint32_t *starts, uint32_t cap);
+
 /* Copies `bytes` (multiple of 16) device to device `iters` times with a
  * plain streaming kernel on the current device; returns the achieved GB/s
  * (read + write bytes / time), or a negative value on error. Used to

@@ -285,3 +285,40 @@ def test_intra_order_shuffled_lists(pkg):
         h = f.pf.hdr
         g = pkg.OwnedFrame(h.width, h.height, b, f.coeffs(), frame_type=h.frame_type)
         assert check_order(pkg, g, b) > 0
+
+
+def brute_levels(deps):
+    """Level of each intra block from its dependency sets: 0 without an
+    intra dependency, else one more than the highest level among them
+    (dependencies are earlier blocks, so list order resolves them)."""
+    lev = {}
+    for blk in sorted(deps):
+        d = deps[blk]
+        lev[blk] = 0 if not d else 1 + max(lev[o] for o in d)
+    return lev
+
+
+@pytest.mark.parametrize("preset,w,h,shuffle", [(1, 320, 184, False), (2, 643, 362, False),
+                                                (1, 264, 136, True)])
+def test_intra_levels_match_dependencies(pkg, preset, w, h, shuffle):
+    rng = np.random.default_rng(5)
+    p = pkg.synth_params(preset, w, h, 33)
+    for idx in range(3):
+        f = pkg.SynthFrame(p, idx)
+        b = f.blocks()
+        if shuffle:
+            keep = rng.random(b.size) >= 0.1
+            b = b[keep][rng.permutation(int(keep.sum()))]
+            f = pkg.OwnedFrame(w, h, b, f.coeffs(), frame_type=f.pf.hdr.frame_type)
+        lev = brute_levels(intra_dependencies(b, w, h))
+        starts = pkg.intra_levels(f)
+        order = np.zeros(max(len(lev), 1), np.uint32)
+        n = pkg.lib().cvp9_hip_intra_order(f.ptr, order.ctypes.data_as(pkg.C.POINTER(pkg.C.c_uint32)),
+                                           order.size)
+        assert n == len(lev)
+        depth = (max(lev.values()) + 1) if lev else 0
+        assert len(starts) == (depth + 1 if depth else 0)
+        for L in range(depth):
+            got = order[starts[L]:starts[L + 1]].tolist()
+            want = sorted(blk for blk, v in lev.items() if v == L)
+            assert got == want, f"frame {idx} level {L}"
