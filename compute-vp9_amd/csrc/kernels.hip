@@ -47,6 +47,16 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *pre, uint32_t n, ui
     return lo;
 }
 
+/* Workgroup b of n runs on XCD b % 8 (the dispatcher deals workgroups out
+ * round-robin over the 8 XCDs, each with its own L2). Returns the logical
+ * index that gives each XCD one contiguous run of the work list, so work
+ * items next to each other in the list (and in the frame) share an L2. */
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
+{
+    const uint32_t per = n >> 3, rem = n & 7, x = b & 7, i = b >> 3;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
 __device__ __forceinline__ uint64_t wave_ballot(bool p)
 {
     return __ballot(p);
@@ -226,7 +236,7 @@ __device__ __forceinline__ void mc_item(const DBatch &B, uint32_t wv, int lane)
 
 __global__ __launch_bounds__(256) void k_mc(DBatch B)
 {
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t wv = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (wv < B.pre_inter[B.njobs]) mc_item(B, wv, threadIdx.x & 63);
 }
 
@@ -510,7 +520,7 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
 {
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t wv = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (wv < B.pre_intra[B.njobs]) nbr_item(B, wv, threadIdx.x & 63);
 }
 
