@@ -12,11 +12,12 @@
  *
  *   Prepared frame ("prep"): the parsed frame staged once into HBM:
  *     DBlock[nblk]   24-byte block records (clamped, normalised modes)
- *     inter[]        local indices of inter blocks          (motion comp.)
+ *     mc_tab[]       motion-compensation units: block << 8 | unit in block
  *     intra[]        local indices of intra blocks, by dependency level
  *                    then list order
- *     cls_blk[c][]   inter non-skip blocks with transform size class c
- *     cls_pre[c][]   exclusive prefix of their transform counts
+ *     tx_tab[c][]    transforms of size class c (inter and intra blocks, list
+ *                    order), 8 bytes each: coefficient offset; x | y << 14,
+ *                    bit 31 set for an intra block
  *     coeffs[]       the parser's int16 coefficients, verbatim
  *
  *   Job: one frame of one stream in a batch: output frame, the 8 reference
@@ -62,19 +63,23 @@
 #define CVP9_NB_BLOCK 35   /* 3 words */
 #define CVP9_NB_WORDS 38
 
-/* Motion compensation splits an inter block into strips of rows; a strip
- * is one wave, one quad of 4 luma samples per lane. */
+/* Motion compensation splits an inter block into units of
+ * cvp9_mc_unit_rows(bw) luma rows. A unit is 16 lanes, one quad of 4 luma
+ * samples per lane (a row of more than 16 quads takes several per lane),
+ * four units per wave: an 8x8 block is one full unit (8 rows x 2 quads),
+ * 16x16 four, 64x64 sixty-four. */
+#define CVP9_MC_UNIT 16
 #if defined(__HIPCC__)
 #define CVP9_HD __host__ __device__
 #else
 #define CVP9_HD
 #endif
-CVP9_HD static inline uint32_t cvp9_mc_rows(uint32_t bw)
+CVP9_HD static inline uint32_t cvp9_mc_unit_rows(uint32_t bw)
 {
     const uint32_t nq = (bw + 3) / 4;
     uint32_t p2 = 1;
     while (p2 < nq) p2 <<= 1;
-    return p2 >= 64 ? 1 : 64 / p2;
+    return p2 >= CVP9_MC_UNIT ? 1 : CVP9_MC_UNIT / p2;
 }
 
 /* block flags */
@@ -110,11 +115,9 @@ struct DJob {
     DFrame ref[8];                 /* y == nullptr: empty slot */
     const DBlock   *blocks;
     const int16_t  *coeffs;
-    const uint32_t *inter;
-    const uint32_t *mc_pre;        /* exclusive prefix of MC strips per inter block */
+    const uint32_t *mc_tab;        /* per MC unit: block index << 8 | unit within the block */
     const uint32_t *intra;
-    const uint32_t *cls_blk[4];
-    const uint32_t *cls_pre[4];
+    const uint32_t *tx_tab[4];
     uint32_t       *owner;         /* ow x oh cells, local block index */
     uint64_t       *edge;          /* per local block: CVP9_EDGE_WORDS edge words */
     uint64_t       *nbr;           /* per intra-list entry: CVP9_NB_WORDS words */
@@ -122,7 +125,7 @@ struct DJob {
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
     uint32_t       *err;           /* error word (spin timeout) */
     uint32_t n_blocks, n_inter, n_intra, n_mc;
-    uint32_t cls_nblk[4], cls_ntx[4];
+    uint32_t cls_ntx[4];
     uint32_t ow, oh;
     uint32_t epoch;
     int32_t  lf_thr;               /* 0: no deblocking */

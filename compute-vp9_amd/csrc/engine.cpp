@@ -151,7 +151,7 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
         if (intra) L.n_intra++;
         else {
             L.n_inter++;
-            const uint32_t rows = cvp9_mc_rows((uint32_t)c.bw);
+            const uint32_t rows = cvp9_mc_unit_rows((uint32_t)c.bw);
             L.n_mc += ((uint32_t)c.bh + rows - 1) / rows;
         }
     }
@@ -160,12 +160,10 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
 
     uint64_t off = 0;
     L.off_blocks = off; off = align256(off + (uint64_t)L.n_blocks * sizeof(DBlock));
-    L.off_inter = off;  off = align256(off + (uint64_t)L.n_inter * 4);
-    L.off_mc_pre = off; off = align256(off + (uint64_t)(L.n_inter + 1) * 4);
+    L.off_mc_tab = off; off = align256(off + (uint64_t)L.n_mc * 4);
     L.off_intra = off;  off = align256(off + (uint64_t)L.n_intra * 4);
     for (int k = 0; k < 4; k++) {
-        L.off_cls_blk[k] = off; off = align256(off + (uint64_t)L.cls_nblk[k] * 4);
-        L.off_cls_pre[k] = off; off = align256(off + (uint64_t)(L.cls_nblk[k] + 1) * 4);
+        L.off_tx_tab[k] = off; off = align256(off + (uint64_t)L.cls_ntx[k] * 8);
     }
     L.off_coeffs = off; off = align256(off + (uint64_t)L.num_coeffs * 2);
     L.bytes = off;
@@ -235,16 +233,12 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
 {
     const int W = (int)L.W, H = (int)L.H;
     DBlock *blocks = (DBlock *)(dst + L.off_blocks);
-    uint32_t *inter = (uint32_t *)(dst + L.off_inter);
-    uint32_t *mc_pre = (uint32_t *)(dst + L.off_mc_pre);
-    uint32_t mc_acc = 0;
+    uint32_t *mc_tab = (uint32_t *)(dst + L.off_mc_tab);
+    uint32_t nm = 0;
     uint32_t *intra = (uint32_t *)(dst + L.off_intra);
-    uint32_t *cblk[4], *cpre[4], cn[4] = {0, 0, 0, 0}, ctx[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 4; k++) {
-        cblk[k] = (uint32_t *)(dst + L.off_cls_blk[k]);
-        cpre[k] = (uint32_t *)(dst + L.off_cls_pre[k]);
-    }
-    uint32_t n = 0, ni = 0, nn = 0;
+    uint32_t *txt[4], cn[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; k++) txt[k] = (uint32_t *)(dst + L.off_tx_tab[k]);
+    uint32_t n = 0, ni = 0;
     for (uint32_t i = 0; i < pf->num_blocks; i++) {
         const vp9_macroblock_info_t &b = pf->blocks[i];
         const Clamped c = clamp_block(b, W, H);
@@ -281,22 +275,22 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         blocks[n] = d;
         if (is_intra) intra[ni++] = n;
         else {
-            mc_pre[nn] = mc_acc;
-            const uint32_t rows = cvp9_mc_rows(d.bw);
-            mc_acc += (d.bh + rows - 1) / rows;
-            inter[nn++] = n;
+            const uint32_t rows = cvp9_mc_unit_rows(d.bw);
+            const uint32_t units = (d.bh + rows - 1) / rows;
+            for (uint32_t u = 0; u < units; u++) mc_tab[nm++] = n << 8 | u;
         }
-        if (res) { /* transforms of inter and intra blocks alike */
+        if (res) { /* transforms of inter and intra blocks alike, list order */
             const int k = d.tx;
-            cblk[k][cn[k]] = n;
-            cpre[k][cn[k]] = ctx[k];
-            cn[k]++;
-            ctx[k] += ntx;
+            const uint32_t T = 4u << k;
+            for (uint32_t l = 0; l < ntx; l++) {
+                const uint32_t ty = l / d.nx, tx = l - ty * d.nx;
+                uint32_t *e = txt[k] + 2 * (size_t)cn[k]++;
+                e[0] = d.coeff + l * T * T;
+                e[1] = (d.x + tx * T) | (d.y + ty * T) << 14 | (is_intra ? 1u << 31 : 0u);
+            }
         }
         n++;
     }
-    for (int k = 0; k < 4; k++) cpre[k][cn[k]] = ctx[k];
-    mc_pre[nn] = mc_acc;
     order_intra(blocks, intra, ni, L.W, L.H, L.levels);
     if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
 }
@@ -479,13 +473,10 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     }
     j.blocks = (const DBlock *)(dev + L.off_blocks);
     j.coeffs = (const int16_t *)(dev + L.off_coeffs);
-    j.inter = (const uint32_t *)(dev + L.off_inter);
-    j.mc_pre = (const uint32_t *)(dev + L.off_mc_pre);
+    j.mc_tab = (const uint32_t *)(dev + L.off_mc_tab);
     j.intra = (const uint32_t *)(dev + L.off_intra);
     for (int k = 0; k < 4; k++) {
-        j.cls_blk[k] = (const uint32_t *)(dev + L.off_cls_blk[k]);
-        j.cls_pre[k] = (const uint32_t *)(dev + L.off_cls_pre[k]);
-        j.cls_nblk[k] = L.cls_nblk[k];
+        j.tx_tab[k] = (const uint32_t *)(dev + L.off_tx_tab[k]);
         j.cls_ntx[k] = L.cls_ntx[k];
     }
     j.owner = owner_;

@@ -27,9 +27,9 @@ struct PrepLayout {
     uint32_t n_blocks = 0, n_inter = 0, n_intra = 0;
     uint32_t cls_nblk[4] = {0, 0, 0, 0}, cls_ntx[4] = {0, 0, 0, 0};
     uint32_t num_coeffs = 0;
-    uint32_t n_mc = 0;              /* motion-compensation strips (one wave each) */
-    uint64_t off_blocks = 0, off_inter = 0, off_intra = 0, off_mc_pre = 0;
-    uint64_t off_cls_blk[4] = {0, 0, 0, 0}, off_cls_pre[4] = {0, 0, 0, 0};
+    uint32_t n_mc = 0;              /* motion-compensation units (16 lanes each) */
+    uint64_t off_blocks = 0, off_intra = 0, off_mc_tab = 0;
+    uint64_t off_tx_tab[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
     /* written by plan_write: intra-list position where each dependency
      * level starts, then the list's end (empty without intra blocks) */

@@ -101,6 +101,7 @@ __global__ __launch_bounds__(256) void k_fill(DBatch B)
 /* ------------------------------------------------------------------------ */
 /* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
 
+/* block wv of the batch on 16 lanes (lane = 0..15) */
 __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_t lane)
 {
     const uint32_t j = find_job(B.pre_blocks, B.njobs, wv);
@@ -110,7 +111,7 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
     const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
     const uint32_t ncx = ((k.x + k.bw - 1) >> 2) - cx0 + 1;
     const uint32_t ncy = ((k.y + k.bh - 1) >> 2) - cy0 + 1;
-    for (uint32_t i = lane; i < ncx * ncy; i += 64) {
+    for (uint32_t i = lane; i < ncx * ncy; i += 16) {
         const uint32_t r = i / ncx, c = i - r * ncx;
         J.owner[(cy0 + r) * J.ow + cx0 + c] = b;
     }
@@ -118,14 +119,14 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
 
 __global__ __launch_bounds__(256) void k_owner(DBatch B)
 {
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv < B.pre_blocks[B.njobs]) owner_item(B, wv, threadIdx.x & 63);
+    const uint32_t wv = blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (wv < B.pre_blocks[B.njobs]) owner_item(B, wv, threadIdx.x & 15);
 }
 
 /* ------------------------------------------------------------------------ */
 /* Motion compensation (vp9_reconstruct.c:94-131, :248-264, :312-334).      */
-/* One wave per strip of an inter block: cvp9_mc_rows(bw) luma rows, one    */
-/* quad of 4 samples per lane, plus the chroma rows under those luma rows.  */
+/* 16 lanes per unit of an inter block: cvp9_mc_unit_rows(bw) luma rows,   */
+/* one quad of 4 samples per lane, plus the chroma rows under those rows.   */
 /* Quads whose 5-sample source window lies inside the reference row read it */
 /* with two aligned dword loads; the others take each tap with the edge     */
 /* clamp.                                                                   */
@@ -178,47 +179,45 @@ __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
     }
 }
 
-__device__ __forceinline__ void mc_item(const DBatch &B, uint32_t wv, int lane)
+/* One motion-compensation unit: entry u of the batch's unit list, on 16
+ * lanes (li = 0..15). */
+__device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li)
 {
-    const uint32_t j = find_job(B.pre_inter, B.njobs, wv);
+    const uint32_t j = find_job(B.pre_inter, B.njobs, u);
     const DJob &J = B.jobs[j];
-    const uint32_t l = wv - B.pre_inter[j];
-    uint32_t lo = 0, hi = J.n_inter;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (J.mc_pre[mid] <= l) lo = mid; else hi = mid;
-    }
-    const DBlock k = J.blocks[J.inter[lo]];
+    const uint32_t e = J.mc_tab[u - B.pre_inter[j]];
+    const DBlock k = J.blocks[e >> 8];
     const int bx = k.x, by = k.y, bw = k.bw, bh = k.bh;
-    const int rows = (int)cvp9_mc_rows((uint32_t)bw);
-    const int r0 = (int)(l - J.mc_pre[lo]) * rows;
+    const int rows = (int)cvp9_mc_unit_rows((uint32_t)bw);
+    const int r0 = (int)(e & 255) * rows;
     const int r1 = min(r0 + rows, bh);
     const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1;
     const int cr0 = (r0 + 1) >> 1, cr1 = (r1 + 1) >> 1; /* chroma rows under [r0, r1) */
-    const int lanes_per_row = 64 / rows;                 /* power of two >= quads per row */
+    const int lanes_per_row = CVP9_MC_UNIT / rows;       /* power of two */
     const int nq = (bw + 3) >> 2, ncq = (cbw + 3) >> 2;
     const DFrame *R = k.ref < 8 ? &J.ref[k.ref] : nullptr;
     const bool missing = !R || !R->y; /* empty slot: flat 127 (:258-263, :328-333) */
 
     /* luma */
     {
-        const int rr = lane / lanes_per_row, q = lane - rr * lanes_per_row;
+        const int rr = li / lanes_per_row, qi = li - rr * lanes_per_row;
         const int r = r0 + rr;
-        if (q < nq && r < r1) {
-            uint32_t v = 0x7f7f7f7fu;
-            if (!missing) {
-                const int px8 = bx * 8 + k.mvx, py8 = by * 8 + k.mvy;
-                v = mc_quad(R->y, R->sy, (int)R->w, (int)R->h, px8 >> 3, py8 >> 3, px8 & 7,
-                            py8 & 7, 4 * q, r);
+        if (r < r1) {
+            const int px8 = bx * 8 + k.mvx, py8 = by * 8 + k.mvy;
+            for (int q = qi; q < nq; q += lanes_per_row) {
+                uint32_t v = 0x7f7f7f7fu;
+                if (!missing)
+                    v = mc_quad(R->y, R->sy, (int)R->w, (int)R->h, px8 >> 3, py8 >> 3, px8 & 7,
+                                py8 & 7, 4 * q, r);
+                store_quad(J.out.y + (size_t)(by + r) * J.out.sy + bx + 4 * q, v, bw - 4 * q);
             }
-            store_quad(J.out.y + (size_t)(by + r) * J.out.sy + bx + 4 * q, v, bw - 4 * q);
         }
     }
     /* chroma: MV halved toward zero, bounds from half the reference luma size */
     {
         const int crows = cr1 - cr0;
         const int cmx = k.mvx / 2, cmy = k.mvy / 2;
-        for (int i = lane; i < 2 * crows * ncq; i += 64) {
+        for (int i = li; i < 2 * crows * ncq; i += CVP9_MC_UNIT) {
             const int p = i / (crows * ncq);
             const int t = i - p * crows * ncq;
             const int cr = cr0 + t / ncq, cq = t - (t / ncq) * ncq;
@@ -236,8 +235,9 @@ __device__ __forceinline__ void mc_item(const DBatch &B, uint32_t wv, int lane)
 
 __global__ __launch_bounds__(256) void k_mc(DBatch B)
 {
-    const uint32_t wv = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (wv < B.pre_inter[B.njobs]) mc_item(B, wv, threadIdx.x & 63);
+    const uint32_t u = xcd_block(blockIdx.x, gridDim.x) * (256 / CVP9_MC_UNIT) +
+                       threadIdx.x / CVP9_MC_UNIT;
+    if (u < B.pre_inter[B.njobs]) mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -348,20 +348,12 @@ __device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
     dst = nullptr;
     rout = nullptr;
     if (txi >= J.cls_ntx[cls]) return;
-    const uint32_t *pre = J.cls_pre[cls];
-    uint32_t lo = 0, hi = J.cls_nblk[cls];
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= txi) lo = mid; else hi = mid;
-    }
-    const DBlock k = J.blocks[J.cls_blk[cls][lo]];
-    const uint32_t l = txi - pre[lo];
-    const uint32_t ty = l / k.nx, tx = l - ty * k.nx;
-    coef = J.coeffs + k.coeff + (size_t)l * (N * N);
-    if (k.flags & DB_INTRA)
-        rout = J.resid + k.coeff + (size_t)l * (N * N);
+    const uint2 t = ((const uint2 *)J.tx_tab[cls])[txi];
+    coef = J.coeffs + t.x;
+    if (t.y >> 31)
+        rout = J.resid + t.x;
     else
-        dst = J.out.y + (size_t)(k.y + ty * N) * J.out.sy + k.x + tx * N;
+        dst = J.out.y + (size_t)((t.y >> 14) & 0x3fff) * J.out.sy + (t.y & 0x3fff);
 }
 
 template <int N, int S>
@@ -1072,8 +1064,10 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     }
     /* owner map and motion compensation (independent) */
     const uint32_t nblk = B.pre_blocks[B.njobs], nmc = B.pre_inter[B.njobs];
-    for (uint32_t wv = blockIdx.x * 4 + wave; wv < nblk; wv += nwg * 4) owner_item(B, wv, lane);
-    for (uint32_t wv = blockIdx.x * 4 + wave; wv < nmc; wv += nwg * 4) mc_item(B, wv, lane);
+    for (uint32_t wv = blockIdx.x * 16 + (threadIdx.x >> 4); wv < nblk; wv += nwg * 16)
+        owner_item(B, wv, threadIdx.x & 15);
+    for (uint32_t u = blockIdx.x * 16 + threadIdx.x / CVP9_MC_UNIT; u < nmc; u += nwg * 16)
+        mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT);
     grid_barrier(B, ++phase * nwg);
     /* transforms: inter ones add to the prediction, intra ones go aside */
     double *lds = (double *)smem;
@@ -1195,10 +1189,10 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b);
         break;
     case STAGE_OWNER:
-        hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 4) : 1), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 16) : 1), dim3(256), 0, s, b);
         break;
     case STAGE_MC:
-        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(div_up(n.inter, 4)), dim3(256), 0, s, b);
+        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(div_up(n.inter, 256 / CVP9_MC_UNIT)), dim3(256), 0, s, b);
         break;
     case STAGE_RES4:
         return launch_residual<4>(b, n, 0, s);
