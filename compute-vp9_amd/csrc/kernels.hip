@@ -432,7 +432,10 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
  * sample's neighbour inside b, and blocks are disjoint rectangles. The
  * chroma samples next to b lie in the same 4x4 luma cells as the luma ones
  * (x, y are multiples of 4). */
-__device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t lane)
+/* Intra block wv of the batch on 32 lanes (li = 0..31), one cell per lane
+ * and pass (at most 33 cells: a second pass only for 64-wide or -high
+ * blocks). */
+__device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t li)
 {
     const uint32_t j = find_job(B.pre_intra, B.njobs, wv);
     const DJob &J = B.jobs[j];
@@ -442,89 +445,110 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
     const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
-    int kind, q = 0;
-    bool present;
-    if ((int)lane < ncx) { kind = 0; q = lane; present = y > 0; }
-    else if ((int)lane < ncx + ncy) { kind = 1; q = lane - ncx; present = x > 0; }
-    else { kind = 2; present = (int)lane == ncx + ncy && x > 0 && y > 0; }
-    uint64_t val = 0;
-    bool dep = false;
-    if (present) {
-        const int cx = kind == 0 ? (x >> 2) + q : (x >> 2) - 1;
-        const int cy = kind == 1 ? (y >> 2) + q : (y >> 2) - 1;
-        const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
-        if (o == CVP9_OWNER_NONE || o >= b) {
-            val = 0x8080808080808080ull; /* still the initial 128 */
-        } else {
-            const DBlock n = J.blocks[o];
-            if (!(n.flags & DB_INTRA)) {
-                const DFrame &F = J.out;
-                if (kind == 0) {
-                    /* rows are 64-byte aligned and x + 4q < W: the dword and
-                     * the chroma pair stay inside their rows */
-                    val = *(const uint32_t *)(F.y + (size_t)(y - 1) * F.sy + x + 4 * q);
-                    const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
-                    val |= (uint64_t)*(const uint16_t *)(F.u + co) << 32;
-                    val |= (uint64_t)*(const uint16_t *)(F.v + co) << 48;
-                } else if (kind == 1) {
-                    for (int e = 0; e < 4 && 4 * q + e < bh; e++)
-                        val |= (uint64_t)F.y[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
-                    for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
-                        const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
-                        val |= (uint64_t)F.u[co] << (32 + 8 * e);
-                        val |= (uint64_t)F.v[co] << (48 + 8 * e);
+    uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+    uint64_t mask = 0;
+    for (int c = (int)li; c <= ncx + ncy; c += 32) {
+        bool dep;
+        int kind, q = 0;
+        bool present;
+        if (c < ncx) { kind = 0; q = c; present = y > 0; }
+        else if (c < ncx + ncy) { kind = 1; q = c - ncx; present = x > 0; }
+        else { kind = 2; present = x > 0 && y > 0; }
+        uint64_t val = 0;
+        dep = false;
+        if (present) {
+            const int cx = kind == 0 ? (x >> 2) + q : (x >> 2) - 1;
+            const int cy = kind == 1 ? (y >> 2) + q : (y >> 2) - 1;
+            const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
+            if (o == CVP9_OWNER_NONE || o >= b) {
+                val = 0x8080808080808080ull; /* still the initial 128 */
+            } else {
+                const DBlock n = J.blocks[o];
+                if (!(n.flags & DB_INTRA)) {
+                    const DFrame &F = J.out;
+                    if (kind == 0) {
+                        /* rows are 64-byte aligned and x + 4q < W: the dword and
+                         * the chroma pair stay inside their rows */
+                        val = *(const uint32_t *)(F.y + (size_t)(y - 1) * F.sy + x + 4 * q);
+                        const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
+                        val |= (uint64_t)*(const uint16_t *)(F.u + co) << 32;
+                        val |= (uint64_t)*(const uint16_t *)(F.v + co) << 48;
+                    } else if (kind == 1) {
+                        for (int e = 0; e < 4 && 4 * q + e < bh; e++)
+                            val |= (uint64_t)F.y[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
+                        for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
+                            const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
+                            val |= (uint64_t)F.u[co] << (32 + 8 * e);
+                            val |= (uint64_t)F.v[co] << (48 + 8 * e);
+                        }
+                    } else {
+                        const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
+                        val = (uint64_t)F.y[(size_t)(y - 1) * F.sy + x - 1] << 24 |
+                              (uint64_t)F.u[co] << 40 | (uint64_t)F.v[co] << 56;
                     }
                 } else {
-                    const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
-                    val = (uint64_t)F.y[(size_t)(y - 1) * F.sy + x - 1] << 24 |
-                          (uint64_t)F.u[co] << 40 | (uint64_t)F.v[co] << 56;
+                    const int ox = n.x, oy = n.y;
+                    const bool bottom_row = y - 1 == oy + n.bh - 1, right_col = x - 1 == ox + n.bw - 1;
+                    int side, lw, ci; /* edge side, luma word, first chroma index */
+                    if (kind == 0 && bottom_row) {
+                        side = 0; lw = (x + 4 * q - ox) >> 2; ci = cbx + 2 * q - (ox >> 1);
+                    } else if (kind == 1 && right_col) {
+                        side = 1; lw = (y + 4 * q - oy) >> 2; ci = cby + 2 * q - (oy >> 1);
+                    } else if (kind == 2 && bottom_row) {
+                        side = 0; lw = (x - 4 - ox) >> 2; ci = cbx - 2 - (ox >> 1);
+                    } else if (kind == 2 && right_col) {
+                        side = 1; lw = (y - 4 - oy) >> 2; ci = cby - 2 - (oy >> 1);
+                    } else {
+                        atomicOr(J.err, 2u); /* not on an edge: impossible for disjoint blocks */
+                        side = 0; lw = 0; ci = 0;
+                    }
+                    val = (uint64_t)o | (uint64_t)side << 32 | (uint64_t)(lw & 15) << 33 |
+                          (uint64_t)(ci & 31) << 40;
+                    dep = true;
                 }
-            } else {
-                const int ox = n.x, oy = n.y;
-                const bool bottom_row = y - 1 == oy + n.bh - 1, right_col = x - 1 == ox + n.bw - 1;
-                int side, lw, c;
-                if (kind == 0 && bottom_row) {
-                    side = 0; lw = (x + 4 * q - ox) >> 2; c = cbx + 2 * q - (ox >> 1);
-                } else if (kind == 1 && right_col) {
-                    side = 1; lw = (y + 4 * q - oy) >> 2; c = cby + 2 * q - (oy >> 1);
-                } else if (kind == 2 && bottom_row) {
-                    side = 0; lw = (x - 4 - ox) >> 2; c = cbx - 2 - (ox >> 1);
-                } else if (kind == 2 && right_col) {
-                    side = 1; lw = (y - 4 - oy) >> 2; c = cby - 2 - (oy >> 1);
-                } else {
-                    atomicOr(J.err, 2u); /* not on an edge: impossible for disjoint blocks */
-                    side = 0; lw = 0; c = 0;
-                }
-                val = (uint64_t)o | (uint64_t)side << 32 | (uint64_t)(lw & 15) << 33 |
-                      (uint64_t)(c & 31) << 40;
-                dep = true;
             }
         }
+        rec[c] = val;
+        if (dep) mask |= 1ull << c;
     }
-    const uint64_t mask = wave_ballot(dep);
-    uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
-    if ((int)lane <= ncx + ncy) rec[lane] = val;
-    if (lane == CVP9_NB_MASK) rec[CVP9_NB_MASK] = mask;
-    if (lane == CVP9_NB_INDEX) rec[CVP9_NB_INDEX] = b;
-    if (lane >= CVP9_NB_BLOCK && lane < CVP9_NB_WORDS)
-        rec[lane] = ((const uint64_t *)(J.blocks + b))[lane - CVP9_NB_BLOCK];
+    /* OR of the 32 lanes' dependency bits */
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) mask |= __shfl_xor(mask, off, 32);
+    if (li == 0) rec[CVP9_NB_MASK] = mask;
+    if (li == 1) rec[CVP9_NB_INDEX] = b;
+    if (li >= 2 && li < 2 + CVP9_NB_WORDS - CVP9_NB_BLOCK)
+        rec[CVP9_NB_BLOCK + li - 2] = ((const uint64_t *)(J.blocks + b))[li - 2];
 }
 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
 {
-    const uint32_t wv = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (wv < B.pre_intra[B.njobs]) nbr_item(B, wv, threadIdx.x & 63);
+    const uint32_t wv = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
+    if (wv < B.pre_intra[B.njobs]) nbr_item(B, wv, threadIdx.x & 31);
 }
 
-/* Edge word of an earlier intra block, waited for (bounded) until it
- * carries this frame's epoch. The poll backs off (64, 512, then 2048
- * clocks between reads), so thousands of waiting lanes do not crowd the
- * memory system that the blocks they wait for are using. */
-__device__ __forceinline__ uint64_t edge_word(const DJob &J, const uint64_t *w, uint32_t spin_limit)
+/* The samples of an intra cell in the resolved layout (bytes 0-3 luma,
+ * 4-5 U, 6-7 V), read from the owner's edge record: its three words (Y, U,
+ * V) are loaded together and re-read, bounded, until each carries this
+ * frame's epoch. The poll backs off (64, 512, then 2048 clocks between
+ * reads), so thousands of waiting lanes do not crowd the memory system
+ * that the blocks they wait for are using. */
+__device__ __forceinline__ uint64_t cell_from_edges(const DJob &J, uint64_t d, uint32_t spin_limit)
 {
-    uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t o = (uint32_t)d;
+    const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
+    const uint32_t c = (uint32_t)(d >> 40) & 31;
+    const uint64_t *rec = J.edge + (size_t)o * CVP9_EDGE_WORDS;
+    const uint64_t *py = rec + (side ? 16 : 0) + lw;
+    const uint64_t *pu = rec + (side ? 40 : 32) + (c >> 2);
+    const uint64_t *pv = rec + (side ? 56 : 48) + (c >> 2);
+    uint64_t wy = __hip_atomic_load(py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t wu = __hip_atomic_load(pu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t wv = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
-    while ((uint32_t)(v >> 32) != J.epoch) {
+    for (;;) {
+        const bool sy = (uint32_t)(wy >> 32) != J.epoch, su = (uint32_t)(wu >> 32) != J.epoch,
+                   sv = (uint32_t)(wv >> 32) != J.epoch;
+        if (!(sy || su || sv)) break;
         if (spins < 4) __builtin_amdgcn_s_sleep(1);
         else if (spins < 16) __builtin_amdgcn_s_sleep(8);
         else __builtin_amdgcn_s_sleep(32);
@@ -535,22 +559,10 @@ __device__ __forceinline__ uint64_t edge_word(const DJob &J, const uint64_t *w, 
         if ((spins & 255) == 0 &&
             __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             break;
-        v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sy) wy = __hip_atomic_load(py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (su) wu = __hip_atomic_load(pu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sv) wv = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    return v;
-}
-
-/* The samples of an intra cell in the resolved layout (bytes 0-3 luma,
- * 4-5 U, 6-7 V), read from the owner's edge record. */
-__device__ __forceinline__ uint64_t cell_from_edges(const DJob &J, uint64_t d, uint32_t spin_limit)
-{
-    const uint32_t o = (uint32_t)d;
-    const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
-    const uint32_t c = (uint32_t)(d >> 40) & 31;
-    const uint64_t *rec = J.edge + (size_t)o * CVP9_EDGE_WORDS;
-    const uint64_t wy = edge_word(J, rec + (side ? 16 : 0) + lw, spin_limit);
-    const uint64_t wu = edge_word(J, rec + (side ? 40 : 32) + (c >> 2), spin_limit);
-    const uint64_t wv = edge_word(J, rec + (side ? 56 : 48) + (c >> 2), spin_limit);
     const uint32_t sh = 8 * (c & 3);
     return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
 }
@@ -1054,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
 {
     __shared__ __align__(16) uint8_t smem[FUSED_LDS];
     const uint32_t nwg = gridDim.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     uint32_t phase = 0;
 
     const uint32_t nfill = B.pre_fill[B.njobs];
@@ -1081,7 +1093,8 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
      * to the barrier: more workers than the chains can feed only poll) */
     const uint32_t nint = B.pre_intra[B.njobs];
     if (nint) {
-        for (uint32_t wv = blockIdx.x * 4 + wave; wv < nint; wv += nwg * 4) nbr_item(B, wv, lane);
+        for (uint32_t wv = blockIdx.x * 8 + (threadIdx.x >> 5); wv < nint; wv += nwg * 8)
+            nbr_item(B, wv, threadIdx.x & 31);
         grid_barrier(B, ++phase * nwg);
         const uint32_t nwk = B.fused_workers < 1 ? 1 : B.fused_workers;
         if (B.fused_lanes == 64) {
@@ -1203,7 +1216,7 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
     case STAGE_RES32:
         return launch_residual<32>(b, n, 3, s);
     case STAGE_NBR:
-        if (n.intra) hipLaunchKernelGGL(k_nbr, dim3(div_up(n.intra, 4)), dim3(256), 0, s, b);
+        if (n.intra) hipLaunchKernelGGL(k_nbr, dim3(div_up(n.intra, 8)), dim3(256), 0, s, b);
         break;
     case STAGE_INTRA:
         for (uint32_t l = 0; l < n.levels; l++) {
