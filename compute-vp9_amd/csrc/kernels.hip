@@ -532,12 +532,13 @@ __global__ __launch_bounds__(256) void k_nbr(DBatch B)
  * frame's epoch. The poll backs off (64, 512, then 2048 clocks between
  * reads), so thousands of waiting lanes do not crowd the memory system
  * that the blocks they wait for are using. */
-__device__ __forceinline__ uint64_t cell_from_edges(const DJob &J, uint64_t d, uint32_t spin_limit)
+__device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32_t epoch, uint32_t *err,
+                                                    uint64_t d, uint32_t spin_limit)
 {
     const uint32_t o = (uint32_t)d;
     const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
     const uint32_t c = (uint32_t)(d >> 40) & 31;
-    const uint64_t *rec = J.edge + (size_t)o * CVP9_EDGE_WORDS;
+    const uint64_t *rec = edge + (size_t)o * CVP9_EDGE_WORDS;
     const uint64_t *py = rec + (side ? 16 : 0) + lw;
     const uint64_t *pu = rec + (side ? 40 : 32) + (c >> 2);
     const uint64_t *pv = rec + (side ? 56 : 48) + (c >> 2);
@@ -546,18 +547,18 @@ __device__ __forceinline__ uint64_t cell_from_edges(const DJob &J, uint64_t d, u
     uint64_t wv = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
     for (;;) {
-        const bool sy = (uint32_t)(wy >> 32) != J.epoch, su = (uint32_t)(wu >> 32) != J.epoch,
-                   sv = (uint32_t)(wv >> 32) != J.epoch;
+        const bool sy = (uint32_t)(wy >> 32) != epoch, su = (uint32_t)(wu >> 32) != epoch,
+                   sv = (uint32_t)(wv >> 32) != epoch;
         if (!(sy || su || sv)) break;
         if (spins < 4) __builtin_amdgcn_s_sleep(1);
         else if (spins < 16) __builtin_amdgcn_s_sleep(8);
         else __builtin_amdgcn_s_sleep(32);
         if (++spins > spin_limit) {
-            atomicOr(J.err, 1u);
+            atomicOr(err, 1u);
             break;
         }
         if ((spins & 255) == 0 &&
-            __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             break;
         if (sy) wy = __hip_atomic_load(py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (su) wu = __hip_atomic_load(pu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -611,6 +612,14 @@ template <int W, bool EARLY>
 __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint32_t tk,
                                             IntraShared &S, int tid)
 {
+    /* the job's fields, loaded once, together: after any store the
+     * compiler would otherwise reload them (a store could alias the
+     * descriptor as far as it knows), one memory round trip each */
+    const DFrame out = J.out;
+    int16_t *const jres = J.resid;
+    uint64_t *const jedge = J.edge;
+    uint32_t *const jerr = J.err;
+    const uint32_t epoch = J.epoch;
     /* block index, block record, dependency mask and this thread's cell
      * all come from the neighbour record, in one round of loads */
     const uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
@@ -637,7 +646,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     auto resid_quad = [&](int i) -> const int16_t * {
         const int r = i / nq, c0 = (i - r * nq) * 4;
         if (r >= rrows || c0 >= rcols) return nullptr;
-        return J.resid + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
+        return jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
                ((r & (T - 1)) << lt) + (c0 & (T - 1));
     };
     int16_t rq0[4] = {0, 0, 0, 0};
@@ -649,7 +658,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     /* residual at one luma sample (0 where the block has none) */
     auto resid_at = [&](int r, int c) -> int {
         if (r >= rrows || c >= rcols) return 0;
-        return J.resid[k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c >> lt)) << (2 * lt)) +
+        return jres[k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c >> lt)) << (2 * lt)) +
                        ((r & (T - 1)) << lt) + (c & (T - 1))];
     };
     /* and this thread's first luma edge sample: bottom row, then right
@@ -670,7 +679,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         uint64_t v = 0x7f7f7f7f7f7f7f7full;
         if (present) {
             v = cell;
-            if ((dmask >> c) & 1) v = cell_from_edges(J, v, B.spin_limit);
+            if ((dmask >> c) & 1) v = cell_from_edges(jedge, epoch, jerr, v, B.spin_limit);
         }
         if (kind == 2) {
             S.tl[0] = (uint8_t)(v >> 24);
@@ -740,11 +749,11 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
             else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
             if (4 * wi < len) {
-                uint64_t d = (uint64_t)J.epoch << 32;
+                uint64_t d = (uint64_t)epoch << 32;
     #pragma unroll
                 for (int e = 0; e < 4; e++)
                     if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
-                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                __hip_atomic_store(jedge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -758,7 +767,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
      *    here and published after. */
     {
         const int tl = S.tl[0], mode = k.ymode;
-        uint8_t *Y = J.out.y + (size_t)y * J.out.sy + x;
+        uint8_t *Y = out.y + (size_t)y * out.sy + x;
         const int nitems = nq * bh;
         for (int i0 = tid; i0 < nitems; i0 += 4 * W) {
             int16_t rq[4][4];
@@ -792,7 +801,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                         }
                     }
                 }
-                store_quad(Y + (size_t)r * J.out.sy + c0, word, bw - c0);
+                store_quad(Y + (size_t)r * out.sy + c0, word, bw - c0);
             }
         }
         const int nqc = (cbw + 3) >> 2;
@@ -816,8 +825,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                     }
                 }
             }
-            uint8_t *pl = p == 1 ? J.out.u : J.out.v;
-            store_quad(pl + (size_t)(cby + r) * J.out.suv + cbx + c0, word, cbw - c0);
+            uint8_t *pl = p == 1 ? out.u : out.v;
+            store_quad(pl + (size_t)(cby + r) * out.suv + cbx + c0, word, cbw - c0);
         }
     }
     if constexpr (!EARLY) {
@@ -830,11 +839,11 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
             else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
             if (4 * wi < len) {
-                uint64_t d = (uint64_t)J.epoch << 32;
+                uint64_t d = (uint64_t)epoch << 32;
     #pragma unroll
                 for (int e = 0; e < 4; e++)
                     if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
-                __hip_atomic_store(J.edge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                __hip_atomic_store(jedge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -908,7 +917,8 @@ __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level)
     __shared__ IntraShared shm[256 / W];
     const uint32_t *pre = B.lvl_pre + (size_t)level * (B.njobs + 1);
     const uint32_t *start = B.lvl_start + (size_t)level * (B.njobs + 1);
-    const uint32_t g = blockIdx.x * (256 / W) + threadIdx.x / W;
+    /* wave-uniform (W >= 64), so the job lookup uses scalar branches */
+    const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * (256 / W) + threadIdx.x / W);
     if (g >= pre[B.njobs]) return; /* uniform per worker */
     const uint32_t j = find_job(pre, B.njobs, g);
     intra_block<W, false>(B, B.jobs[j], start[j] + (g - pre[j]), shm[threadIdx.x / W],
