@@ -111,9 +111,11 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
     const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
     const uint32_t ncx = ((k.x + k.bw - 1) >> 2) - cx0 + 1;
     const uint32_t ncy = ((k.y + k.bh - 1) >> 2) - cy0 + 1;
+    uint32_t *const owner = J.owner; /* held in registers across the stores */
+    const uint32_t ow = J.ow;
     for (uint32_t i = lane; i < ncx * ncy; i += 16) {
         const uint32_t r = i / ncx, c = i - r * ncx;
-        J.owner[(cy0 + r) * J.ow + cx0 + c] = b;
+        owner[(cy0 + r) * ow + cx0 + c] = b;
     }
 }
 
@@ -195,8 +197,12 @@ __device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li)
     const int cr0 = (r0 + 1) >> 1, cr1 = (r1 + 1) >> 1; /* chroma rows under [r0, r1) */
     const int lanes_per_row = CVP9_MC_UNIT / rows;       /* power of two */
     const int nq = (bw + 3) >> 2, ncq = (cbw + 3) >> 2;
-    const DFrame *R = k.ref < 8 ? &J.ref[k.ref] : nullptr;
-    const bool missing = !R || !R->y; /* empty slot: flat 127 (:258-263, :328-333) */
+    /* frames held in registers: a store could alias the descriptor as far
+     * as the compiler knows, and would make it reload them */
+    const DFrame out = J.out;
+    DFrame R = {};
+    if (k.ref < 8) R = J.ref[k.ref];
+    const bool missing = !R.y; /* empty slot: flat 127 (:258-263, :328-333) */
 
     /* luma */
     {
@@ -207,9 +213,9 @@ __device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li)
             for (int q = qi; q < nq; q += lanes_per_row) {
                 uint32_t v = 0x7f7f7f7fu;
                 if (!missing)
-                    v = mc_quad(R->y, R->sy, (int)R->w, (int)R->h, px8 >> 3, py8 >> 3, px8 & 7,
+                    v = mc_quad(R.y, R.sy, (int)R.w, (int)R.h, px8 >> 3, py8 >> 3, px8 & 7,
                                 py8 & 7, 4 * q, r);
-                store_quad(J.out.y + (size_t)(by + r) * J.out.sy + bx + 4 * q, v, bw - 4 * q);
+                store_quad(out.y + (size_t)(by + r) * out.sy + bx + 4 * q, v, bw - 4 * q);
             }
         }
     }
@@ -224,11 +230,11 @@ __device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li)
             uint32_t v = 0x7f7f7f7fu;
             if (!missing) {
                 const int px8 = cbx * 8 + cmx, py8 = cby * 8 + cmy;
-                v = mc_quad(p ? R->v : R->u, R->suv, (int)R->w / 2, (int)R->h / 2, px8 >> 3,
+                v = mc_quad(p ? R.v : R.u, R.suv, (int)R.w / 2, (int)R.h / 2, px8 >> 3,
                             py8 >> 3, px8 & 7, py8 & 7, 4 * cq, cr);
             }
-            uint8_t *pl = p ? J.out.v : J.out.u;
-            store_quad(pl + (size_t)(cby + cr) * J.out.suv + cbx + 4 * cq, v, cbw - 4 * cq);
+            uint8_t *pl = p ? out.v : out.u;
+            store_quad(pl + (size_t)(cby + cr) * out.suv + cbx + 4 * cq, v, cbw - 4 * cq);
         }
     }
 }
@@ -446,6 +452,11 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
     uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+    /* fields held in registers across the record stores */
+    const DFrame F = J.out;
+    const uint32_t *const owner = J.owner;
+    const DBlock *const blocks = J.blocks;
+    const uint32_t ow = J.ow;
     uint64_t mask = 0;
     for (int c = (int)li; c <= ncx + ncy; c += 32) {
         bool dep;
@@ -459,13 +470,12 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
         if (present) {
             const int cx = kind == 0 ? (x >> 2) + q : (x >> 2) - 1;
             const int cy = kind == 1 ? (y >> 2) + q : (y >> 2) - 1;
-            const uint32_t o = J.owner[(uint32_t)cy * J.ow + (uint32_t)cx];
+            const uint32_t o = owner[(uint32_t)cy * ow + (uint32_t)cx];
             if (o == CVP9_OWNER_NONE || o >= b) {
                 val = 0x8080808080808080ull; /* still the initial 128 */
             } else {
-                const DBlock n = J.blocks[o];
+                const DBlock n = blocks[o];
                 if (!(n.flags & DB_INTRA)) {
-                    const DFrame &F = J.out;
                     if (kind == 0) {
                         /* rows are 64-byte aligned and x + 4q < W: the dword and
                          * the chroma pair stay inside their rows */
@@ -517,7 +527,7 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     if (li == 0) rec[CVP9_NB_MASK] = mask;
     if (li == 1) rec[CVP9_NB_INDEX] = b;
     if (li >= 2 && li < 2 + CVP9_NB_WORDS - CVP9_NB_BLOCK)
-        rec[CVP9_NB_BLOCK + li - 2] = ((const uint64_t *)(J.blocks + b))[li - 2];
+        rec[CVP9_NB_BLOCK + li - 2] = ((const uint64_t *)(blocks + b))[li - 2];
 }
 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
@@ -1006,16 +1016,17 @@ __device__ __forceinline__ void nv12_item(const DBatch &B, uint32_t w)
     const uint32_t tx = (W + 63) / 64;
     const uint32_t ty0 = (l / tx) * 64, tx0 = (l % tx) * 64;
     uint8_t *dst = J.nv12;
+    const DFrame F = J.out; /* held in registers across the stores */
     for (uint32_t i = threadIdx.x; i < 64 * 64; i += 256) {
         const uint32_t r = ty0 + (i >> 6), c = tx0 + (i & 63);
-        if (r < H && c < W) dst[(size_t)r * W + c] = J.out.y[(size_t)r * J.out.sy + c];
+        if (r < H && c < W) dst[(size_t)r * W + c] = F.y[(size_t)r * F.sy + c];
     }
     uint8_t *uv = dst + (size_t)W * H;
     for (uint32_t i = threadIdx.x; i < 32 * 32; i += 256) {
         const uint32_t r = ty0 / 2 + (i >> 5), c = tx0 / 2 + (i & 31);
         if (r < ch && c < cw) {
-            uv[(size_t)r * 2 * cw + 2 * c] = J.out.u[(size_t)r * J.out.suv + c];
-            uv[(size_t)r * 2 * cw + 2 * c + 1] = J.out.v[(size_t)r * J.out.suv + c];
+            uv[(size_t)r * 2 * cw + 2 * c] = F.u[(size_t)r * F.suv + c];
+            uv[(size_t)r * 2 * cw + 2 * c + 1] = F.v[(size_t)r * F.suv + c];
         }
     }
 }
