@@ -219,6 +219,10 @@ SIGNATURES = {
     "cvp9_hip_plan_check": (C.c_int, [PF_P]),
     "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_intra_levels": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
+    "cvp9_hip_export_read": (C.c_int, [C.c_void_p, C.POINTER(ExportBuffer), C.POINTER(C.c_uint8),
+                                       C.c_uint64]),
+    "cvp9_hip_dmabuf_read": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint8),
+                                       C.c_uint64]),
     "hip_backend_set_debug": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cvp9_hip_copy_probe": (C.c_double, [C.c_uint64, C.c_int]),
 }
@@ -382,6 +386,30 @@ class HipBackend:
 
     def set_debug(self, flags):
         check(lib().hip_backend_set_debug(self.ctx, flags), "hip_backend_set_debug")
+
+    def export_buffer_alloc(self, size):
+        """(code, ExportBuffer): hip_export_buffer_alloc."""
+        buf = ExportBuffer()
+        return lib().hip_export_buffer_alloc(self.ctx, size, C.byref(buf)), buf
+
+    def export_buffer_free(self, buf):
+        lib().hip_export_buffer_free(self.ctx, C.byref(buf))
+
+    def set_render_target(self, buf):
+        """hip_set_render_target (None clears a pending target)."""
+        return lib().hip_set_render_target(self.ctx, C.byref(buf) if buf is not None else None)
+
+    def get_frame_dmabuf(self):
+        """(code, DmabufFrame): the caller closes the returned fd."""
+        f = DmabufFrame()
+        return lib().hip_get_frame_dmabuf(self.ctx, C.byref(f)), f
+
+    def export_read(self, buf, n):
+        """First n bytes of an export buffer, read back through HIP."""
+        out = np.zeros(n, np.uint8)
+        check(lib().cvp9_hip_export_read(self.ctx, C.byref(buf),
+                                         out.ctypes.data_as(C.POINTER(C.c_uint8)), n), "export_read")
+        return out
 
     def close(self):
         if self.ctx:

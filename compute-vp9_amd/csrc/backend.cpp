@@ -38,7 +38,8 @@ struct Slot {
     uint8_t *stage_dev = nullptr;   size_t stage_dev_cap = 0;
     uint8_t *out_host = nullptr;    size_t out_host_cap = 0;
     uint8_t *out_dev = nullptr;     size_t out_dev_cap = 0;   /* packed I420, VRAM */
-    int out_fd = -1;
+    int out_fd = -1;                /* its DMA-BUF, or -1 */
+    uint64_t out_off = 0;           /* out_dev's offset inside that DMA-BUF */
     uint32_t *err_host = nullptr;
 };
 
@@ -50,15 +51,21 @@ struct Backend {
     uint32_t head = 0, count = 0;
     uint8_t *pending_target = nullptr;   /* device address of the armed NV12 target */
     uint64_t pending_target_size = 0;
+    struct ExportPriv *pending_priv = nullptr;
     uint32_t debug = 0;
     Tune tune = cvp9_default_tune(256);
     bool dmabuf = false;
 };
 
+/* An export buffer: dedicated VRAM with its DMA-BUF, and the CPU view the
+ * caller gets as `mapped`: the DMA-BUF mapped into this process where the
+ * driver allows it, else pinned host memory that every decode into the
+ * buffer refreshes on the stream before the frame is reported done. */
 struct ExportPriv {
     uint8_t *dev;
     size_t size;
     void *map;
+    bool shadow;   /* map is a hipHostMalloc copy, not a mapping of dev */
 };
 
 inline size_t i420_bytes(uint32_t w, uint32_t h)
@@ -66,17 +73,24 @@ inline size_t i420_bytes(uint32_t w, uint32_t h)
     return (size_t)w * h + 2 * (size_t)((w + 1) / 2) * ((h + 1) / 2);
 }
 
-/* DMA-BUF fd of a VRAM allocation, or -1 */
-int export_fd(void *ptr, size_t size)
+/* DMA-BUF fd of a VRAM allocation and ptr's offset inside it, or -1 */
+int export_fd(void *ptr, size_t size, uint64_t *off)
 {
     int fd = -1;
-    uint64_t off = 0;
-    if (hsa_amd_portable_export_dmabuf(ptr, size, &fd, &off) != HSA_STATUS_SUCCESS) return -1;
-    if (off != 0) {
-        hsa_amd_portable_close_dmabuf(fd);
-        return -1;
-    }
+    *off = 0;
+    if (hsa_amd_portable_export_dmabuf(ptr, size, &fd, off) != HSA_STATUS_SUCCESS) return -1;
     return fd;
+}
+
+/* Device memory for a buffer handed out as a DMA-BUF: whole 2 MiB units, so
+ * the allocator gives it an allocation of its own (smaller requests can
+ * share one, and then export at an offset). */
+hipError_t alloc_exportable(uint8_t **p, size_t *cap, size_t need)
+{
+    const size_t c = (need + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+    hipError_t e = hipMalloc((void **)p, c);
+    *cap = e == hipSuccess ? c : 0;
+    return e;
 }
 
 hipError_t grow_pinned(uint8_t **p, size_t *cap, size_t need)
@@ -181,7 +195,8 @@ cvp9_err_t hip_backend_init(void **ctx)
     {
         void *probe = nullptr;
         if (hipMalloc(&probe, 4096) == hipSuccess) {
-            int fd = export_fd(probe, 4096);
+            uint64_t off = 0;
+            int fd = export_fd(probe, 4096, &off);
             if (fd >= 0) {
                 b->dmabuf = true;
                 hsa_amd_portable_close_dmabuf(fd);
@@ -228,8 +243,10 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     if (s.out_dev_cap < ob) {
         if (s.out_fd >= 0) hsa_amd_portable_close_dmabuf(s.out_fd);
         s.out_fd = -1;
-        if ((e = grow_device(&s.out_dev, &s.out_dev_cap, ob)) != hipSuccess) return to_err(e);
-        if (b->dmabuf) s.out_fd = export_fd(s.out_dev, s.out_dev_cap);
+        if (s.out_dev) hipFree(s.out_dev);
+        s.out_dev = nullptr;
+        if ((e = alloc_exportable(&s.out_dev, &s.out_dev_cap, ob)) != hipSuccess) return to_err(e);
+        if (b->dmabuf) s.out_fd = export_fd(s.out_dev, s.out_dev_cap, &s.out_off);
     }
 
     plan_write(pf, L, s.stage_host);
@@ -256,6 +273,9 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
         (e = hipMemcpy2DAsync(du, cw, f.u, f.suv, cw, ch, hipMemcpyDeviceToDevice, b->stream)) ||
         (e = hipMemcpy2DAsync(dv, cw, f.v, f.suv, cw, ch, hipMemcpyDeviceToDevice, b->stream)) ||
         (e = hipMemcpyAsync(s.out_host, s.out_dev, ob, hipMemcpyDeviceToHost, b->stream)) ||
+        (b->pending_priv && b->pending_priv->shadow &&
+         (e = hipMemcpyAsync(b->pending_priv->map, b->pending_priv->dev, ob, hipMemcpyDeviceToHost,
+                             b->stream))) ||
         (e = hipMemcpyAsync(s.err_host, b->st->err_word(), 4, hipMemcpyDeviceToHost, b->stream)) ||
         (e = hipEventRecord(s.done, b->stream)))
         return to_err(e);
@@ -267,6 +287,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     b->count++;
     b->pending_target = nullptr; /* one-shot (vulkan_backend.c:1059-1062) */
     b->pending_target_size = 0;
+    b->pending_priv = nullptr;
     return CVP9_OK;
 }
 
@@ -305,12 +326,13 @@ cvp9_err_t hip_get_frame_dmabuf(void *ctx, cvp9_dmabuf_frame_t *out)
     const uint32_t w = s->w, h = s->h, cw = (w + 1) / 2, ch = (h + 1) / 2;
     out->fd = dup(s->out_fd);
     if (out->fd < 0) return CVP9_ERR_GPU;
-    out->size = s->out_dev_cap;
+    const uint32_t o = (uint32_t)s->out_off;
+    out->size = s->out_off + s->out_dev_cap;
     out->width = w;
     out->height = h;
-    out->offsets[0] = 0;
-    out->offsets[1] = w * h;
-    out->offsets[2] = w * h + cw * ch;
+    out->offsets[0] = o;
+    out->offsets[1] = o + w * h;
+    out->offsets[2] = o + w * h + cw * ch;
     out->pitches[0] = w;
     out->pitches[1] = cw;
     out->pitches[2] = cw;
@@ -324,24 +346,32 @@ cvp9_err_t hip_export_buffer_alloc(void *ctx, uint64_t size, cvp9_export_buffer_
     if (!b || !out) return CVP9_ERR_INVALID_DATA;
     if (!b->dmabuf || size == 0) return CVP9_ERR_UNSUPPORTED;
     hipSetDevice(b->device);
-    const size_t sz = (size + 4095) & ~(size_t)4095;
     uint8_t *dev = nullptr;
-    if (hipMalloc((void **)&dev, sz) != hipSuccess) return CVP9_ERR_GPU;
-    int fd = export_fd(dev, sz);
-    if (fd < 0) {
+    size_t sz = 0;
+    if (alloc_exportable(&dev, &sz, size) != hipSuccess) return CVP9_ERR_GPU;
+    uint64_t off = 0;
+    int fd = export_fd(dev, sz, &off);
+    if (fd < 0 || off != 0) { /* the buffer must start the DMA-BUF: no offset field */
+        if (fd >= 0) hsa_amd_portable_close_dmabuf(fd);
         hipFree(dev);
         return CVP9_ERR_UNSUPPORTED;
     }
-    /* CPU view of the VRAM through the DMA-BUF (BAR mapping) */
+    /* CPU view: the DMA-BUF mapped (BAR), else a pinned shadow */
+    bool shadow = false;
     void *map = mmap(nullptr, sz, PROT_READ | PROT_WRITE, MAP_SHARED, fd, 0);
     if (map == MAP_FAILED) {
-        hsa_amd_portable_close_dmabuf(fd);
-        hipFree(dev);
-        return CVP9_ERR_UNSUPPORTED;
+        map = nullptr;
+        if (hipHostMalloc(&map, sz, hipHostMallocDefault) != hipSuccess) {
+            hsa_amd_portable_close_dmabuf(fd);
+            hipFree(dev);
+            return CVP9_ERR_NOMEM;
+        }
+        memset(map, 0, sz);
+        shadow = true;
     }
-    ExportPriv *p = new (std::nothrow) ExportPriv{dev, sz, map};
+    ExportPriv *p = new (std::nothrow) ExportPriv{dev, sz, map, shadow};
     if (!p) {
-        munmap(map, sz);
+        if (shadow) hipHostFree(map); else munmap(map, sz);
         hsa_amd_portable_close_dmabuf(fd);
         hipFree(dev);
         return CVP9_ERR_NOMEM;
@@ -361,7 +391,12 @@ void hip_export_buffer_free(void *ctx, cvp9_export_buffer_t *buf)
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
     ExportPriv *p = (ExportPriv *)buf->priv_buf;
-    munmap(p->map, p->size);
+    if (b->pending_priv == p) {
+        b->pending_target = nullptr;
+        b->pending_target_size = 0;
+        b->pending_priv = nullptr;
+    }
+    if (p->shadow) hipHostFree(p->map); else munmap(p->map, p->size);
     if (buf->fd >= 0) hsa_amd_portable_close_dmabuf(buf->fd);
     hipFree(p->dev);
     delete p;
@@ -376,12 +411,14 @@ cvp9_err_t hip_set_render_target(void *ctx, const cvp9_export_buffer_t *target)
     if (!target) {
         b->pending_target = nullptr;
         b->pending_target_size = 0;
+        b->pending_priv = nullptr;
         return CVP9_OK;
     }
     if (!b->dmabuf || !target->priv_buf) return CVP9_ERR_UNSUPPORTED;
     const ExportPriv *p = (const ExportPriv *)target->priv_buf;
     b->pending_target = p->dev;
     b->pending_target_size = target->size;
+    b->pending_priv = (ExportPriv *)target->priv_buf;
     return CVP9_OK;
 }
 
@@ -443,6 +480,49 @@ int cvp9_hip_intra_levels(const vp9_parsed_frame_t *pf, uint32_t *starts, uint32
         memcpy(starts, L.levels.data(), (size_t)(depth + 1) * 4);
     }
     return (int)depth;
+}
+
+cvp9_err_t cvp9_hip_export_read(void *ctx, const cvp9_export_buffer_t *buf, uint8_t *dst,
+                                uint64_t n)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b || !buf || !buf->priv_buf || !dst) return CVP9_ERR_INVALID_DATA;
+    const ExportPriv *p = (const ExportPriv *)buf->priv_buf;
+    if (n > p->size) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(b->device);
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return CVP9_ERR_GPU;
+    return hipMemcpy(dst, p->dev, n, hipMemcpyDeviceToHost) == hipSuccess ? CVP9_OK : CVP9_ERR_GPU;
+}
+
+cvp9_err_t cvp9_hip_dmabuf_read(int fd, uint64_t size, uint64_t offset, uint8_t *dst, uint64_t n)
+{
+    if (fd < 0 || !dst || offset + n > size) return CVP9_ERR_INVALID_DATA;
+    /* the import takes ownership of the descriptor it is given */
+    const int own = dup(fd);
+    if (own < 0) return CVP9_ERR_GPU;
+    hipExternalMemoryHandleDesc d;
+    memset(&d, 0, sizeof(d));
+    d.type = hipExternalMemoryHandleTypeOpaqueFd;
+    d.handle.fd = own;
+    d.size = size;
+    hipExternalMemory_t ext = nullptr;
+    if (hipImportExternalMemory(&ext, &d) != hipSuccess) {
+        close(own);
+        return CVP9_ERR_UNSUPPORTED;
+    }
+    hipExternalMemoryBufferDesc bd;
+    memset(&bd, 0, sizeof(bd));
+    bd.offset = 0;
+    bd.size = size;
+    void *ptr = nullptr;
+    cvp9_err_t err = CVP9_OK;
+    if (hipExternalMemoryGetMappedBuffer(&ptr, ext, &bd) != hipSuccess)
+        err = CVP9_ERR_UNSUPPORTED;
+    else if (hipMemcpy(dst, (uint8_t *)ptr + offset, n, hipMemcpyDeviceToHost) != hipSuccess)
+        err = CVP9_ERR_GPU;
+    if (ptr) hipFree(ptr);
+    hipDestroyExternalMemory(ext);
+    return err;
 }
 
 double cvp9_hip_copy_probe(uint64_t bytes, int iters)

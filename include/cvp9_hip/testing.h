@@ -43,6 +43,19 @@ int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t
  * blocks (depth + 1 entries, at most cap), or a negative cvp9_err_t. */
 int cvp9_hip_intra_levels(const vp9_parsed_frame_t *pf, uint32_t *starts, uint32_t cap);
 
+/* Copies the first n bytes of an export buffer (hip_export_buffer_alloc)
+ * to host memory through HIP, after the backend's queued work; for checking
+ * what a render-target decode wrote independently of the CPU mapping. */
+cvp9_err_t cvp9_hip_export_read(void *ctx, const cvp9_export_buffer_t *buf, uint8_t *dst,
+                                uint64_t n);
+
+/* Reads n bytes at `offset` of a DMA-BUF of `size` bytes by importing it
+ * as HIP external memory (as another GPU API would): for checking the
+ * frames hip_get_frame_dmabuf hands out where the buffer cannot be mapped
+ * into the CPU. The caller keeps fd. CVP9_ERR_UNSUPPORTED if the import
+ * is refused. */
+cvp9_err_t cvp9_hip_dmabuf_read(int fd, uint64_t size, uint64_t offset, uint8_t *dst, uint64_t n);
+
 /* Copies `bytes` (multiple of 16) device to device `iters` times with a
  * plain streaming kernel on the current device; returns the achieved GB/s
  * (read + write bytes / time), or a negative value on error. Used to
