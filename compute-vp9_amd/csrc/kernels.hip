@@ -968,9 +968,18 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
     const uint32_t sy = J.out.sy;
     const int tid = threadIdx.x;
 
-    for (int i = tid; i < 64 * 64; i += 256) {
-        const int r = i >> 6, c = i & 63;
-        if (r < nr && c < nc) tile[r][c] = Y[(size_t)(r0 + r) * sy + c0 + c];
+    /* rows move as aligned dwords: the window starts 2 bytes into one (rows
+     * are 64-byte aligned and every dword that meets [c0, c1) lies inside
+     * the row's pitch) */
+    const int a0 = c0 & ~3, nd = (c1 - a0 + 3) >> 2;
+    for (int i = tid; i < nr * nd; i += 256) {
+        const int r = i / nd, p = a0 + 4 * (i - r * nd);
+        const uint32_t v = *(const uint32_t *)(Y + (size_t)(r0 + r) * sy + p);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int c = p + e - c0;
+            if (c >= 0 && c < nc) tile[r][c] = (uint8_t)(v >> (8 * e));
+        }
     }
     __syncthreads();
     for (int i = tid; i < 8 * 64; i += 256) { /* horizontal edges */
@@ -991,9 +1000,28 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
         }
     }
     __syncthreads();
-    for (int i = tid; i < 64 * 64; i += 256) {
-        const int r = i >> 6, c = i & 63;
-        if (r < nr && c < nc) Y[(size_t)(r0 + r) * sy + c0 + c] = tile[r][c];
+    /* whole dwords where the window covers them; the partial dwords at the
+     * window's ends (their other bytes belong to the neighbouring window)
+     * byte by byte */
+    for (int i = tid; i < nr * nd; i += 256) {
+        const int r = i / nd, p = a0 + 4 * (i - r * nd);
+        uint32_t v = 0, mask = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int c = p + e - c0;
+            if (c >= 0 && c < nc) {
+                v |= (uint32_t)tile[r][c] << (8 * e);
+                mask |= 1u << e;
+            }
+        }
+        uint8_t *d = Y + (size_t)(r0 + r) * sy + p;
+        if (mask == 15) {
+            *(uint32_t *)d = v;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if ((mask >> e) & 1) d[e] = (uint8_t)(v >> (8 * e));
+        }
     }
 }
 
