@@ -193,6 +193,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_uint32), C.c_int]),
     "cvp9_engine_stream_reset": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_set_queues": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_capture_begin": (C.c_int, [C.c_void_p]),
     "cvp9_engine_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "cvp9_engine_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -513,6 +514,9 @@ class Engine:
 
     def set_fused(self, on):
         check(lib().cvp9_engine_set_fused(self.e, 1 if on else 0), "set_fused")
+
+    def set_queues(self, n):
+        check(lib().cvp9_engine_set_queues(self.e, n), "set_queues")
 
     def capture_begin(self):
         check(lib().cvp9_engine_capture_begin(self.e), "capture_begin")

@@ -97,7 +97,19 @@ struct cvp9_graph {
 
 struct cvp9_engine {
     int device = 0;
-    hipStream_t stream = nullptr;
+    /* Two HIP queues. A batch runs on the queue its streams last ran on, or,
+     * for streams that have not run, on the queue the previous batch did not
+     * use; when its streams last ran on both, the chosen queue first waits for
+     * the other one's tail. Batches of disjoint stream sets (the usual pattern:
+     * a caller alternating two halves of its streams) thus overlap, and every
+     * stream's frames stay in order. Capture, fused mode and per-stage timing
+     * use queue 0 only. */
+    hipStream_t stream = nullptr;  /* queue 0 */
+    hipStream_t stream2 = nullptr; /* queue 1 */
+    int nqueues = 2;
+    int last_q = 1;
+    std::vector<int8_t> sq;        /* per stream id: queue of its last batch, -1 none */
+    hipEvent_t xev[2] = {nullptr, nullptr};
     std::vector<StreamState *> streams;
     BatchSlot slots[kBatchSlots];
     int next = 0;
@@ -112,6 +124,22 @@ struct cvp9_engine {
     bool fused = false;
     uint32_t fused_wgs = 0;
     cvp9_graph *capture = nullptr; /* recording batches into a graph */
+
+    hipStream_t queue(int q) const { return q ? stream2 : stream; }
+
+    hipError_t sync_all()
+    {
+        hipError_t a = hipStreamSynchronize(stream);
+        hipError_t b = stream2 ? hipStreamSynchronize(stream2) : hipSuccess;
+        return a != hipSuccess ? a : b;
+    }
+
+    /* queue dst waits (on the device) for everything enqueued on src so far */
+    hipError_t order(int src, int dst)
+    {
+        hipError_t e = hipEventRecord(xev[src], queue(src));
+        return e != hipSuccess ? e : hipStreamWaitEvent(queue(dst), xev[src], 0);
+    }
 
     hipEvent_t take_event()
     {
@@ -163,6 +191,12 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
         delete e;
         return CVP9_ERR_GPU;
     }
+    if (hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->xev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->xev[1], hipEventDisableTiming) != hipSuccess) {
+        cvp9_engine_destroy(e);
+        return CVP9_ERR_GPU;
+    }
     e->fused_wgs = cvp9_fused_max_wgs(device);
     for (BatchSlot &s : e->slots) {
         if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
@@ -178,7 +212,7 @@ void cvp9_engine_destroy(cvp9_engine_t *e)
 {
     if (!e) return;
     hipSetDevice(e->device);
-    if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->stream) e->sync_all();
     for (StreamState *s : e->streams) delete s;
     for (BatchSlot &s : e->slots) {
         if (s.done) hipEventDestroy(s.done);
@@ -187,6 +221,9 @@ void cvp9_engine_destroy(cvp9_engine_t *e)
     }
     e->resolve_timing();
     for (hipEvent_t ev : e->free_events) hipEventDestroy(ev);
+    for (hipEvent_t ev : e->xev)
+        if (ev) hipEventDestroy(ev);
+    if (e->stream2) hipStreamDestroy(e->stream2);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
 }
@@ -199,10 +236,12 @@ cvp9_err_t cvp9_engine_stream_open(cvp9_engine_t *e, int *id)
     for (size_t i = 0; i < e->streams.size(); i++)
         if (!e->streams[i]) {
             e->streams[i] = s;
+            e->sq[i] = -1;
             *id = (int)i;
             return CVP9_OK;
         }
     e->streams.push_back(s);
+    e->sq.push_back(-1);
     *id = (int)e->streams.size() - 1;
     return CVP9_OK;
 }
@@ -211,7 +250,7 @@ void cvp9_engine_stream_close(cvp9_engine_t *e, int id)
 {
     if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return;
     hipSetDevice(e->device);
-    hipStreamSynchronize(e->stream); /* in-flight batches may still read it */
+    e->sync_all(); /* in-flight batches may still read it */
     delete e->streams[id];
     e->streams[id] = nullptr;
 }
@@ -220,12 +259,22 @@ cvp9_err_t cvp9_engine_stream_reset(cvp9_engine_t *e, int id)
 {
     if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
+    hipError_t he = e->sync_all();
+    if (he != hipSuccess) return to_err(he);
+    e->sq[id] = 0;
     return to_err(e->streams[id]->reset(e->stream));
 }
 
 void *cvp9_engine_hip_stream(cvp9_engine_t *e)
 {
-    return e ? (void *)e->stream : nullptr;
+    return e ? (void *)e->queue(e->last_q) : nullptr;
+}
+
+cvp9_err_t cvp9_engine_set_queues(cvp9_engine_t *e, int n)
+{
+    if (!e || n < 1 || n > 2) return CVP9_ERR_INVALID_DATA;
+    e->nqueues = n;
+    return CVP9_OK;
 }
 
 uint64_t cvp9_engine_stream_bytes(cvp9_engine_t *e, int id)
@@ -265,7 +314,7 @@ void cvp9_engine_prep_free(cvp9_engine_t *e, cvp9_prep_t *p)
     if (!p) return;
     if (e) {
         hipSetDevice(e->device);
-        hipStreamSynchronize(e->stream);
+        e->sync_all();
     }
     hipFree(p->dev);
     delete p;
@@ -291,13 +340,23 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             if (streams[k] == id) return CVP9_ERR_INVALID_DATA;
     }
     hipError_t he;
+    /* the queue (see the engine's description) */
+    const bool single = e->capture || e->fused || e->timing || e->nqueues < 2;
+    int q = -1;
+    for (int i = 0; i < n && q < 0; i++) q = e->sq[streams[i]];
+    if (single) q = 0;
+    else if (q < 0) q = 1 - e->last_q;
+    bool other = false;
+    for (int i = 0; i < n; i++) other |= e->sq[streams[i]] >= 0 && e->sq[streams[i]] != q;
+    if (other && (he = e->order(1 - q, q)) != hipSuccess) return to_err(he);
+    const hipStream_t qs = e->queue(q);
     if (e->capture) /* nothing may synchronise or memset while recording */
         for (int i = 0; i < n; i++)
             if (!e->streams[streams[i]]->scratch_fits(preps[i]->L)) return CVP9_ERR_UNSUPPORTED;
     for (int i = 0; i < n; i++) {
         StreamState *s = e->streams[streams[i]];
         const PrepLayout &L = preps[i]->L;
-        if (!e->capture && (he = s->ensure_scratch(L, e->stream)) != hipSuccess) return to_err(he);
+        if (!e->capture && (he = s->ensure_scratch(L, qs)) != hipSuccess) return to_err(he);
         FrameBuf *out = nullptr;
         if ((he = s->acquire(L.W, L.H, &out, !e->capture)) != hipSuccess) return to_err(he);
         jobs[i] = s->make_job(L, preps[i]->dev, out, nv12 ? nv12[i] : nullptr);
@@ -339,7 +398,7 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
     Tune tune = e->tune;
     if (e->fused) tune.intra_level_min = 0; /* the fused kernel runs persistent workers only */
     bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, kSpinLimit);
-    if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, e->stream)) != hipSuccess)
+    if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, qs)) != hipSuccess)
         return to_err(he);
     const bool timing = e->timing && !e->capture;
     if (e->fused) {
@@ -348,11 +407,11 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             t.a = e->take_event();
             t.b = e->take_event();
             if (!t.a || !t.b) return CVP9_ERR_GPU;
-            hipEventRecord(t.a, e->stream);
+            hipEventRecord(t.a, qs);
         }
-        if ((he = cvp9_launch_fused(bb.batch(), e->fused_wgs, e->stream)) != hipSuccess) return to_err(he);
+        if ((he = cvp9_launch_fused(bb.batch(), e->fused_wgs, qs)) != hipSuccess) return to_err(he);
         if (timing) {
-            hipEventRecord(t.b, e->stream);
+            hipEventRecord(t.b, qs);
             e->timed.push_back(t);
         }
     }
@@ -363,19 +422,21 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             t.a = e->take_event();
             t.b = e->take_event();
             if (!t.a || !t.b) return CVP9_ERR_GPU;
-            hipEventRecord(t.a, e->stream);
+            hipEventRecord(t.a, qs);
         }
-        if ((he = cvp9_launch_stage(st, bb.batch(), bb.counts(), e->stream)) != hipSuccess)
+        if ((he = cvp9_launch_stage(st, bb.batch(), bb.counts(), qs)) != hipSuccess)
             return to_err(he);
         if (timed) {
-            hipEventRecord(t.b, e->stream);
+            hipEventRecord(t.b, qs);
             e->timed.push_back(t);
         }
     }
     if (bs) {
-        if ((he = hipEventRecord(bs->done, e->stream)) != hipSuccess) return to_err(he);
+        if ((he = hipEventRecord(bs->done, qs)) != hipSuccess) return to_err(he);
         bs->used = true;
     }
+    for (int i = 0; i < n; i++) e->sq[streams[i]] = (int8_t)q;
+    e->last_q = q;
     return CVP9_OK;
 }
 
@@ -403,7 +464,8 @@ cvp9_err_t cvp9_engine_capture_begin(cvp9_engine_t *e)
 {
     if (!e || e->capture) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
-    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    if (e->sync_all() != hipSuccess) return CVP9_ERR_GPU;
+    for (int8_t &q : e->sq) q = q >= 0 ? 0 : -1; /* queue 1 is idle: all work is ordered on 0 */
     e->capture = new (std::nothrow) cvp9_graph();
     if (!e->capture) return CVP9_ERR_NOMEM;
     if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
@@ -433,7 +495,12 @@ cvp9_err_t cvp9_engine_graph_launch(cvp9_engine_t *e, cvp9_graph_t *g)
 {
     if (!e || !g || !g->exec) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
-    return hipGraphLaunch(g->exec, e->stream) == hipSuccess ? CVP9_OK : CVP9_ERR_GPU;
+    /* the recorded batches run on queue 0, after everything queued so far */
+    if (e->order(1, 0) != hipSuccess || hipGraphLaunch(g->exec, e->stream) != hipSuccess)
+        return CVP9_ERR_GPU;
+    for (int8_t &q : e->sq) q = q >= 0 ? 0 : -1;
+    e->last_q = 0;
+    return CVP9_OK;
 }
 
 void cvp9_engine_graph_free(cvp9_engine_t *e, cvp9_graph_t *g)
@@ -441,7 +508,7 @@ void cvp9_engine_graph_free(cvp9_engine_t *e, cvp9_graph_t *g)
     if (!g) return;
     if (e) {
         hipSetDevice(e->device);
-        hipStreamSynchronize(e->stream);
+        e->sync_all();
     }
     free_graph(g);
 }
@@ -450,7 +517,7 @@ cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e)
 {
     if (!e) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
-    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    if (e->sync_all() != hipSuccess) return CVP9_ERR_GPU;
     e->resolve_timing();
     for (StreamState *s : e->streams) {
         if (!s || !s->err_word()) continue;
@@ -473,7 +540,7 @@ cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int id, uint8_t *dst, uint64_t cap
     const uint64_t need = (uint64_t)w * h + 2ull * cw * ch;
     if (!dst || cap < need) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
-    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    if (e->sync_all() != hipSuccess) return CVP9_ERR_GPU;
     const DFrame d = f->dframe();
     if (hipMemcpy2D(dst, w, d.y, d.sy, w, h, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy2D(dst + (size_t)w * h, cw, d.u, d.suv, cw, ch, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -590,7 +657,7 @@ cvp9_err_t cvp9_engine_stage_stats(cvp9_engine_t *e, double *ms, uint32_t *launc
 {
     if (!e) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
-    if (hipStreamSynchronize(e->stream) != hipSuccess) return CVP9_ERR_GPU;
+    if (e->sync_all() != hipSuccess) return CVP9_ERR_GPU;
     e->resolve_timing();
     for (int i = 0; i < STAGE_COUNT; i++) {
         if (ms) ms[i] = e->stage_ms[i];

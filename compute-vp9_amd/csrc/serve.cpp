@@ -70,17 +70,23 @@ struct cvp9_server {
     uint32_t queued = 0, running = 0;
     size_t rr = 0;
 
+    /* batches may finish out of launch order (the engine runs batches of
+     * disjoint streams on two queues); frames of one stream still finish in
+     * order, since its batches run in order */
     cvp9_err_t retire()
     {
-        while (!inflight.empty()) {
-            hipError_t q = hipEventQuery(inflight.front().ev);
-            if (q == hipErrorNotReady) break;
+        for (size_t i = 0; i < inflight.size();) {
+            hipError_t q = hipEventQuery(inflight[i].ev);
+            if (q == hipErrorNotReady) {
+                i++;
+                continue;
+            }
             if (q != hipSuccess) return CVP9_ERR_GPU;
             const uint64_t t = now_ns();
-            for (uint64_t tag : inflight.front().tags) done.emplace_back(tag, t);
-            running -= (uint32_t)inflight.front().tags.size();
-            events.push_back(inflight.front().ev);
-            inflight.pop_front();
+            for (uint64_t tag : inflight[i].tags) done.emplace_back(tag, t);
+            running -= (uint32_t)inflight[i].tags.size();
+            events.push_back(inflight[i].ev);
+            inflight.erase(inflight.begin() + (long)i);
         }
         return CVP9_OK;
     }
@@ -115,7 +121,7 @@ cvp9_err_t cvp9_server_create(cvp9_engine_t *e, const cvp9_server_config_t *cfg,
 void cvp9_server_destroy(cvp9_server_t *s)
 {
     if (!s) return;
-    hipStreamSynchronize((hipStream_t)cvp9_engine_hip_stream(s->e));
+    cvp9_engine_sync(s->e);
     for (Batch &b : s->inflight) hipEventDestroy(b.ev);
     for (hipEvent_t ev : s->events) hipEventDestroy(ev);
     for (SStream &st : s->streams) {
@@ -256,7 +262,8 @@ cvp9_err_t cvp9_server_read_nv12(cvp9_server_t *s, int stream, uint64_t frame, u
     const SStream &st = s->streams[stream];
     const uint64_t need = nv12_bytes(st.w, st.h);
     if (cap < need || !st.nv12) return CVP9_ERR_INVALID_DATA;
-    if (hipStreamSynchronize((hipStream_t)cvp9_engine_hip_stream(s->e)) != hipSuccess) return CVP9_ERR_GPU;
+    cvp9_err_t err = cvp9_engine_sync(s->e);
+    if (err != CVP9_OK) return err;
     const uint8_t *src = st.nv12 + (frame % CVP9_SERVER_NV12_RING) * st.nv12_cap;
     return hipMemcpy(dst, src, need, hipMemcpyDeviceToHost) == hipSuccess ? CVP9_OK : CVP9_ERR_GPU;
 }
@@ -267,8 +274,10 @@ cvp9_err_t cvp9_server_drain(cvp9_server_t *s)
     while (s->queued) {
         int r = cvp9_server_pump(s);
         if (r < 0) return (cvp9_err_t)r;
-        if (r == 0 && hipStreamSynchronize((hipStream_t)cvp9_engine_hip_stream(s->e)) != hipSuccess)
-            return CVP9_ERR_GPU;
+        if (r == 0) {
+            cvp9_err_t err = cvp9_engine_sync(s->e);
+            if (err != CVP9_OK) return err;
+        }
     }
     cvp9_err_t err = cvp9_engine_sync(s->e);
     cvp9_err_t r2 = s->retire();

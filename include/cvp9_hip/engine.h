@@ -32,8 +32,17 @@ void       cvp9_engine_stream_close(cvp9_engine_t *e, int stream_id);
 /* Empties a stream's reference pool (as a new stream) and clears its error
  * state, keeping its device memory for reuse. */
 cvp9_err_t cvp9_engine_stream_reset(cvp9_engine_t *e, int stream_id);
-/* The engine's HIP stream (hipStream_t), for callers recording events. */
+/* The HIP stream (hipStream_t) the most recent cvp9_engine_run enqueued
+ * on, for callers recording an event after a batch. */
 void      *cvp9_engine_hip_stream(cvp9_engine_t *e);
+/* HIP queues the engine spreads batches over (1 or 2; default 2). A batch
+ * runs on the queue its streams last ran on; a batch of streams that have
+ * not run takes the queue the previous batch did not use; a batch whose
+ * streams last ran on both waits for the other queue first. So batches of
+ * disjoint stream sets (a caller alternating between two halves of its
+ * streams) overlap on the GPU, and each stream's frames stay in order.
+ * Graph capture, fused mode and per-stage timing use one queue. */
+cvp9_err_t cvp9_engine_set_queues(cvp9_engine_t *e, int n);
 /* HBM held by a stream (frame pool, scratch). */
 uint64_t   cvp9_engine_stream_bytes(cvp9_engine_t *e, int stream_id);
 
@@ -52,8 +61,8 @@ uint64_t   cvp9_engine_prep_bytes(const cvp9_prep_t *p);
  * optional device pointer receiving the frame as NV12 (pitch W). */
 cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams,
                            cvp9_prep_t *const *preps, uint8_t *const *nv12);
-/* Waits for all enqueued batches. CVP9_ERR_GPU if a device error was
- * flagged for any stream. */
+/* Waits for all enqueued batches (both queues). CVP9_ERR_GPU if a device
+ * error was flagged for any stream. */
 cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e);
 
 /* Copies the stream's most recent frame (packed I420: Y W*H, U, V

@@ -75,3 +75,31 @@ def test_autotune_installs_a_valid_shape(pkg, gpu_ready):
         assert all(s in (1, 2, 4) for s in t.res_split)
         assert t.intra_lanes in (64, 256) and t.intra_workers > 0
         assert eng.get_tune().as_dict() == t.as_dict()
+
+
+def test_two_queues_alternating_halves(pkg, orc, gpu_ready):
+    """Batches of two disjoint stream halves, alternated, run on the engine's
+    two queues; a later batch holding streams of both halves waits for the
+    other queue. Every stream's frames equal the stream decoded alone."""
+    sizes = [(136, 72), (200, 120), (264, 136), (320, 184)]
+    n = 4
+    seqs = [[pkg.SynthFrame(pkg.synth_params(1 + (s & 1), w, h, 300 + s), i) for i in range(n)]
+            for s, (w, h) in enumerate(sizes)]
+    want = [orc.run_oracle(q) for q in seqs]
+    print("[case] two queues, alternating halves then a mixed batch", flush=True)
+    with pkg.Engine(0) as eng:
+        eng.set_queues(2)
+        sids = [eng.open_stream() for _ in sizes]
+        preps = [[eng.prepare(f) for f in q] for q in seqs]
+        halves = [[0, 2], [1, 3]]
+        for i in range(n - 1):
+            for hf in halves:
+                eng.run([sids[s] for s in hf], [preps[s][i] for s in hf])
+            if i == 1:  # read back mid-sequence (synchronises both queues)
+                for s, (w, h) in enumerate(sizes):
+                    assert np.array_equal(eng.read(sids[s], w, h), want[s][i]), f"stream {s} frame {i}"
+        # one batch with every stream: waits for the queue it does not run on
+        eng.run(sids, [preps[s][n - 1] for s in range(len(sizes))])
+        eng.sync()
+        for s, (w, h) in enumerate(sizes):
+            assert np.array_equal(eng.read(sids[s], w, h), want[s][n - 1]), f"stream {s}"
