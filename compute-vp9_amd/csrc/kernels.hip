@@ -47,6 +47,25 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *pre, uint32_t n, ui
     return lo;
 }
 
+/* The same lookup by a whole wave for a wave-uniform w (every lane active,
+ * n <= 4096): the lanes compare 64 prefix entries at once, so one or two
+ * rounds of loads replace the binary search's dependent chain. */
+__device__ __forceinline__ uint32_t find_job_wave(const uint32_t *pre, uint32_t n, uint32_t w)
+{
+    const uint32_t lane = __lane_id();
+    if (n <= 64) {
+        const bool le = lane < n && pre[lane] <= w;
+        return (uint32_t)__builtin_popcountll(__ballot(le)) - 1;
+    }
+    const uint32_t step = (n + 63) >> 6;
+    const uint32_t i = lane * step;
+    const bool le = i < n && pre[i] <= w;
+    const uint32_t base = ((uint32_t)__builtin_popcountll(__ballot(le)) - 1) * step;
+    const uint32_t k = base + lane;
+    const bool le2 = lane < step && k < n && pre[k] <= w;
+    return base + (uint32_t)__builtin_popcountll(__ballot(le2)) - 1;
+}
+
 /* Workgroup b of n runs on XCD b % 8 (the dispatcher deals workgroups out
  * round-robin over the 8 XCDs, each with its own L2). Returns the logical
  * index that gives each XCD one contiguous run of the work list, so work
@@ -68,7 +87,7 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p)
 
 __device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w)
 {
-    const uint32_t j = find_job(B.pre_fill, B.njobs, w);
+    const uint32_t j = find_job_wave(B.pre_fill, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t chunk = w - B.pre_fill[j];
     const uint64_t fbytes = (uint64_t)J.out.sy * J.out.h +
@@ -101,10 +120,11 @@ __global__ __launch_bounds__(256) void k_fill(DBatch B)
 /* ------------------------------------------------------------------------ */
 /* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
 
-/* block wv of the batch on 16 lanes (lane = 0..15) */
-__device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_t lane)
+/* block wv of the batch on 16 lanes (lane = 0..15); jhint: the job of the
+ * wave's first block */
+__device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_t lane, uint32_t jhint)
 {
-    const uint32_t j = find_job(B.pre_blocks, B.njobs, wv);
+    const uint32_t j = wv < B.pre_blocks[jhint + 1] ? jhint : find_job(B.pre_blocks, B.njobs, wv);
     const DJob &J = B.jobs[j];
     const uint32_t b = wv - B.pre_blocks[j];
     const DBlock k = J.blocks[b];
@@ -121,8 +141,12 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
 
 __global__ __launch_bounds__(256) void k_owner(DBatch B)
 {
-    const uint32_t wv = blockIdx.x * 16 + (threadIdx.x >> 4);
-    if (wv < B.pre_blocks[B.njobs]) owner_item(B, wv, threadIdx.x & 15);
+    const uint32_t total = B.pre_blocks[B.njobs];
+    const uint32_t w0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4; /* the wave's first block */
+    if (w0 >= total) return;
+    const uint32_t jh = find_job_wave(B.pre_blocks, B.njobs, w0);
+    const uint32_t wv = w0 + ((threadIdx.x & 63) >> 4);
+    if (wv < total) owner_item(B, wv, threadIdx.x & 15, jh);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -183,9 +207,10 @@ __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
 
 /* One motion-compensation unit: entry u of the batch's unit list, on 16
  * lanes (li = 0..15). */
-__device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li)
+__device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li, uint32_t jhint)
 {
-    const uint32_t j = find_job(B.pre_inter, B.njobs, u);
+    /* the wave's job (looked up for its first unit) unless u lies past it */
+    const uint32_t j = u < B.pre_inter[jhint + 1] ? jhint : find_job(B.pre_inter, B.njobs, u);
     const DJob &J = B.jobs[j];
     const uint32_t e = J.mc_tab[u - B.pre_inter[j]];
     const DBlock k = J.blocks[e >> 8];
@@ -241,9 +266,13 @@ __device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li)
 
 __global__ __launch_bounds__(256) void k_mc(DBatch B)
 {
-    const uint32_t u = xcd_block(blockIdx.x, gridDim.x) * (256 / CVP9_MC_UNIT) +
-                       threadIdx.x / CVP9_MC_UNIT;
-    if (u < B.pre_inter[B.njobs]) mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT);
+    const uint32_t total = B.pre_inter[B.njobs];
+    const uint32_t u0 = xcd_block(blockIdx.x, gridDim.x) * (256 / CVP9_MC_UNIT) +
+                        (threadIdx.x >> 6) * (64 / CVP9_MC_UNIT); /* the wave's first unit */
+    if (u0 >= total) return;
+    const uint32_t jh = find_job_wave(B.pre_inter, B.njobs, u0);
+    const uint32_t u = u0 + (threadIdx.x & 63) / CVP9_MC_UNIT;
+    if (u < total) mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT, jh);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -387,7 +416,7 @@ __device__ __forceinline__ void residual_unit(const DBatch &B, uint32_t unit, do
     int16_t *rout = nullptr;
     uint32_t stride = 0;
     if (grp < pre[B.njobs]) {
-        const uint32_t j = find_job(pre, B.njobs, grp);
+        const uint32_t j = find_job_wave(pre, B.njobs, grp); /* grp is wave-uniform */
         const DJob &J = B.jobs[j];
         stride = J.out.sy;
         locate_tx<N>(J, C, (grp - pre[j]) * G + lane / N, coef, dst, rout);
@@ -441,9 +470,9 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
 /* Intra block wv of the batch on 32 lanes (li = 0..31), one cell per lane
  * and pass (at most 33 cells: a second pass only for 64-wide or -high
  * blocks). */
-__device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t li)
+__device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t li, uint32_t jhint)
 {
-    const uint32_t j = find_job(B.pre_intra, B.njobs, wv);
+    const uint32_t j = wv < B.pre_intra[jhint + 1] ? jhint : find_job(B.pre_intra, B.njobs, wv);
     const DJob &J = B.jobs[j];
     const uint32_t tk = wv - B.pre_intra[j];
     const uint32_t b = J.intra[tk];
@@ -532,8 +561,12 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
 {
-    const uint32_t wv = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
-    if (wv < B.pre_intra[B.njobs]) nbr_item(B, wv, threadIdx.x & 31);
+    const uint32_t total = B.pre_intra[B.njobs];
+    const uint32_t w0 = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 6) * 2;
+    if (w0 >= total) return;
+    const uint32_t jh = find_job_wave(B.pre_intra, B.njobs, w0); /* the wave's first block */
+    const uint32_t wv = w0 + ((threadIdx.x & 63) >> 5);
+    if (wv < total) nbr_item(B, wv, threadIdx.x & 31, jh);
 }
 
 /* The samples of an intra cell in the resolved layout (bytes 0-3 luma,
@@ -930,7 +963,7 @@ __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level)
     /* wave-uniform (W >= 64), so the job lookup uses scalar branches */
     const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * (256 / W) + threadIdx.x / W);
     if (g >= pre[B.njobs]) return; /* uniform per worker */
-    const uint32_t j = find_job(pre, B.njobs, g);
+    const uint32_t j = find_job_wave(pre, B.njobs, g);
     intra_block<W, false>(B, B.jobs[j], start[j] + (g - pre[j]), shm[threadIdx.x / W],
                           threadIdx.x % W);
 }
@@ -955,7 +988,7 @@ __device__ __forceinline__ void lf_edge(uint8_t &p1, uint8_t &p0, uint8_t &q0, u
 
 __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*tile)[65])
 {
-    const uint32_t j = find_job(B.pre_lf, B.njobs, w);
+    const uint32_t j = find_job_wave(B.pre_lf, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t l = w - B.pre_lf[j];
     const int wy = (int)(l / J.lf_wx), wx = (int)(l - (uint32_t)wy * J.lf_wx);
@@ -1037,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_lf(DBatch B)
 
 __device__ __forceinline__ void nv12_item(const DBatch &B, uint32_t w)
 {
-    const uint32_t j = find_job(B.pre_nv12, B.njobs, w);
+    const uint32_t j = find_job_wave(B.pre_nv12, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t l = w - B.pre_nv12[j];
     const uint32_t W = J.out.w, H = J.out.h, cw = (W + 1) / 2, ch = (H + 1) / 2;
@@ -1126,9 +1159,9 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     /* owner map and motion compensation (independent) */
     const uint32_t nblk = B.pre_blocks[B.njobs], nmc = B.pre_inter[B.njobs];
     for (uint32_t wv = blockIdx.x * 16 + (threadIdx.x >> 4); wv < nblk; wv += nwg * 16)
-        owner_item(B, wv, threadIdx.x & 15);
+        owner_item(B, wv, threadIdx.x & 15, find_job(B.pre_blocks, B.njobs, wv));
     for (uint32_t u = blockIdx.x * 16 + threadIdx.x / CVP9_MC_UNIT; u < nmc; u += nwg * 16)
-        mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT);
+        mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT, find_job(B.pre_inter, B.njobs, u));
     grid_barrier(B, ++phase * nwg);
     /* transforms: inter ones add to the prediction, intra ones go aside */
     double *lds = (double *)smem;
@@ -1143,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     const uint32_t nint = B.pre_intra[B.njobs];
     if (nint) {
         for (uint32_t wv = blockIdx.x * 8 + (threadIdx.x >> 5); wv < nint; wv += nwg * 8)
-            nbr_item(B, wv, threadIdx.x & 31);
+            nbr_item(B, wv, threadIdx.x & 31, find_job(B.pre_intra, B.njobs, wv));
         grid_barrier(B, ++phase * nwg);
         const uint32_t nwk = B.fused_workers < 1 ? 1 : B.fused_workers;
         if (B.fused_lanes == 64) {
