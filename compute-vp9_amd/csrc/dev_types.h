@@ -148,9 +148,11 @@ struct DBatch {
                                       to where the persistent workers start */
     uint32_t       *bar;           /* grid-barrier counter of the fused kernel (zeroed likewise) */
     /* leading levels of the intra stage, per dependency level l (rows of
-     * njobs + 1): exclusive prefix over jobs of the blocks on level l, and
-     * each job's intra-list position where level l starts */
+     * njobs + 1): exclusive prefixes over jobs of the small (at most 16x16)
+     * and of the large blocks on level l, and each job's intra-list
+     * position where level l starts (its small blocks come first) */
     const uint32_t *lvl_pre;
+    const uint32_t *lvl_pre2;
     const uint32_t *lvl_start;
     uint32_t        spin_limit;    /* bound on every dependency wait */
     uint32_t        fused_lanes;   /* fused kernel: threads per intra worker (64 or 256) */

@@ -170,24 +170,33 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
     return CVP9_OK;
 }
 
-/* Orders the intra list by dependency level, then list index. An intra
- * block depends on the earlier intra blocks owning its above row, left
- * column and above-left sample (for chroma too: a chroma neighbour lies in
- * the same 4x4 luma cells). Level 0 has no such block; otherwise the level
- * is one more than the highest level among them. Intra workers take blocks
- * in this order, so every block a worker waits for was handed out earlier,
- * and the blocks of one level can all run at once. Those neighbour samples
+/* Blocks that one 64-thread intra worker handles in a single pass. */
+static inline bool intra_small(const DBlock &k)
+{
+    return k.bw <= 16 && k.bh <= 16;
+}
+
+/* Orders the intra list by dependency level, then size class (small
+ * first), then list index. An intra block depends on the earlier intra
+ * blocks owning its above row, left column and above-left sample (for
+ * chroma too: a chroma neighbour lies in the same 4x4 luma cells). Level 0
+ * has no such block; otherwise the level is one more than the highest
+ * level among them. Intra workers take blocks in this order, so every block
+ * a worker waits for was handed out earlier, and the blocks of one level
+ * can all run at once. Those neighbour samples
  * lie on the owner's bottom row or right column, so only those cells record
  * the owner's level (plus one) in the scratch map, in list order: a cell
  * still at zero when block b reads it is not owned by an earlier intra
  * block. */
 static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint32_t W, uint32_t H,
-                        std::vector<uint32_t> &levels)
+                        std::vector<uint32_t> &levels, std::vector<uint32_t> &level_small)
 {
     levels.clear();
+    level_small.clear();
     if (ni == 0) return;
     if (ni == 1) {
         levels = {0, 1};
+        level_small = {intra_small(blocks[intra[0]]) ? 1u : 0u};
         return;
     }
     const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
@@ -220,12 +229,20 @@ static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint
         for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = 0;
         for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = 0;
     }
-    cnt.assign(depth + 1, 0);
-    for (uint32_t i = 0; i < ni; i++) cnt[lev[i] + 1]++;
-    for (uint32_t l = 0; l < depth; l++) cnt[l + 1] += cnt[l];
-    levels.assign(cnt.begin(), cnt.end());
+    /* stable counting sort by (level, size class): small blocks of a level
+     * first, so a level launch gives them 64-thread workers four to a
+     * workgroup and the large ones a workgroup each */
+    cnt.assign(2 * depth + 1, 0);
+    for (uint32_t i = 0; i < ni; i++)
+        cnt[2 * lev[i] + (intra_small(blocks[intra[i]]) ? 0 : 1) + 1]++;
+    level_small.resize(depth);
+    for (uint32_t l = 0; l < depth; l++) level_small[l] = cnt[2 * l + 1];
+    for (uint32_t k = 0; k < 2 * depth; k++) cnt[k + 1] += cnt[k];
+    levels.resize(depth + 1);
+    for (uint32_t l = 0; l <= depth; l++) levels[l] = cnt[2 * l];
     tmp.resize(ni);
-    for (uint32_t i = 0; i < ni; i++) tmp[cnt[lev[i]]++] = intra[i];
+    for (uint32_t i = 0; i < ni; i++)
+        tmp[cnt[2 * lev[i] + (intra_small(blocks[intra[i]]) ? 0 : 1)]++] = intra[i];
     memcpy(intra, tmp.data(), (size_t)ni * 4);
 }
 
@@ -291,7 +308,7 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         }
         n++;
     }
-    order_intra(blocks, intra, ni, L.W, L.H, L.levels);
+    order_intra(blocks, intra, ni, L.W, L.H, L.levels, L.level_small);
     if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
 }
 
@@ -513,7 +530,7 @@ size_t StreamState::resident_bytes() const
 size_t BatchBuilder::bytes_for(uint32_t njobs)
 {
     return align256((uint64_t)njobs * sizeof(DJob)) + align256(12ull * (njobs + 1) * 4) + 256 +
-           2ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
+           3ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
 }
 
 void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uint32_t njobs,
@@ -577,15 +594,20 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     uint32_t *tick = pre + P_N * stride; /* row P_N: per-job intra tickets */
     uint32_t head = 0;
     level_n_.clear();
+    level_ns_.clear();
     if (tune.intra_level_min > 0) {
         for (; head < CVP9_MAX_HEAD_LEVELS; head++) {
-            uint32_t c = 0;
+            uint32_t c = 0, cs = 0;
             for (uint32_t i = 0; i < njobs; i++) {
                 const std::vector<uint32_t> &lv = layouts[i]->levels;
-                if (head + 1 < lv.size()) c += lv[head + 1] - lv[head];
+                if (head + 1 < lv.size()) {
+                    c += lv[head + 1] - lv[head];
+                    cs += layouts[i]->level_small[head];
+                }
             }
             if (c == 0 || c < tune.intra_level_min) break;
-            level_n_.push_back(c);
+            level_n_.push_back(c - cs);
+            level_ns_.push_back(cs);
         }
     }
     uint32_t tail = 0;
@@ -596,25 +618,32 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     }
     n_.levels = head;
     n_.level_n = head ? level_n_.data() : nullptr;
+    n_.level_ns = head ? level_ns_.data() : nullptr;
     n_.intra_tail = tail;
-    b_.lvl_pre = b_.lvl_start = nullptr;
+    b_.lvl_pre = b_.lvl_pre2 = b_.lvl_start = nullptr;
     if (head) {
-        uint32_t *lp = (uint32_t *)(host + used_), *ls = lp + (size_t)head * stride;
+        uint32_t *ps = (uint32_t *)(host + used_), *pl = ps + (size_t)head * stride,
+                 *ls = pl + (size_t)head * stride;
         for (uint32_t l = 0; l < head; l++) {
-            uint32_t acc = 0;
+            uint32_t as = 0, al = 0;
             for (uint32_t i = 0; i < njobs; i++) {
                 const std::vector<uint32_t> &lv = layouts[i]->levels;
                 const bool has = l + 1 < lv.size();
-                lp[(size_t)l * stride + i] = acc;
+                const uint32_t cs = has ? layouts[i]->level_small[l] : 0;
+                ps[(size_t)l * stride + i] = as;
+                pl[(size_t)l * stride + i] = al;
                 ls[(size_t)l * stride + i] = has ? lv[l] : 0;
-                acc += has ? lv[l + 1] - lv[l] : 0;
+                as += cs;
+                al += has ? lv[l + 1] - lv[l] - cs : 0;
             }
-            lp[(size_t)l * stride + njobs] = acc;
+            ps[(size_t)l * stride + njobs] = as;
+            pl[(size_t)l * stride + njobs] = al;
             ls[(size_t)l * stride + njobs] = 0;
         }
         b_.lvl_pre = (const uint32_t *)(dev + used_);
-        b_.lvl_start = b_.lvl_pre + (size_t)head * stride;
-        used_ += 2ull * head * stride * 4;
+        b_.lvl_pre2 = b_.lvl_pre + (size_t)head * stride;
+        b_.lvl_start = b_.lvl_pre2 + (size_t)head * stride;
+        used_ += 3ull * head * stride * 4;
     }
 }
 

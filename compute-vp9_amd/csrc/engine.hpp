@@ -32,8 +32,10 @@ struct PrepLayout {
     uint64_t off_tx_tab[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
     /* written by plan_write: intra-list position where each dependency
-     * level starts, then the list's end (empty without intra blocks) */
+     * level starts, then the list's end (empty without intra blocks), and
+     * the number of small blocks (at most 16x16, first in their level) */
     std::vector<uint32_t> levels;
+    std::vector<uint32_t> level_small;
 };
 
 /* Validates pf and fills L. Returns CVP9_OK, CVP9_ERR_UNSUPPORTED (layout
@@ -118,7 +120,7 @@ private:
     DBatch b_{};
     StageCounts n_{};
     size_t used_ = 0;
-    std::vector<uint32_t> level_n_;
+    std::vector<uint32_t> level_n_, level_ns_;
 };
 
 /* Once per device: upload the IDCT matrices (computed with the C library's

@@ -37,7 +37,8 @@ struct Tune {
 struct StageCounts {
     uint32_t fill, blocks, inter, grp[4], intra, lf, nv12;
     uint32_t levels;          /* leading levels launched one by one */
-    const uint32_t *level_n;  /* host array: intra blocks per leading level over the batch */
+    const uint32_t *level_n;  /* host arrays, per leading level over the batch: large and */
+    const uint32_t *level_ns; /* small (at most 16x16) intra blocks */
     uint32_t intra_tail;      /* intra blocks left to the persistent workers */
     Tune tune;
 };

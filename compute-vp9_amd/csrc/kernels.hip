@@ -950,22 +950,31 @@ __global__ __launch_bounds__(256) void k_intra(DBatch B)
     intra_worker<W>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W, gridDim.x * (256 / W));
 }
 
-/* Level-synchronous alternative: one launch per dependency level, one
- * worker per intra block of that level across all jobs. Every block a
- * block depends on lies on a lower level, finished by an earlier launch, so
- * no worker waits (its edge words already carry the frame's epoch). */
-template <int W>
-__global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level)
+/* Leading dependency levels: one launch per level, one worker per intra
+ * block of that level across all jobs. Every block a block depends on lies
+ * on a lower level, finished by an earlier launch, so no worker waits (its
+ * edge words already carry the frame's epoch). The first small_wgs
+ * workgroups take the level's small blocks (at most 16x16: one pass of a
+ * 64-thread worker), four to a workgroup; the rest take one large block
+ * each on all 256 threads. */
+__global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level, uint32_t small_wgs)
 {
-    __shared__ IntraShared shm[256 / W];
-    const uint32_t *pre = B.lvl_pre + (size_t)level * (B.njobs + 1);
-    const uint32_t *start = B.lvl_start + (size_t)level * (B.njobs + 1);
-    /* wave-uniform (W >= 64), so the job lookup uses scalar branches */
-    const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * (256 / W) + threadIdx.x / W);
-    if (g >= pre[B.njobs]) return; /* uniform per worker */
-    const uint32_t j = find_job_wave(pre, B.njobs, g);
-    intra_block<W, false>(B, B.jobs[j], start[j] + (g - pre[j]), shm[threadIdx.x / W],
-                          threadIdx.x % W);
+    __shared__ IntraShared shm[4];
+    const size_t row = (size_t)level * (B.njobs + 1);
+    const uint32_t *ps = B.lvl_pre + row, *pl = B.lvl_pre2 + row, *start = B.lvl_start + row;
+    if (blockIdx.x < small_wgs) {
+        /* wave-uniform, so the job lookup runs on the whole wave */
+        const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
+        if (g >= ps[B.njobs]) return; /* uniform per worker */
+        const uint32_t j = find_job_wave(ps, B.njobs, g);
+        intra_block<64, false>(B, B.jobs[j], start[j] + (g - ps[j]), shm[threadIdx.x / 64],
+                               threadIdx.x % 64);
+    } else {
+        const uint32_t g = blockIdx.x - small_wgs; /* < pl[njobs] by the grid */
+        const uint32_t j = find_job_wave(pl, B.njobs, g);
+        intra_block<256, false>(B, B.jobs[j], start[j] + (ps[j + 1] - ps[j]) + (g - pl[j]), shm[0],
+                                threadIdx.x);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1302,11 +1311,8 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         break;
     case STAGE_INTRA:
         for (uint32_t l = 0; l < n.levels; l++) {
-            const uint32_t c = n.level_n[l];
-            if (n.tune.intra_lanes == 64)
-                hipLaunchKernelGGL(k_intra_level<64>, dim3(div_up(c, 4)), dim3(256), 0, s, b, l);
-            else
-                hipLaunchKernelGGL(k_intra_level<256>, dim3(c), dim3(256), 0, s, b, l);
+            const uint32_t sw = div_up(n.level_ns[l], 4);
+            hipLaunchKernelGGL(k_intra_level, dim3(sw + n.level_n[l]), dim3(256), 0, s, b, l, sw);
         }
         if (n.intra_tail) {
             const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;

@@ -260,10 +260,21 @@ def check_order(pkg, frame, b):
     pos = {blk: p for p, blk in enumerate(order)}
     for blk, d in deps.items():
         assert all(pos[o] < pos[blk] for o in d), f"block {blk} handed out before a dependency"
-    # level 0 (no intra dependency) comes first, in list order
-    free = sorted(blk for blk, d in deps.items() if not d)
-    assert order[:len(free)] == free
+    # level 0 (no intra dependency) comes first: small blocks, then large, each in list order
+    free = [blk for blk, d in deps.items() if not d]
+    assert order[:len(free)] == by_class(b, w, h, free)
     return n
+
+
+def small(k, w, h):
+    """At most 16x16 once clamped to the frame (one pass of a 64-thread worker)."""
+    return min(int(k["width"]), w - int(k["x"])) <= 16 and min(int(k["height"]), h - int(k["y"])) <= 16
+
+
+def by_class(b, w, h, blks):
+    """The planner's order inside one level: small blocks, then large, each in list order."""
+    return (sorted(x for x in blks if small(b[x], w, h)) +
+            sorted(x for x in blks if not small(b[x], w, h)))
 
 
 @pytest.mark.parametrize("preset,w,h", [(1, 320, 184), (2, 643, 362), (0, 136, 72)])
@@ -320,5 +331,5 @@ def test_intra_levels_match_dependencies(pkg, preset, w, h, shuffle):
         assert len(starts) == (depth + 1 if depth else 0)
         for L in range(depth):
             got = order[starts[L]:starts[L + 1]].tolist()
-            want = sorted(blk for blk, v in lev.items() if v == L)
+            want = by_class(b, w, h, [blk for blk, v in lev.items() if v == L])
             assert got == want, f"frame {idx} level {L}"
