@@ -413,8 +413,9 @@ def main():
     ap.add_argument("--mframes", type=int, default=60)
     ap.add_argument("--msteps", type=int, default=50)
     ap.add_argument("--mwarmup", type=int, default=5)
-    ap.add_argument("--mgroups", type=int, default=2,
-                    help="batches per multi-stream step (disjoint stream groups overlap)")
+    ap.add_argument("--mgroups", type=int, default=2, choices=(1, 2),
+                    help="batches per multi-stream step (two disjoint stream groups overlap on "
+                         "the engine's two queues; more groups are not supported)")
     ap.add_argument("--cpu-frames", type=int, default=100)
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
