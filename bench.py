@@ -193,7 +193,11 @@ def multi_stream(P, args):
     preps = [eng.prepare(f) for f in frames]
     del frames
     tune = None
-    if not args.no_tune:  # tune on streams opened for it, then drop them
+    if args.mtune:  # a fixed launch shape (profiling runs)
+        a = args.mtune
+        eng.set_tune(a[:4], a[4], a[5], a[6])
+        tune = eng.get_tune().as_dict()
+    elif not args.no_tune:  # tune on streams opened for it, then drop them
         tsids = [eng.open_stream() for _ in layout]
         eng.run(tsids, [preps[s * F] for s in range(len(tsids))])
         tune = eng.autotune(tsids, [preps[s * F + 1] for s in range(len(tsids))], iters=3).as_dict()
@@ -417,6 +421,11 @@ def main():
                     help="batches per multi-stream step (two disjoint stream groups overlap on "
                          "the engine's two queues; more groups are not supported)")
     ap.add_argument("--cpu-frames", type=int, default=100)
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-stream phase (for profiling the others); value is null")
+    ap.add_argument("--mtune", type=int, nargs=7, default=None,
+                    help="fixed multi-stream launch shape: res_split x4, intra_lanes, "
+                         "intra_workers, intra_level_min (no autotuning)")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
@@ -450,35 +459,37 @@ def main():
         return
     log(f"copy probe: {copy_gbps:.1f} GB/s")
 
-    ss = single_stream(P, args)
-    value = args.steps / ss["clean_s"]
-    stats = ss["stats"]
-    dom = max(stats, key=lambda k: stats[k]["ms_total"])
-    d = stats[dom]
-    kern = kernel_of(dom, ss["tune"])
-    traffic, tsrc = measured_traffic(SINGLE_WORKLOAD, kern)
-    roof = {"kernel": kern, "bound": "hbm", "achieved": d["algo_GBps"],
-            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(d["algo_GBps"] / HBM_PEAK_GBS, 4) if d["algo_GBps"] else None,
-            "traffic": traffic, "traffic_source": tsrc,
-            "algo_bytes_per_launch": d.get("algo_bytes_per_launch"),
-            "us_per_launch": d["us_per_launch"]}
-    if dom == "intra":
-        # the ordered stage is bounded by its dependency chain, not by HBM:
-        # levels per frame and the time per level
-        order = [(args.warmup + k) % args.frames for k in range(args.steps)]
-        depth = [ss["depth"][i] for i in order]
-        lm = (ss["tune"] or {}).get("intra_level_min", 0)
-        heads = sum(1 for i in order if lm and ss["width0"][i] >= lm)
-        roof["dependency_chain"] = {
-            "levels_per_frame": round(float(np.mean(depth)), 2),
-            "max_levels": int(max(depth)),
-            "us_per_level": round(d["us_per_launch"] / float(np.mean(depth)), 3),
-            "frames_with_level_launches": heads}
-    if "fp64_dense_TOPs" in d:
-        roof["fp64_dense_TOPs"] = d["fp64_dense_TOPs"]
-        roof["fp64_peak_no_fma_TOPs"] = FP64_PEAK_TOPS / 2
-    log("stages:", json.dumps(stats))
+    value, stats, roof, ss = None, {}, None, None
+    if not args.no_single:  # (skipped only for profiling the other workloads)
+        ss = single_stream(P, args)
+        value = args.steps / ss["clean_s"]
+        stats = ss["stats"]
+        dom = max(stats, key=lambda k: stats[k]["ms_total"])
+        d = stats[dom]
+        kern = kernel_of(dom, ss["tune"])
+        traffic, tsrc = measured_traffic(SINGLE_WORKLOAD, kern)
+        roof = {"kernel": kern, "bound": "hbm", "achieved": d["algo_GBps"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["algo_GBps"] / HBM_PEAK_GBS, 4) if d["algo_GBps"] else None,
+                "traffic": traffic, "traffic_source": tsrc,
+                "algo_bytes_per_launch": d.get("algo_bytes_per_launch"),
+                "us_per_launch": d["us_per_launch"]}
+        if dom == "intra":
+            # the ordered stage is bounded by its dependency chain, not by HBM:
+            # levels per frame and the time per level
+            order = [(args.warmup + k) % args.frames for k in range(args.steps)]
+            depth = [ss["depth"][i] for i in order]
+            lm = (ss["tune"] or {}).get("intra_level_min", 0)
+            heads = sum(1 for i in order if lm and ss["width0"][i] >= lm)
+            roof["dependency_chain"] = {
+                "levels_per_frame": round(float(np.mean(depth)), 2),
+                "max_levels": int(max(depth)),
+                "us_per_level": round(d["us_per_launch"] / float(np.mean(depth)), 3),
+                "frames_with_level_launches": heads}
+        if "fp64_dense_TOPs" in d:
+            roof["fp64_dense_TOPs"] = d["fp64_dense_TOPs"]
+            roof["fp64_peak_no_fma_TOPs"] = FP64_PEAK_TOPS / 2
+        log("stages:", json.dumps(stats))
 
     multi = None if args.no_multi else multi_stream(P, args)
     if multi:
@@ -492,16 +503,16 @@ def main():
     serve = None if args.no_serve else serving(P, args)
     if serve:
         log("serving:", json.dumps(serve))
-    cpu = None if args.no_cpu else cpu_baseline(P, ss["frames"], args.cpu_frames)
+    cpu = None if args.no_cpu or ss is None else cpu_baseline(P, ss["frames"], args.cpu_frames)
 
     line = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": round(value, 2) if value is not None else None,
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * ss["clean_s"] / args.steps, 4),
+        "ms_per_step": round(1e3 * ss["clean_s"] / args.steps, 4) if ss else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -509,10 +520,10 @@ def main():
         "data": "synthetic parsed frames (seeded generator), inputs resident in HBM",
         "config": {"workload": SINGLE_WORKLOAD, "frames": args.frames,
                    "step": "one frame"},
-        "timed_with_events_fps": round(args.steps / ss["timed_s"], 2),
-        "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1),
+        "timed_with_events_fps": round(args.steps / ss["timed_s"], 2) if ss else None,
+        "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1) if ss else None,
         "stages": stats,
-        "tune": ss["tune"],
+        "tune": ss["tune"] if ss else None,
         "roofline": roof,
         "copy_peak_GBps": round(copy_gbps, 1),
         "multi_stream": None if not multi else {
@@ -525,7 +536,7 @@ def main():
             "one_batch_per_step_p1080_equiv_frames_per_s": round(multi["one_batch_p1080_equiv_per_s"], 1),
             "stages_one_batch": multi["stages"],
             "tune": multi["tune"],
-            "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2)},
+            "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2) if value else None},
         "config2_2160p": c2,
         "serving": serve,
         "cpu_baseline": cpu,
