@@ -64,11 +64,17 @@
 #define CVP9_NB_WORDS 38
 
 /* Motion compensation splits an inter block into units of
- * cvp9_mc_unit_rows(bw) luma rows. A unit is 16 lanes, one quad of 4 luma
- * samples per lane (a row of more than 16 quads takes several per lane),
- * four units per wave: an 8x8 block is one full unit (8 rows x 2 quads),
- * 16x16 four, 64x64 sixty-four. */
+ * cvp9_mc_unit_rows(bw) luma rows, four units per wave. A unit is 16 lanes;
+ * each lane takes one column of 4-sample quads over a run of up to
+ * CVP9_MC_RUN rows (luma, then chroma), so a reference row is read once
+ * for two output rows. Blocks are at most 64x64: an 8x8 or a 16x16 block is
+ * one unit, 32x32 four, 64x64 sixteen. */
 #define CVP9_MC_UNIT 16
+#define CVP9_MC_RUN  4
+/* Units per workgroup of the motion-compensation kernel. Each job's unit
+ * count is padded to a multiple of it in the batch prefix, so a workgroup
+ * works for one job. */
+#define CVP9_MC_WG_UNITS 16
 #if defined(__HIPCC__)
 #define CVP9_HD __host__ __device__
 #else
@@ -79,7 +85,7 @@ CVP9_HD static inline uint32_t cvp9_mc_unit_rows(uint32_t bw)
     const uint32_t nq = (bw + 3) / 4;
     uint32_t p2 = 1;
     while (p2 < nq) p2 <<= 1;
-    return p2 >= CVP9_MC_UNIT ? 1 : CVP9_MC_UNIT / p2;
+    return p2 >= CVP9_MC_UNIT ? CVP9_MC_RUN : CVP9_MC_RUN * CVP9_MC_UNIT / p2;
 }
 
 /* block flags */
@@ -115,7 +121,9 @@ struct DJob {
     DFrame ref[8];                 /* y == nullptr: empty slot */
     const DBlock   *blocks;
     const int16_t  *coeffs;
-    const uint32_t *mc_tab;        /* per MC unit: block index << 8 | unit within the block */
+    const uint4    *mc_tab;        /* per MC unit, a copy of the block's MC fields:
+                                      x | y << 16, bw | bh << 8 | ref << 16 | unit << 24,
+                                      (uint16)mvx | (uint16)mvy << 16, block index */
     const uint32_t *intra;
     const uint32_t *tx_tab[4];
     uint32_t       *owner;         /* ow x oh cells, local block index */

@@ -127,8 +127,10 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
         /* blocks on the 4x4 grid: neighbour ownership is exact per cell */
         if ((b.x & 3) || (b.y & 3) || (b.width & 3) || (b.height & 3)) return CVP9_ERR_UNSUPPORTED;
         const bool intra = b.is_intra != 0;
-        /* intra blocks publish their edges in fixed 64-sample records */
-        if (intra && (c.bw > CVP9_MAX_INTRA_DIM || c.bh > CVP9_MAX_INTRA_DIM)) return CVP9_ERR_UNSUPPORTED;
+        /* VP9 blocks are at most 64x64; intra blocks publish their edges in
+         * fixed 64-sample records, motion compensation spreads a block row
+         * over at most 16 lanes */
+        if (c.bw > CVP9_MAX_INTRA_DIM || c.bh > CVP9_MAX_INTRA_DIM) return CVP9_ERR_UNSUPPORTED;
         bool res = !b.skip;
         if (res) {
             if (b.tx_size > 3) return CVP9_ERR_UNSUPPORTED;
@@ -160,7 +162,7 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
 
     uint64_t off = 0;
     L.off_blocks = off; off = align256(off + (uint64_t)L.n_blocks * sizeof(DBlock));
-    L.off_mc_tab = off; off = align256(off + (uint64_t)L.n_mc * 4);
+    L.off_mc_tab = off; off = align256(off + (uint64_t)L.n_mc * 16);
     L.off_intra = off;  off = align256(off + (uint64_t)L.n_intra * 4);
     for (int k = 0; k < 4; k++) {
         L.off_tx_tab[k] = off; off = align256(off + (uint64_t)L.cls_ntx[k] * 8);
@@ -250,7 +252,7 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
 {
     const int W = (int)L.W, H = (int)L.H;
     DBlock *blocks = (DBlock *)(dst + L.off_blocks);
-    uint32_t *mc_tab = (uint32_t *)(dst + L.off_mc_tab);
+    uint32_t *mc_tab = (uint32_t *)(dst + L.off_mc_tab); /* 4 words per unit */
     uint32_t nm = 0;
     uint32_t *intra = (uint32_t *)(dst + L.off_intra);
     uint32_t *txt[4], cn[4] = {0, 0, 0, 0};
@@ -294,7 +296,13 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         else {
             const uint32_t rows = cvp9_mc_unit_rows(d.bw);
             const uint32_t units = (d.bh + rows - 1) / rows;
-            for (uint32_t u = 0; u < units; u++) mc_tab[nm++] = n << 8 | u;
+            for (uint32_t u = 0; u < units; u++, nm++) {
+                uint32_t *m = mc_tab + 4 * (size_t)nm;
+                m[0] = (uint32_t)d.x | (uint32_t)d.y << 16;
+                m[1] = (uint32_t)d.bw | (uint32_t)d.bh << 8 | (uint32_t)d.ref << 16 | u << 24;
+                m[2] = (uint32_t)(uint16_t)d.mvx | (uint32_t)(uint16_t)d.mvy << 16;
+                m[3] = n;
+            }
         }
         if (res) { /* transforms of inter and intra blocks alike, list order */
             const int k = d.tx;
@@ -490,7 +498,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     }
     j.blocks = (const DBlock *)(dev + L.off_blocks);
     j.coeffs = (const int16_t *)(dev + L.off_coeffs);
-    j.mc_tab = (const uint32_t *)(dev + L.off_mc_tab);
+    j.mc_tab = (const uint4 *)(dev + L.off_mc_tab);
     j.intra = (const uint32_t *)(dev + L.off_intra);
     for (int k = 0; k < 4; k++) {
         j.tx_tab[k] = (const uint32_t *)(dev + L.off_tx_tab[k]);
@@ -554,7 +562,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
         }
         c[P_FILL] = fill;
         c[P_BLK] = J.n_blocks;
-        c[P_INTER] = J.n_mc;
+        c[P_INTER] = (J.n_mc + CVP9_MC_WG_UNITS - 1) / CVP9_MC_WG_UNITS * CVP9_MC_WG_UNITS;
         for (int k = 0; k < 4; k++) c[P_G0 + k] = div_up(J.cls_ntx[k], 64u >> (2 + k));
         c[P_INTRA] = J.n_intra;
         c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;

@@ -593,9 +593,9 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
      * is uniform over sizes inside a run and chosen per size afterwards */
     struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; uint32_t level_min; };
     const Cand cands[] = {
-        {1, 256, 2 * cu, 0},  {4, 256, 2 * cu, 0},      {2, 64, 2 * cu, 0},
-        {1, 64, 2 * cu, 64},  {2, 64, 2 * cu, 2 * cu},  {4, 256, 2 * cu, 8 * cu},
-        {1, 256, 2 * cu, 2 * cu}, {2, 256, 4 * cu, 64},
+        {1, 256, 2 * cu, 0},      {4, 256, 2 * cu, 0},      {2, 64, 2 * cu, 0},
+        {1, 64, 4 * cu, 0},       {2, 64, 4 * cu, 4 * cu},  {4, 256, 2 * cu, 8 * cu},
+        {1, 256, 2 * cu, 2 * cu}, {2, 256, 2 * cu, 4 * cu}, {1, 256, 4 * cu, 4 * cu},
     };
     const bool was_timing = e->timing;
     const Tune saved = e->tune;

@@ -151,47 +151,50 @@ __global__ __launch_bounds__(256) void k_owner(DBatch B)
 
 /* ------------------------------------------------------------------------ */
 /* Motion compensation (vp9_reconstruct.c:94-131, :248-264, :312-334).      */
-/* 16 lanes per unit of an inter block: cvp9_mc_unit_rows(bw) luma rows,   */
-/* one quad of 4 samples per lane, plus the chroma rows under those rows.   */
-/* Quads whose 5-sample source window lies inside the reference row read it */
-/* with two aligned dword loads; the others take each tap with the edge     */
-/* clamp.                                                                   */
+/* 16 lanes per unit of an inter block (cvp9_mc_unit_rows(bw) luma rows and */
+/* the chroma rows under them). A lane computes one column of 4-sample      */
+/* quads over a run of rows: each reference row of the run is loaded and    */
+/* filtered horizontally once and serves the output rows above and below    */
+/* it. A source window of 5 samples inside the reference row is read with   */
+/* two aligned dword loads; others take each tap with the edge clamp.       */
 
-__device__ __forceinline__ uint32_t mc_quad(const uint8_t *__restrict__ ref, uint32_t rs, int rw,
-                                            int rh, int ix, int iy, int fx, int fy, int c0, int r)
+/* the 5 source samples [sx, sx+5) of reference row y (clamped), packed */
+__device__ __forceinline__ uint64_t mc_src(const uint8_t *__restrict__ ref, uint32_t rs, int rw,
+                                           int rh, int sx, int y)
 {
-    const uint8_t *r0 = ref + (size_t)clampi(iy + r, 0, rh - 1) * rs;
-    const uint8_t *r1 = ref + (size_t)clampi(iy + r + 1, 0, rh - 1) * rs;
-    const int sx = ix + c0;
-    uint32_t a[5], b[5];
+    const uint8_t *row = ref + (size_t)clampi(y, 0, rh - 1) * rs;
     if (sx >= 0 && sx + 4 <= rw - 1) {
-        /* rows start 64-byte aligned, so both windows share one offset; the
-         * 8-byte window may pass the row end by 3 bytes, inside the row pitch
-         * or the frame's tail slack */
+        /* rows start 64-byte aligned; the 8-byte window may pass the row end
+         * by 3 bytes, inside the row pitch or the frame's tail slack */
         const uint32_t o = (uint32_t)sx & 3;
-        const uint32_t *qa = (const uint32_t *)(r0 + (sx - (int)o));
-        const uint32_t *qb = (const uint32_t *)(r1 + (sx - (int)o));
-        const uint64_t va = (((uint64_t)qa[1] << 32) | qa[0]) >> (8 * o);
-        const uint64_t vb = (((uint64_t)qb[1] << 32) | qb[0]) >> (8 * o);
-#pragma unroll
-        for (int e = 0; e < 5; e++) {
-            a[e] = (uint32_t)(va >> (8 * e)) & 0xff;
-            b[e] = (uint32_t)(vb >> (8 * e)) & 0xff;
-        }
-    } else {
-#pragma unroll
-        for (int e = 0; e < 5; e++) {
-            const int x = clampi(sx + e, 0, rw - 1);
-            a[e] = r0[x];
-            b[e] = r1[x];
-        }
+        const uint32_t *q = (const uint32_t *)(row + (sx - (int)o));
+        return (((uint64_t)q[1] << 32) | q[0]) >> (8 * o);
     }
+    uint64_t v = 0;
+#pragma unroll
+    for (int e = 0; e < 5; e++) v |= (uint64_t)row[clampi(sx + e, 0, rw - 1)] << (8 * e);
+    return v;
+}
+
+/* horizontal pass of a packed source window: 4 samples, 8 bits each */
+__device__ __forceinline__ uint32_t mc_hpass(uint64_t v, uint32_t fx)
+{
+    uint32_t h = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint32_t a = (uint32_t)(v >> (8 * e)) & 0xff, b = (uint32_t)(v >> (8 * e + 8)) & 0xff;
+        h |= ((a * (8 - fx) + b * fx + 4) >> 3) << (8 * e);
+    }
+    return h;
+}
+
+__device__ __forceinline__ uint32_t mc_vpass(uint32_t t, uint32_t b, uint32_t fy)
+{
     uint32_t out = 0;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        const uint32_t top = (a[e] * (8 - fx) + a[e + 1] * fx + 4) >> 3;
-        const uint32_t bot = (b[e] * (8 - fx) + b[e + 1] * fx + 4) >> 3;
-        out |= ((top * (8 - fy) + bot * fy + 4) >> 3) << (8 * e);
+        const uint32_t x = (t >> (8 * e)) & 0xff, y = (b >> (8 * e)) & 0xff;
+        out |= ((x * (8 - fy) + y * fy + 4) >> 3) << (8 * e);
     }
     return out;
 }
@@ -205,74 +208,125 @@ __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
     }
 }
 
-/* One motion-compensation unit: entry u of the batch's unit list, on 16
- * lanes (li = 0..15). */
-__device__ __forceinline__ void mc_unit(const DBatch &B, uint32_t u, int li, uint32_t jhint)
+/* n output rows of one quad column: prediction rows y0 .. y0+n-1 of the
+ * plane at 1/8-sample position (ix + fx/8, y0 + fy/8) (ix: the quad's first
+ * sample), stored at d with pitch ds, nv samples of the quad valid.
+ * ref == nullptr: an empty slot predicts flat 127 (:258-263, :328-333). */
+__device__ __forceinline__ void mc_run(const uint8_t *__restrict__ ref, uint32_t rs, int rw, int rh,
+                                       int ix, int y0, uint32_t fx, uint32_t fy, int n,
+                                       uint8_t *d, uint32_t ds, int nv)
 {
-    /* the wave's job (looked up for its first unit) unless u lies past it */
-    const uint32_t j = u < B.pre_inter[jhint + 1] ? jhint : find_job(B.pre_inter, B.njobs, u);
-    const DJob &J = B.jobs[j];
-    const uint32_t e = J.mc_tab[u - B.pre_inter[j]];
-    const DBlock k = J.blocks[e >> 8];
-    const int bx = k.x, by = k.y, bw = k.bw, bh = k.bh;
-    const int rows = (int)cvp9_mc_unit_rows((uint32_t)bw);
-    const int r0 = (int)(e & 255) * rows;
-    const int r1 = min(r0 + rows, bh);
-    const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1;
-    const int cr0 = (r0 + 1) >> 1, cr1 = (r1 + 1) >> 1; /* chroma rows under [r0, r1) */
-    const int lanes_per_row = CVP9_MC_UNIT / rows;       /* power of two */
-    const int nq = (bw + 3) >> 2, ncq = (cbw + 3) >> 2;
-    /* frames held in registers: a store could alias the descriptor as far
-     * as the compiler knows, and would make it reload them */
-    const DFrame out = J.out;
-    DFrame R = {};
-    if (k.ref < 8) R = J.ref[k.ref];
-    const bool missing = !R.y; /* empty slot: flat 127 (:258-263, :328-333) */
-
-    /* luma */
-    {
-        const int rr = li / lanes_per_row, qi = li - rr * lanes_per_row;
-        const int r = r0 + rr;
-        if (r < r1) {
-            const int px8 = bx * 8 + k.mvx, py8 = by * 8 + k.mvy;
-            for (int q = qi; q < nq; q += lanes_per_row) {
-                uint32_t v = 0x7f7f7f7fu;
-                if (!missing)
-                    v = mc_quad(R.y, R.sy, (int)R.w, (int)R.h, px8 >> 3, py8 >> 3, px8 & 7,
-                                py8 & 7, 4 * q, r);
-                store_quad(out.y + (size_t)(by + r) * out.sy + bx + 4 * q, v, bw - 4 * q);
-            }
-        }
+    if (!ref) {
+        for (int t = 0; t < n; t++) store_quad(d + (size_t)t * ds, 0x7f7f7f7fu, nv);
+        return;
     }
-    /* chroma: MV halved toward zero, bounds from half the reference luma size */
-    {
-        const int crows = cr1 - cr0;
-        const int cmx = k.mvx / 2, cmy = k.mvy / 2;
-        for (int i = li; i < 2 * crows * ncq; i += CVP9_MC_UNIT) {
-            const int p = i / (crows * ncq);
-            const int t = i - p * crows * ncq;
-            const int cr = cr0 + t / ncq, cq = t - (t / ncq) * ncq;
-            uint32_t v = 0x7f7f7f7fu;
-            if (!missing) {
-                const int px8 = cbx * 8 + cmx, py8 = cby * 8 + cmy;
-                v = mc_quad(p ? R.v : R.u, R.suv, (int)R.w / 2, (int)R.h / 2, px8 >> 3,
-                            py8 >> 3, px8 & 7, py8 & 7, 4 * cq, cr);
+    for (int t0 = 0; t0 < n; t0 += CVP9_MC_RUN) {
+        const int m = min(n - t0, CVP9_MC_RUN);
+        uint64_t src[CVP9_MC_RUN + 1];
+#pragma unroll
+        for (int t = 0; t <= CVP9_MC_RUN; t++)
+            if (t <= m) src[t] = mc_src(ref, rs, rw, rh, ix, y0 + t0 + t);
+        uint32_t top = mc_hpass(src[0], fx);
+#pragma unroll
+        for (int t = 0; t < CVP9_MC_RUN; t++) {
+            if (t < m) {
+                const uint32_t bot = mc_hpass(src[t + 1], fx);
+                store_quad(d + (size_t)(t0 + t) * ds, mc_vpass(top, bot, fy), nv);
+                top = bot;
             }
-            uint8_t *pl = p ? out.v : out.u;
-            store_quad(pl + (size_t)(cby + cr) * out.suv + cbx + 4 * cq, v, cbw - 4 * cq);
         }
     }
 }
 
+/* Spreads ncol (<= 16) columns by nrows rows over the 16 lanes of a unit:
+ * 16 / ncol groups of lanes, each a run of rows. False for an idle lane. */
+__device__ __forceinline__ bool mc_lane(int li, int ncol, int nrows, int &col, int &ra, int &n)
+{
+    const int groups = CVP9_MC_UNIT / ncol;
+    const int g = li / ncol;
+    if (g >= groups) return false;
+    col = li - g * ncol;
+    const int per = (nrows + groups - 1) / groups;
+    ra = g * per;
+    n = min(nrows - ra, per);
+    return n > 0;
+}
+
+/* One motion-compensation unit of job J on 16 lanes (li = 0..15): e is its
+ * unit-table entry, R the reference frame of its block (R.y == nullptr: the
+ * slot is empty or the block names none). */
+__device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R, int li)
+{
+    const int bx = (int)(e.x & 0xffff), by = (int)(e.x >> 16);
+    const int bw = (int)(e.y & 0xff), bh = (int)((e.y >> 8) & 0xff);
+    const int mvx = (int16_t)(e.z & 0xffff), mvy = (int16_t)(e.z >> 16);
+    const int rows = (int)cvp9_mc_unit_rows((uint32_t)bw);
+    const int r0 = (int)(e.y >> 24) * rows;
+    const int r1 = min(r0 + rows, bh);
+    const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1;
+    const int cr0 = (r0 + 1) >> 1, cr1 = (r1 + 1) >> 1; /* chroma rows under [r0, r1) */
+    const int nq = (bw + 3) >> 2, ncq = (cbw + 3) >> 2;   /* nq <= 16, ncq <= 8 */
+    /* held in registers: a store could alias the descriptor as far as the
+     * compiler knows, and would make it reload it */
+    const DFrame out = J.out;
+    int col, ra, n;
+    /* luma */
+    if (mc_lane(li, nq, r1 - r0, col, ra, n)) {
+        const int px8 = bx * 8 + mvx, py8 = by * 8 + mvy;
+        const int r = r0 + ra;
+        mc_run(R.y, R.sy, (int)R.w, (int)R.h, (px8 >> 3) + 4 * col, (py8 >> 3) + r, px8 & 7,
+               py8 & 7, n, out.y + (size_t)(by + r) * out.sy + bx + 4 * col, out.sy, bw - 4 * col);
+    }
+    /* chroma: MV halved toward zero, bounds from half the reference luma
+     * size; the unit's U columns, then its V columns */
+    if (mc_lane(li, 2 * ncq, cr1 - cr0, col, ra, n)) {
+        const int p = col >= ncq, cq = col - p * ncq;
+        const int px8 = cbx * 8 + mvx / 2, py8 = cby * 8 + mvy / 2;
+        const int cr = cr0 + ra;
+        uint8_t *pl = p ? out.v : out.u;
+        mc_run(R.y ? (p ? R.v : R.u) : nullptr, R.suv, (int)R.w / 2, (int)R.h / 2,
+               (px8 >> 3) + 4 * cq, (py8 >> 3) + cr, px8 & 7, py8 & 7, n,
+               pl + (size_t)(cby + cr) * out.suv + cbx + 4 * cq, out.suv, cbw - 4 * cq);
+    }
+}
+
+/* unit u of the batch list (the fused kernel's grid-stride form) */
+__device__ __forceinline__ void mc_unit_at(const DBatch &B, uint32_t u, int li)
+{
+    const uint32_t j = find_job(B.pre_inter, B.njobs, u);
+    const DJob &J = B.jobs[j];
+    const uint32_t lu = u - B.pre_inter[j];
+    if (lu >= J.n_mc) return; /* padding of the job's count */
+    const uint4 e = J.mc_tab[lu];
+    const uint32_t ref = (e.y >> 16) & 0xff;
+    DFrame R = {};
+    if (ref < 8) R = J.ref[ref];
+    mc_unit(J, e, R, li);
+}
+
+/* A workgroup takes CVP9_MC_WG_UNITS units of one job (job counts are
+ * padded to whole workgroups): the job descriptor is read with scalar loads
+ * and its reference table is staged in LDS once. */
 __global__ __launch_bounds__(256) void k_mc(DBatch B)
 {
-    const uint32_t total = B.pre_inter[B.njobs];
-    const uint32_t u0 = xcd_block(blockIdx.x, gridDim.x) * (256 / CVP9_MC_UNIT) +
-                        (threadIdx.x >> 6) * (64 / CVP9_MC_UNIT); /* the wave's first unit */
-    if (u0 >= total) return;
-    const uint32_t jh = find_job_wave(B.pre_inter, B.njobs, u0);
-    const uint32_t u = u0 + (threadIdx.x & 63) / CVP9_MC_UNIT;
-    if (u < total) mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT, jh);
+    static_assert(CVP9_MC_WG_UNITS * CVP9_MC_UNIT == 256, "one unit per 16 lanes");
+    __shared__ DFrame s_ref[8];
+    const uint32_t u0 = xcd_block(blockIdx.x, gridDim.x) * CVP9_MC_WG_UNITS;
+    const uint32_t j = __builtin_amdgcn_readfirstlane(find_job_wave(B.pre_inter, B.njobs, u0));
+    const DJob &J = B.jobs[j];
+    constexpr uint32_t ref_words = sizeof(s_ref) / 4;
+    if (threadIdx.x < ref_words)
+        ((uint32_t *)s_ref)[threadIdx.x] = ((const uint32_t *)J.ref)[threadIdx.x];
+    const uint32_t lu = u0 - B.pre_inter[j] + threadIdx.x / CVP9_MC_UNIT;
+    const bool live = lu < J.n_mc;
+    uint4 e = make_uint4(0, 0, 0, 0);
+    if (live) e = J.mc_tab[lu];
+    __syncthreads();
+    if (!live) return;
+    const uint32_t ref = (e.y >> 16) & 0xff;
+    DFrame R = {};
+    if (ref < 8) R = s_ref[ref];
+    mc_unit(J, e, R, threadIdx.x % CVP9_MC_UNIT);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -593,9 +647,8 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
         const bool sy = (uint32_t)(wy >> 32) != epoch, su = (uint32_t)(wu >> 32) != epoch,
                    sv = (uint32_t)(wv >> 32) != epoch;
         if (!(sy || su || sv)) break;
-        if (spins < 4) __builtin_amdgcn_s_sleep(1);
-        else if (spins < 16) __builtin_amdgcn_s_sleep(8);
-        else __builtin_amdgcn_s_sleep(32);
+        if (spins < 8) __builtin_amdgcn_s_sleep(1); /* 64 clocks */
+        else __builtin_amdgcn_s_sleep(4);           /* 256 clocks */
         if (++spins > spin_limit) {
             atomicOr(err, 1u);
             break;
@@ -607,6 +660,28 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
         if (su) wu = __hip_atomic_load(pu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (sv) wv = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    const uint32_t sh = 8 * (c & 3);
+    return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
+}
+
+/* The cell of an intra neighbour from its edge record: one load of each
+ * of its Y, U and V words; stale when a word does not carry the frame's
+ * epoch yet (the neighbour has not published it). */
+__device__ __forceinline__ uint64_t edge_try(const uint64_t *edge, uint32_t epoch, uint64_t d,
+                                             bool &stale)
+{
+    const uint32_t o = (uint32_t)d;
+    const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
+    const uint32_t c = (uint32_t)(d >> 40) & 31;
+    const uint64_t *rec = edge + (size_t)o * CVP9_EDGE_WORDS;
+    const uint64_t wy = __hip_atomic_load(rec + (side ? 16 : 0) + lw, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t wu = __hip_atomic_load(rec + (side ? 40 : 32) + (c >> 2), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t wv = __hip_atomic_load(rec + (side ? 56 : 48) + (c >> 2), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    stale = (uint32_t)(wy >> 32) != epoch || (uint32_t)(wu >> 32) != epoch ||
+            (uint32_t)(wv >> 32) != epoch;
     const uint32_t sh = 8 * (c & 3);
     return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
 }
@@ -644,6 +719,7 @@ struct IntraShared {
     uint8_t edge[3][2][64]; /* final bottom row / right column */
     uint32_t sum[3];
     uint32_t ticket;
+    uint32_t job;
     uint8_t tl[3];
 };
 
@@ -715,16 +791,20 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
      *    edge records. ncx + ncy + 1 <= 33 cells: one per thread. */
     if (tid < 3) S.sum[tid] = 0;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
-    if (tid <= ncx + ncy) {
+    if (tid < 64) { /* the worker's first wave, whole */
         const int c = tid;
         const int kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
         const bool present = kind == 0 ? y > 0 : (kind == 1 ? x > 0 : (x > 0 && y > 0));
-        uint64_t v = 0x7f7f7f7f7f7f7f7full;
-        if (present) {
-            v = cell;
-            if ((dmask >> c) & 1) v = cell_from_edges(jedge, epoch, jerr, v, B.spin_limit);
-        }
-        if (kind == 2) {
+        const bool pend = c <= ncx + ncy && present && ((dmask >> c) & 1);
+        uint64_t v = present ? cell : 0x7f7f7f7f7f7f7f7full;
+        bool stale = false;
+        if (pend) v = edge_try(jedge, epoch, cell, stale);
+        /* a neighbour still running: poll the cell's words until it has
+         * published them */
+        if (stale) v = cell_from_edges(jedge, epoch, jerr, cell, B.spin_limit);
+        if (c > ncx + ncy) {
+            /* no cell */
+        } else if (kind == 2) {
             S.tl[0] = (uint8_t)(v >> 24);
             S.tl[1] = (uint8_t)(v >> 40);
             S.tl[2] = (uint8_t)(v >> 56);
@@ -893,6 +973,26 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     }
 }
 
+/* A job of the batch with intra tickets left, looked for by one whole wave
+ * from job j + 1 on, round robin; B.njobs when every job's tickets are
+ * taken. Each lane reads one job's ticket counter: a plain load, so workers
+ * that run out of work do not queue up on the counters' atomics. */
+__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
+{
+    const uint32_t lane = __lane_id();
+    for (uint32_t k = 0; k < B.njobs; k += 64) {
+        uint32_t idx = j + 1 + k + lane;
+        while (idx >= B.njobs) idx -= B.njobs;
+        bool open = false;
+        if (k + lane < B.njobs)
+            open = __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                   B.jobs[idx].n_intra;
+        const uint64_t m = __ballot(open);
+        if (m) return __shfl(idx, __builtin_ctzll(m));
+    }
+    return B.njobs;
+}
+
 /* Persistent workers, W threads each: 256 (one workgroup per block) or 64
  * (one wave per block, four independent workers per workgroup); this is
  * worker wk of nwk. */
@@ -912,18 +1012,25 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
      * forever. */
     uint32_t j = __builtin_amdgcn_readfirstlane(
         find_job(B.pre_intra, B.njobs, (uint32_t)((uint64_t)wk * total / nwk)));
-    uint32_t left = B.njobs;
     if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
     worker_sync<W>();
     uint32_t tk = __builtin_amdgcn_readfirstlane(S.ticket);
     for (;;) {
         const DJob &J = B.jobs[j];
         if (tk >= J.n_intra) {
-            /* this job's blocks are all taken: try the next one */
-            if (--left == 0) break;
-            j = j + 1 == B.njobs ? 0 : j + 1;
-            if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
+            /* this job's blocks are all taken: go on with a job that has
+             * some left (the first sync: every thread has read S.ticket) */
             worker_sync<W>();
+            if (tid < 64) {
+                const uint32_t nj = find_open_job(B, j);
+                if (tid == 0) {
+                    S.job = nj;
+                    if (nj < B.njobs) S.ticket = atomicAdd(B.jticket + nj, 1u);
+                }
+            }
+            worker_sync<W>();
+            j = __builtin_amdgcn_readfirstlane(S.job);
+            if (j >= B.njobs) break;
             tk = __builtin_amdgcn_readfirstlane(S.ticket);
             continue;
         }
@@ -1170,7 +1277,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     for (uint32_t wv = blockIdx.x * 16 + (threadIdx.x >> 4); wv < nblk; wv += nwg * 16)
         owner_item(B, wv, threadIdx.x & 15, find_job(B.pre_blocks, B.njobs, wv));
     for (uint32_t u = blockIdx.x * 16 + threadIdx.x / CVP9_MC_UNIT; u < nmc; u += nwg * 16)
-        mc_unit(B, u, threadIdx.x % CVP9_MC_UNIT, find_job(B.pre_inter, B.njobs, u));
+        mc_unit_at(B, u, threadIdx.x % CVP9_MC_UNIT);
     grid_barrier(B, ++phase * nwg);
     /* transforms: inter ones add to the prediction, intra ones go aside */
     double *lds = (double *)smem;
@@ -1296,7 +1403,7 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 16) : 1), dim3(256), 0, s, b);
         break;
     case STAGE_MC:
-        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(div_up(n.inter, 256 / CVP9_MC_UNIT)), dim3(256), 0, s, b);
+        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b);
         break;
     case STAGE_RES4:
         return launch_residual<4>(b, n, 0, s);
