@@ -424,8 +424,11 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             if (!t.a || !t.b) return CVP9_ERR_GPU;
             hipEventRecord(t.a, qs);
         }
-        if ((he = cvp9_launch_stage(st, bb.batch(), bb.counts(), qs)) != hipSuccess)
+        /* outside per-stage timing, independent stages share launches */
+        const int span = timing ? 1 : cvp9_stage_span(st, bb.counts());
+        if ((he = cvp9_launch_stages(st, span, bb.batch(), bb.counts(), qs)) != hipSuccess)
             return to_err(he);
+        st += span - 1;
         if (timed) {
             hipEventRecord(t.b, qs);
             e->timed.push_back(t);

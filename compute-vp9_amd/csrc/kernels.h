@@ -51,6 +51,15 @@ hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double
 hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t s);
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s);
 
+/* Consecutive stages that can share one launch, starting at `stage`: the
+ * owner map with motion compensation (2), and all four transform sizes (4)
+ * while they need at most CVP9_RES_SMALL_MAX_WGS workgroups; 1 otherwise.
+ * cvp9_launch_stages runs stages [stage, stage + span) with that launch. */
+#define CVP9_RES_SMALL_MAX_WGS 2048
+int cvp9_stage_span(int stage, const StageCounts &n);
+hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageCounts &n,
+                              hipStream_t s);
+
 /* Fused persistent kernel: every stage of the batch in one launch of `wgs`
  * co-resident workgroups (at most cvp9_fused_max_wgs()). */
 hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s);

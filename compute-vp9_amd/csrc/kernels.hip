@@ -139,14 +139,20 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
     }
 }
 
-__global__ __launch_bounds__(256) void k_owner(DBatch B)
+/* workgroup wg of the owner-map stage: 16 blocks, 16 lanes each */
+__device__ __forceinline__ void owner_wg(const DBatch &B, uint32_t wg)
 {
     const uint32_t total = B.pre_blocks[B.njobs];
-    const uint32_t w0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4; /* the wave's first block */
+    const uint32_t w0 = wg * 16 + (threadIdx.x >> 6) * 4; /* the wave's first block */
     if (w0 >= total) return;
     const uint32_t jh = find_job_wave(B.pre_blocks, B.njobs, w0);
     const uint32_t wv = w0 + ((threadIdx.x & 63) >> 4);
     if (wv < total) owner_item(B, wv, threadIdx.x & 15, jh);
+}
+
+__global__ __launch_bounds__(256) void k_owner(DBatch B)
+{
+    owner_wg(B, blockIdx.x);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -238,6 +244,41 @@ __device__ __forceinline__ void mc_run(const uint8_t *__restrict__ ref, uint32_t
     }
 }
 
+/* The first CVP9_MC_RUN rows of a run in two steps, so that a lane's loads
+ * for luma and chroma are in flight together: the m + 1 source rows
+ * (m = min(n, CVP9_MC_RUN)), then the filtered rows stored. */
+struct McRun {
+    uint64_t src[CVP9_MC_RUN + 1];
+    int m;
+};
+
+__device__ __forceinline__ void mc_load(McRun &q, const uint8_t *__restrict__ ref, uint32_t rs,
+                                        int rw, int rh, int ix, int y0, int n)
+{
+    q.m = min(n, CVP9_MC_RUN);
+#pragma unroll
+    for (int t = 0; t <= CVP9_MC_RUN; t++)
+        if (ref && t <= q.m) q.src[t] = mc_src(ref, rs, rw, rh, ix, y0 + t);
+}
+
+__device__ __forceinline__ void mc_emit(const McRun &q, bool flat, uint32_t fx, uint32_t fy,
+                                        uint8_t *d, uint32_t ds, int nv)
+{
+    uint32_t top = flat ? 0 : mc_hpass(q.src[0], fx);
+#pragma unroll
+    for (int t = 0; t < CVP9_MC_RUN; t++) {
+        if (t < q.m) {
+            uint32_t v = 0x7f7f7f7fu; /* empty slot: flat 127 */
+            if (!flat) {
+                const uint32_t bot = mc_hpass(q.src[t + 1], fx);
+                v = mc_vpass(top, bot, fy);
+                top = bot;
+            }
+            store_quad(d + (size_t)t * ds, v, nv);
+        }
+    }
+}
+
 /* Spreads ncol (<= 16) columns by nrows rows over the 16 lanes of a unit:
  * 16 / ncol groups of lanes, each a run of rows. False for an idle lane. */
 __device__ __forceinline__ bool mc_lane(int li, int ncol, int nrows, int &col, int &ra, int &n)
@@ -269,25 +310,35 @@ __device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R,
     /* held in registers: a store could alias the descriptor as far as the
      * compiler knows, and would make it reload it */
     const DFrame out = J.out;
-    int col, ra, n;
-    /* luma */
-    if (mc_lane(li, nq, r1 - r0, col, ra, n)) {
-        const int px8 = bx * 8 + mvx, py8 = by * 8 + mvy;
-        const int r = r0 + ra;
-        mc_run(R.y, R.sy, (int)R.w, (int)R.h, (px8 >> 3) + 4 * col, (py8 >> 3) + r, px8 & 7,
-               py8 & 7, n, out.y + (size_t)(by + r) * out.sy + bx + 4 * col, out.sy, bw - 4 * col);
-    }
+    /* luma and chroma runs of this lane; all their source rows are loaded
+     * before either is filtered */
+    int lcol, lra, ln, ccol, cra, cn;
+    const bool lon = mc_lane(li, nq, r1 - r0, lcol, lra, ln);
+    const bool con = mc_lane(li, 2 * ncq, cr1 - cr0, ccol, cra, cn);
+    const int lpx8 = bx * 8 + mvx, lpy8 = by * 8 + mvy;
     /* chroma: MV halved toward zero, bounds from half the reference luma
      * size; the unit's U columns, then its V columns */
-    if (mc_lane(li, 2 * ncq, cr1 - cr0, col, ra, n)) {
-        const int p = col >= ncq, cq = col - p * ncq;
-        const int px8 = cbx * 8 + mvx / 2, py8 = cby * 8 + mvy / 2;
-        const int cr = cr0 + ra;
-        uint8_t *pl = p ? out.v : out.u;
-        mc_run(R.y ? (p ? R.v : R.u) : nullptr, R.suv, (int)R.w / 2, (int)R.h / 2,
-               (px8 >> 3) + 4 * cq, (py8 >> 3) + cr, px8 & 7, py8 & 7, n,
-               pl + (size_t)(cby + cr) * out.suv + cbx + 4 * cq, out.suv, cbw - 4 * cq);
-    }
+    const int cpx8 = cbx * 8 + mvx / 2, cpy8 = cby * 8 + mvy / 2;
+    const int cp = ccol >= ncq, cq = ccol - cp * ncq;
+    const uint8_t *cref = R.y ? (cp ? R.v : R.u) : nullptr;
+    const int lx = (lpx8 >> 3) + 4 * lcol, ly = (lpy8 >> 3) + r0 + lra;
+    const int cx = (cpx8 >> 3) + 4 * cq, cy = (cpy8 >> 3) + cr0 + cra;
+    uint8_t *ld = out.y + (size_t)(by + r0 + lra) * out.sy + bx + 4 * lcol;
+    uint8_t *cd = (cp ? out.v : out.u) + (size_t)(cby + cr0 + cra) * out.suv + cbx + 4 * cq;
+    McRun lq, cqr;
+    lq.m = cqr.m = 0;
+    if (lon) mc_load(lq, R.y, R.sy, (int)R.w, (int)R.h, lx, ly, ln);
+    if (con) mc_load(cqr, cref, R.suv, (int)R.w / 2, (int)R.h / 2, cx, cy, cn);
+    if (lon) mc_emit(lq, !R.y, lpx8 & 7, lpy8 & 7, ld, out.sy, bw - 4 * lcol);
+    if (con) mc_emit(cqr, !cref, cpx8 & 7, cpy8 & 7, cd, out.suv, cbw - 4 * cq);
+    /* rows past the first CVP9_MC_RUN of a run (none with blocks of at most
+     * 64x64, whose runs are at most four rows) */
+    if (lon && ln > CVP9_MC_RUN)
+        mc_run(R.y, R.sy, (int)R.w, (int)R.h, lx, ly + CVP9_MC_RUN, lpx8 & 7, lpy8 & 7,
+               ln - CVP9_MC_RUN, ld + (size_t)CVP9_MC_RUN * out.sy, out.sy, bw - 4 * lcol);
+    if (con && cn > CVP9_MC_RUN)
+        mc_run(cref, R.suv, (int)R.w / 2, (int)R.h / 2, cx, cy + CVP9_MC_RUN, cpx8 & 7, cpy8 & 7,
+               cn - CVP9_MC_RUN, cd + (size_t)CVP9_MC_RUN * out.suv, out.suv, cbw - 4 * cq);
 }
 
 /* unit u of the batch list (the fused kernel's grid-stride form) */
@@ -307,11 +358,11 @@ __device__ __forceinline__ void mc_unit_at(const DBatch &B, uint32_t u, int li)
 /* A workgroup takes CVP9_MC_WG_UNITS units of one job (job counts are
  * padded to whole workgroups): the job descriptor is read with scalar loads
  * and its reference table is staged in LDS once. */
-__global__ __launch_bounds__(256) void k_mc(DBatch B)
+__device__ __forceinline__ void mc_wg(const DBatch &B, uint32_t wg, uint32_t nwg)
 {
     static_assert(CVP9_MC_WG_UNITS * CVP9_MC_UNIT == 256, "one unit per 16 lanes");
     __shared__ DFrame s_ref[8];
-    const uint32_t u0 = xcd_block(blockIdx.x, gridDim.x) * CVP9_MC_WG_UNITS;
+    const uint32_t u0 = xcd_block(wg, nwg) * CVP9_MC_WG_UNITS;
     const uint32_t j = __builtin_amdgcn_readfirstlane(find_job_wave(B.pre_inter, B.njobs, u0));
     const DJob &J = B.jobs[j];
     constexpr uint32_t ref_words = sizeof(s_ref) / 4;
@@ -327,6 +378,19 @@ __global__ __launch_bounds__(256) void k_mc(DBatch B)
     DFrame R = {};
     if (ref < 8) R = s_ref[ref];
     mc_unit(J, e, R, threadIdx.x % CVP9_MC_UNIT);
+}
+
+__global__ __launch_bounds__(256) void k_mc(DBatch B)
+{
+    mc_wg(B, blockIdx.x, gridDim.x);
+}
+
+/* The owner map and motion compensation in one launch (they are
+ * independent): the first owner_wgs workgroups build the owner map. */
+__global__ __launch_bounds__(256) void k_front(DBatch B, uint32_t owner_wgs)
+{
+    if (blockIdx.x < owner_wgs) owner_wg(B, blockIdx.x);
+    else mc_wg(B, blockIdx.x - owner_wgs, gridDim.x - owner_wgs);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -361,6 +425,22 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
     const int t = lane / N;
     const int j = lane % N;
     const int i0 = wave * R;
+    /* the prediction this lane's pass-2 outputs add to (row j, the wave's
+     * columns), loaded now so it arrives during the transform */
+    constexpr int RD = R >= 4 ? R / 4 : 1;
+    uint32_t pred[RD];
+    const bool inter = coef && !rout;
+    if (inter) {
+        const uint8_t *row = dst + (size_t)j * stride + i0;
+        if constexpr (R >= 4) {
+#pragma unroll
+            for (int c4 = 0; c4 < RD; c4++) pred[c4] = *(const uint32_t *)(row + 4 * c4);
+        } else {
+            pred[0] = 0;
+#pragma unroll
+            for (int c = 0; c < R; c++) pred[0] |= (uint32_t)row[c] << (8 * c);
+        }
+    }
     double acc[R];
 #pragma unroll
     for (int i = 0; i < R; i++) acc[i] = 0.0;
@@ -385,7 +465,14 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
     uint64_t m = wave_ballot(col_nz);
 #pragma unroll
     for (int s = 32; s >= N; s >>= 1) m |= m >> s;
-    __syncthreads();
+    if constexpr (S == 1) {
+        /* the group's temp is this wave's own */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
 
     const int r = j;
 #pragma unroll
@@ -405,12 +492,12 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
         int16_t *o = rout + r * N + i0;
 #pragma unroll
         for (int c = 0; c < R; c++) o[c] = (int16_t)round_clamp16(acc[c]);
-    } else if (coef) {
+    } else if (inter) {
         uint8_t *row = dst + (size_t)r * stride + i0;
         if constexpr (R >= 4) {
 #pragma unroll
             for (int c4 = 0; c4 < R; c4 += 4) {
-                const uint32_t px = *(const uint32_t *)(row + c4);
+                const uint32_t px = pred[c4 / 4];
                 uint32_t o = 0;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
@@ -422,7 +509,8 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
         } else {
 #pragma unroll
             for (int c = 0; c < R; c++)
-                row[c] = (uint8_t)clampi((int)row[c] + round_clamp16(acc[c]), 0, 255);
+                row[c] = (uint8_t)clampi((int)((pred[0] >> (8 * c)) & 0xff) + round_clamp16(acc[c]),
+                                         0, 255);
         }
     }
 }
@@ -483,6 +571,26 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
 {
     __shared__ double lds[ResShape<N, S>::LDS_DOUBLES];
     residual_unit<N, S>(B, blockIdx.x, lds);
+}
+
+/* Every transform size in one launch, for batches too small to fill the
+ * device with one size at a time: workgroups [0, wg.x) take 4x4, then
+ * [wg.x, wg.y) 8x8, [wg.y, wg.z) 16x16 and the rest 32x32, each with the
+ * wave split whose registers and LDS suit a small batch (1, 1, 2, 4). The
+ * sizes write disjoint samples, so their order does not matter. */
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t RES_SMALL_LDS =
+    cmax(cmax(ResShape<4, 1>::LDS_DOUBLES, ResShape<8, 1>::LDS_DOUBLES),
+         cmax(ResShape<16, 2>::LDS_DOUBLES, ResShape<32, 4>::LDS_DOUBLES));
+
+__global__ __launch_bounds__(256) void k_res_small(DBatch B, uint3 wg)
+{
+    __shared__ double lds[RES_SMALL_LDS];
+    const uint32_t b = blockIdx.x;
+    if (b < wg.x) residual_unit<4, 1>(B, b, lds);
+    else if (b < wg.y) residual_unit<8, 1>(B, b - wg.x, lds);
+    else if (b < wg.z) residual_unit<16, 2>(B, b - wg.y, lds);
+    else residual_unit<32, 4>(B, b - wg.z, lds);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1243,7 +1351,6 @@ __device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
 
 /* transform split used inside the fused kernel, per size 4, 8, 16, 32 */
 constexpr int FS4 = 1, FS8 = 1, FS16 = 2, FS32 = 4;
-constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr size_t FUSED_LDS =
     cmax(8 * cmax(cmax(ResShape<4, FS4>::LDS_DOUBLES, ResShape<8, FS8>::LDS_DOUBLES),
                   cmax(ResShape<16, FS16>::LDS_DOUBLES, ResShape<32, FS32>::LDS_DOUBLES)),
@@ -1391,6 +1498,43 @@ static hipError_t launch_residual(const DBatch &b, const StageCounts &n, int c, 
     default: hipLaunchKernelGGL((k_residual<N, 4>), dim3(g), dim3(256), 0, s, b); break;
     }
     return hipGetLastError();
+}
+
+/* workgroups of the one-launch transform stage: per size, then the total */
+static inline uint3 res_small_starts(const StageCounts &n, uint32_t *total)
+{
+    const uint32_t a = div_up(n.grp[0], 4), b = a + div_up(n.grp[1], 4),
+                   c = b + div_up(n.grp[2], 2);
+    *total = c + n.grp[3];
+    return make_uint3(a, b, c);
+}
+
+int cvp9_stage_span(int stage, const StageCounts &n)
+{
+    if (stage == STAGE_OWNER && n.inter) return 2; /* owner map + motion compensation */
+    if (stage == STAGE_RES4) {
+        uint32_t total;
+        res_small_starts(n, &total);
+        if (total <= CVP9_RES_SMALL_MAX_WGS) return 4;
+    }
+    return 1;
+}
+
+hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageCounts &n,
+                              hipStream_t s)
+{
+    if (span == 2 && stage == STAGE_OWNER) {
+        const uint32_t ow = n.blocks ? div_up(n.blocks, 16) : 1;
+        hipLaunchKernelGGL(k_front, dim3(ow + n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, ow);
+        return hipGetLastError();
+    }
+    if (span == 4 && stage == STAGE_RES4) {
+        uint32_t total;
+        const uint3 wg = res_small_starts(n, &total);
+        if (total) hipLaunchKernelGGL(k_res_small, dim3(total), dim3(256), 0, s, b, wg);
+        return hipGetLastError();
+    }
+    return span == 1 ? cvp9_launch_stage(stage, b, n, s) : hipErrorInvalidValue;
 }
 
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s)
