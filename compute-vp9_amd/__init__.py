@@ -194,6 +194,7 @@ SIGNATURES = {
     "cvp9_engine_stream_reset": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_set_queues": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_set_pipeline": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_capture_begin": (C.c_int, [C.c_void_p]),
     "cvp9_engine_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "cvp9_engine_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -517,6 +518,9 @@ class Engine:
 
     def set_queues(self, n):
         check(lib().cvp9_engine_set_queues(self.e, n), "set_queues")
+
+    def set_pipeline(self, on):
+        check(lib().cvp9_engine_set_pipeline(self.e, int(on)), "set_pipeline")
 
     def capture_begin(self):
         check(lib().cvp9_engine_capture_begin(self.e), "capture_begin")

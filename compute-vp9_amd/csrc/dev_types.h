@@ -122,8 +122,11 @@ struct DJob {
     const DBlock   *blocks;
     const int16_t  *coeffs;
     const uint4    *mc_tab;        /* per MC unit, a copy of the block's MC fields:
-                                      x | y << 16, bw | bh << 8 | ref << 16 | unit << 24,
-                                      (uint16)mvx | (uint16)mvy << 16, block index */
+                                      x | y << 16,
+                                      bw | bh << 8 | (ref | unit << 4) << 16 |
+                                        (tx | skip << 2 | nx << 3) << 24,
+                                      (uint16)mvx | (uint16)mvy << 16,
+                                      coefficient offset of the block's first transform */
     const uint32_t *intra;
     const uint32_t *tx_tab[4];
     uint32_t       *owner;         /* ow x oh cells, local block index */
@@ -163,6 +166,11 @@ struct DBatch {
     const uint32_t *lvl_pre2;
     const uint32_t *lvl_start;
     uint32_t        spin_limit;    /* bound on every dependency wait */
+    uint32_t        resid_all;     /* 1: every transform's residual goes to the int16
+                                      residual buffer and motion compensation adds the
+                                      inter ones (the transform stage then does not depend
+                                      on the previous frame); 0: inter transforms add to
+                                      the prediction in place */
     uint32_t        fused_lanes;   /* fused kernel: threads per intra worker (64 or 256) */
     uint32_t        fused_workers; /* fused kernel: persistent intra workers */
 };

@@ -298,10 +298,12 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
             const uint32_t units = (d.bh + rows - 1) / rows;
             for (uint32_t u = 0; u < units; u++, nm++) {
                 uint32_t *m = mc_tab + 4 * (size_t)nm;
+                const uint32_t skip = (d.flags & DB_SKIP) ? 1u : 0u;
                 m[0] = (uint32_t)d.x | (uint32_t)d.y << 16;
-                m[1] = (uint32_t)d.bw | (uint32_t)d.bh << 8 | (uint32_t)d.ref << 16 | u << 24;
+                m[1] = (uint32_t)d.bw | (uint32_t)d.bh << 8 | ((uint32_t)d.ref | u << 4) << 16 |
+                       ((uint32_t)d.tx | skip << 2 | d.nx << 3) << 24;
                 m[2] = (uint32_t)(uint16_t)d.mvx | (uint32_t)(uint16_t)d.mvy << 16;
-                m[3] = n;
+                m[3] = d.coeff;
             }
         }
         if (res) { /* transforms of inter and intra blocks alike, list order */
@@ -353,18 +355,21 @@ StreamState::~StreamState()
         hipFree(f->dev);
         delete f;
     }
-    hipFree(owner_);
+    for (int p = 0; p < 2; p++) {
+        hipFree(owner_[p]);
+        hipFree(resid_[p]);
+    }
     hipFree(edge_);
     hipFree(nbr_);
-    hipFree(resid_);
     hipFree(err_);
 }
 
 bool StreamState::scratch_fits(const PrepLayout &L) const
 {
     const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
-    return err_ && cells <= owner_cap_ && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
-           (L.n_intra ? L.n_intra : 1) <= nbr_cap_ && (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_;
+    return err_ && cells <= owner_cap_[par_] && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
+           (L.n_intra ? L.n_intra : 1) <= nbr_cap_ &&
+           (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_[par_];
 }
 
 hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free)
@@ -436,14 +441,15 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
         if ((e = hipMalloc(&err_, 256)) != hipSuccess) return e;
         if ((e = hipMemsetAsync(err_, 0, 256, s)) != hipSuccess) return e;
     }
-    if (cells > owner_cap_) {
-        if (owner_) {
+    const int p = par_;
+    if (cells > owner_cap_[p]) {
+        if (owner_[p]) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            hipFree(owner_);
-            owner_ = nullptr;
+            hipFree(owner_[p]);
+            owner_[p] = nullptr;
         }
-        if ((e = hipMalloc(&owner_, cells * 4)) != hipSuccess) return e;
-        owner_cap_ = cells;
+        if ((e = hipMalloc(&owner_[p], cells * 4)) != hipSuccess) return e;
+        owner_cap_[p] = cells;
     }
     const size_t nb = L.n_blocks ? L.n_blocks : 1;
     if (nb > edge_cap_) {
@@ -474,21 +480,22 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
         nbr_cap_ = cap;
     }
     const size_t nc = L.num_coeffs ? L.num_coeffs : 1;
-    if (nc > resid_cap_) {
-        size_t cap = resid_cap_ ? resid_cap_ : 65536;
+    if (nc > resid_cap_[p]) {
+        size_t cap = resid_cap_[p] ? resid_cap_[p] : 65536;
         while (cap < nc) cap *= 2;
-        if (resid_) {
+        if (resid_[p]) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            hipFree(resid_);
-            resid_ = nullptr;
+            hipFree(resid_[p]);
+            resid_[p] = nullptr;
         }
-        if ((e = hipMalloc(&resid_, cap * sizeof(int16_t))) != hipSuccess) return e;
-        resid_cap_ = cap;
+        if ((e = hipMalloc(&resid_[p], cap * sizeof(int16_t))) != hipSuccess) return e;
+        resid_cap_[p] = cap;
     }
     return hipSuccess;
 }
 
-DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *out, uint8_t *nv12)
+DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *out, uint8_t *nv12,
+                           uint64_t serial)
 {
     DJob j;
     memset(&j, 0, sizeof(j));
@@ -504,10 +511,12 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
         j.tx_tab[k] = (const uint32_t *)(dev + L.off_tx_tab[k]);
         j.cls_ntx[k] = L.cls_ntx[k];
     }
-    j.owner = owner_;
+    j.owner = owner_[par_];
     j.edge = edge_;
     j.nbr = nbr_;
-    j.resid = resid_;
+    j.resid = resid_[par_];
+    reader_[par_] = serial;
+    if (two_sets_) par_ ^= 1;
     j.nv12 = nv12;
     j.err = err_;
     j.n_blocks = L.n_blocks;
@@ -527,8 +536,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
 
 size_t StreamState::resident_bytes() const
 {
-    size_t b = owner_cap_ * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 + nbr_cap_ * CVP9_NB_WORDS * 8 +
-               resid_cap_ * 2 + (err_ ? 256 : 0);
+    size_t b = (owner_cap_[0] + owner_cap_[1]) * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 +
+               nbr_cap_ * CVP9_NB_WORDS * 8 + (resid_cap_[0] + resid_cap_[1]) * 2 + (err_ ? 256 : 0);
     for (const FrameBuf *f : all_) b += f->cap;
     return b;
 }

@@ -67,6 +67,14 @@ public:
     hipError_t acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free = true);
     /* scratch already large enough for L (no allocation needed) */
     bool scratch_fits(const PrepLayout &L) const;
+    /* Frame pipelining: the owner map and the residual buffer come in two
+     * sets, alternated frame by frame, so the owner map and transforms of
+     * one frame can run while the previous frame still reads its set. With
+     * two_sets off every frame uses set 0. */
+    void set_two_sets(bool on) { two_sets_ = on; if (!on) par_ = 0; }
+    int parity() const { return par_; }
+    /* serial of the batch whose later stages last read set p (0: none) */
+    uint64_t reader(int p) const { return reader_[p]; }
     void release(FrameBuf *f);
     /* reference-pool rule of the CPU path: KEY -> all 8 slots, else slot 0
      * (src/decoder/vp9_decoder.c:224-231) */
@@ -78,8 +86,11 @@ public:
      * enough for L;
      * may synchronize s */
     hipError_t ensure_scratch(const PrepLayout &L, hipStream_t s);
-    /* job descriptor for one frame of this stream */
-    DJob make_job(const PrepLayout &L, const uint8_t *dev_prep, FrameBuf *out, uint8_t *nv12);
+    /* job descriptor for one frame of this stream, on the current set; the
+     * batch with serial `serial` reads that set, and the next frame takes
+     * the other one (with two sets) */
+    DJob make_job(const PrepLayout &L, const uint8_t *dev_prep, FrameBuf *out, uint8_t *nv12,
+                  uint64_t serial = 0);
     /* bytes of HBM this stream holds */
     size_t resident_bytes() const;
     uint32_t *err_word() const { return err_; }
@@ -89,16 +100,19 @@ private:
     int device_;
     std::vector<FrameBuf *> all_;
     FrameBuf *refs_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    uint32_t *owner_ = nullptr;
-    size_t owner_cap_ = 0;
+    uint32_t *owner_[2] = {nullptr, nullptr};
+    size_t owner_cap_[2] = {0, 0};
     uint64_t *edge_ = nullptr;
     size_t edge_cap_ = 0;
     uint64_t *nbr_ = nullptr;
     size_t nbr_cap_ = 0;      /* intra blocks */
-    int16_t *resid_ = nullptr;
-    size_t resid_cap_ = 0;
+    int16_t *resid_[2] = {nullptr, nullptr};
+    size_t resid_cap_[2] = {0, 0};
     uint32_t *err_ = nullptr;
     uint32_t epoch_ = 0;
+    bool two_sets_ = false;
+    int par_ = 0;
+    uint64_t reader_[2] = {0, 0};
 };
 
 /* Packs job descriptors + per-stage prefix sums for one batch into a host
@@ -113,6 +127,7 @@ public:
                uint8_t *dev, const Tune &tune, uint32_t spin_limit);
     size_t used() const { return used_; }
     const DBatch &batch() const { return b_; }
+    void set_resid_all(bool on) { b_.resid_all = on ? 1u : 0u; }
     const StageCounts &counts() const { return n_; }
     static hipError_t launch(const DBatch &b, const StageCounts &n, hipStream_t s);
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -30,7 +31,19 @@ struct BatchSlot {
     size_t cap = 0;
     hipEvent_t done = nullptr;
     bool used = false;
+    uint64_t serial = 0; /* of the batch that last used the slot */
 };
+
+/* Stage order without frame pipelining, then the two phases with it: the
+ * first depends only on the frame's own data (owner map, transforms to
+ * int16 residuals), the second on the reference frames (motion
+ * compensation with the residual add, neighbours, intra, deblocking). */
+constexpr int kOrderPlain[] = {STAGE_FILL, STAGE_OWNER, STAGE_MC, STAGE_RES4, STAGE_RES8,
+                               STAGE_RES16, STAGE_RES32, STAGE_NBR, STAGE_INTRA, STAGE_LF,
+                               STAGE_NV12};
+constexpr int kOrderA[] = {STAGE_FILL, STAGE_OWNER, STAGE_RES4, STAGE_RES8, STAGE_RES16,
+                           STAGE_RES32};
+constexpr int kOrderB[] = {STAGE_FILL, STAGE_MC, STAGE_NBR, STAGE_INTRA, STAGE_LF, STAGE_NV12};
 
 struct TimedLaunch {
     int stage;
@@ -106,6 +119,20 @@ struct cvp9_engine {
      * use queue 0 only. */
     hipStream_t stream = nullptr;  /* queue 0 */
     hipStream_t stream2 = nullptr; /* queue 1 */
+    /* Frame pipelining (default on): a batch's first phase runs on the
+     * phase queue paired with its queue, so it overlaps the previous batch's
+     * second phase; the second phase waits for it (event aev). A stream's
+     * owner map and residual buffer alternate between two sets; a first
+     * phase waits for the batch that last read its set. Capture, fused mode,
+     * per-stage timing and one queue run both phases in order on one queue. */
+    bool pipeline = true;
+    hipStream_t astream[2] = {nullptr, nullptr};
+    hipEvent_t aev[2] = {nullptr, nullptr};
+    int last_mode = -1;            /* overlap of the previous batch (0/1), -1 none */
+    uint64_t serial = 0;           /* batches run so far */
+    hipEvent_t oev[3] = {nullptr, nullptr, nullptr}; /* queue 1 / phase queues into queue 0 */
+    hipEvent_t gev = nullptr;      /* after the latest graph replay on queue 0 */
+    bool graph_pending[2] = {false, false}; /* phase queue q has not waited for it */
     int nqueues = 2;
     int last_q = 1;
     std::vector<int8_t> sq;        /* per stream id: queue of its last batch, -1 none */
@@ -129,9 +156,32 @@ struct cvp9_engine {
 
     hipError_t sync_all()
     {
-        hipError_t a = hipStreamSynchronize(stream);
-        hipError_t b = stream2 ? hipStreamSynchronize(stream2) : hipSuccess;
-        return a != hipSuccess ? a : b;
+        hipError_t r = hipSuccess;
+        for (hipStream_t s : {stream, stream2, astream[0], astream[1]}) {
+            hipError_t x = s ? hipStreamSynchronize(s) : hipSuccess;
+            if (r == hipSuccess) r = x;
+        }
+        return r;
+    }
+
+    /* queue 0 waits (on the device) for everything enqueued on the others */
+    hipError_t order_all_into_0()
+    {
+        const hipStream_t src[3] = {stream2, astream[0], astream[1]};
+        for (int i = 0; i < 3; i++) {
+            hipError_t r = hipEventRecord(oev[i], src[i]);
+            if (r == hipSuccess) r = hipStreamWaitEvent(stream, oev[i], 0);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+
+    /* done event of the batch with this serial while it may still run */
+    hipEvent_t batch_event(uint64_t s) const
+    {
+        for (const BatchSlot &b : slots)
+            if (b.used && b.serial == s) return b.done;
+        return nullptr;
     }
 
     /* queue dst waits (on the device) for everything enqueued on src so far */
@@ -193,7 +243,15 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
     }
     if (hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->xev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->xev[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->xev[1], hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->astream[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->astream[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->aev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->aev[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->oev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->oev[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->oev[2], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->gev, hipEventDisableTiming) != hipSuccess) {
         cvp9_engine_destroy(e);
         return CVP9_ERR_GPU;
     }
@@ -223,6 +281,13 @@ void cvp9_engine_destroy(cvp9_engine_t *e)
     for (hipEvent_t ev : e->free_events) hipEventDestroy(ev);
     for (hipEvent_t ev : e->xev)
         if (ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : e->aev)
+        if (ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : e->oev)
+        if (ev) hipEventDestroy(ev);
+    if (e->gev) hipEventDestroy(e->gev);
+    for (hipStream_t s : e->astream)
+        if (s) hipStreamDestroy(s);
     if (e->stream2) hipStreamDestroy(e->stream2);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -274,6 +339,13 @@ cvp9_err_t cvp9_engine_set_queues(cvp9_engine_t *e, int n)
 {
     if (!e || n < 1 || n > 2) return CVP9_ERR_INVALID_DATA;
     e->nqueues = n;
+    return CVP9_OK;
+}
+
+cvp9_err_t cvp9_engine_set_pipeline(cvp9_engine_t *e, int on)
+{
+    if (!e) return CVP9_ERR_INVALID_DATA;
+    e->pipeline = on != 0;
     return CVP9_OK;
 }
 
@@ -350,16 +422,37 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
     for (int i = 0; i < n; i++) other |= e->sq[streams[i]] >= 0 && e->sq[streams[i]] != q;
     if (other && (he = e->order(1 - q, q)) != hipSuccess) return to_err(he);
     const hipStream_t qs = e->queue(q);
-    if (e->capture) /* nothing may synchronise or memset while recording */
+    const bool resid_all = e->pipeline && !e->fused;
+    const bool overlap = resid_all && !single;
+    const hipStream_t qa = overlap ? e->astream[q] : qs; /* first phase */
+    if (e->last_mode >= 0 && e->last_mode != (int)overlap && !e->capture &&
+        (he = e->sync_all()) != hipSuccess) /* sets and queues change meaning */
+        return to_err(he);
+    e->last_mode = overlap;
+    for (int i = 0; i < n; i++) e->streams[streams[i]]->set_two_sets(overlap);
+    if (e->capture) { /* nothing may synchronise or memset while recording */
         for (int i = 0; i < n; i++)
             if (!e->streams[streams[i]]->scratch_fits(preps[i]->L)) return CVP9_ERR_UNSUPPORTED;
+    } else if (overlap) {
+        /* buffers that in-flight work reads are reallocated only when idle */
+        bool fits = true;
+        for (int i = 0; i < n; i++) fits &= e->streams[streams[i]]->scratch_fits(preps[i]->L);
+        if (!fits && (he = e->sync_all()) != hipSuccess) return to_err(he);
+    }
+    const uint64_t serial = ++e->serial;
+    std::vector<hipEvent_t> readers; /* batches still reading the sets written now */
     for (int i = 0; i < n; i++) {
         StreamState *s = e->streams[streams[i]];
         const PrepLayout &L = preps[i]->L;
-        if (!e->capture && (he = s->ensure_scratch(L, qs)) != hipSuccess) return to_err(he);
+        if (!e->capture && (he = s->ensure_scratch(L, qa)) != hipSuccess) return to_err(he);
+        if (overlap) {
+            hipEvent_t ev = e->batch_event(s->reader(s->parity()));
+            if (ev && std::find(readers.begin(), readers.end(), ev) == readers.end())
+                readers.push_back(ev);
+        }
         FrameBuf *out = nullptr;
         if ((he = s->acquire(L.W, L.H, &out, !e->capture)) != hipSuccess) return to_err(he);
-        jobs[i] = s->make_job(L, preps[i]->dev, out, nv12 ? nv12[i] : nullptr);
+        jobs[i] = s->make_job(L, preps[i]->dev, out, nv12 ? nv12[i] : nullptr, serial);
         s->update_refs(L.key != 0, out);
     }
 
@@ -398,7 +491,14 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
     Tune tune = e->tune;
     if (e->fused) tune.intra_level_min = 0; /* the fused kernel runs persistent workers only */
     bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, kSpinLimit);
-    if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, qs)) != hipSuccess)
+    bb.set_resid_all(resid_all);
+    for (hipEvent_t ev : readers)
+        if ((he = hipStreamWaitEvent(qa, ev, 0)) != hipSuccess) return to_err(he);
+    if (overlap && e->graph_pending[q]) {
+        if ((he = hipStreamWaitEvent(qa, e->gev, 0)) != hipSuccess) return to_err(he);
+        e->graph_pending[q] = false;
+    }
+    if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, qa)) != hipSuccess)
         return to_err(he);
     const bool timing = e->timing && !e->capture;
     if (e->fused) {
@@ -415,28 +515,54 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             e->timed.push_back(t);
         }
     }
-    for (int st = 0; !e->fused && st < STAGE_FUSED; st++) {
-        const bool timed = timing && stage_has_work(st, bb.counts());
-        TimedLaunch t{st, nullptr, nullptr};
-        if (timed) {
-            t.a = e->take_event();
-            t.b = e->take_event();
-            if (!t.a || !t.b) return CVP9_ERR_GPU;
-            hipEventRecord(t.a, qs);
+    /* stages in order; fill_part 1/2: the owner-map / frame part of the fill */
+    auto run_stages = [&](const int *order, int count, hipStream_t qx, int fill_part) -> hipError_t {
+        for (int i = 0; i < count; i++) {
+            const int st = order[i];
+            const bool timed = timing && stage_has_work(st, bb.counts());
+            TimedLaunch t{st, nullptr, nullptr};
+            if (timed) {
+                t.a = e->take_event();
+                t.b = e->take_event();
+                if (!t.a || !t.b) return hipErrorOutOfMemory;
+                hipEventRecord(t.a, qx);
+            }
+            hipError_t r;
+            int span = 1;
+            if (st == STAGE_FILL && fill_part) {
+                r = cvp9_launch_fill(bb.batch(), bb.counts(), fill_part, qx);
+            } else {
+                /* outside per-stage timing, independent stages share launches */
+                span = timing ? 1 : cvp9_stage_span(st, bb.batch(), bb.counts());
+                r = cvp9_launch_stages(st, span, bb.batch(), bb.counts(), qx);
+            }
+            if (r != hipSuccess) return r;
+            i += span - 1;
+            if (timed) {
+                hipEventRecord(t.b, qx);
+                e->timed.push_back(t);
+            }
         }
-        /* outside per-stage timing, independent stages share launches */
-        const int span = timing ? 1 : cvp9_stage_span(st, bb.counts());
-        if ((he = cvp9_launch_stages(st, span, bb.batch(), bb.counts(), qs)) != hipSuccess)
-            return to_err(he);
-        st += span - 1;
-        if (timed) {
-            hipEventRecord(t.b, qs);
-            e->timed.push_back(t);
+        return hipSuccess;
+    };
+    if (!e->fused) {
+        if (!resid_all) {
+            he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, 0);
+        } else {
+            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, 1);
+            if (he == hipSuccess && overlap) {
+                he = hipEventRecord(e->aev[q], qa);
+                if (he == hipSuccess) he = hipStreamWaitEvent(qs, e->aev[q], 0);
+            }
+            if (he == hipSuccess)
+                he = run_stages(kOrderB, (int)(sizeof(kOrderB) / sizeof(int)), qs, 2);
         }
+        if (he != hipSuccess) return to_err(he);
     }
     if (bs) {
         if ((he = hipEventRecord(bs->done, qs)) != hipSuccess) return to_err(he);
         bs->used = true;
+        bs->serial = serial;
     }
     for (int i = 0; i < n; i++) e->sq[streams[i]] = (int8_t)q;
     e->last_q = q;
@@ -498,9 +624,12 @@ cvp9_err_t cvp9_engine_graph_launch(cvp9_engine_t *e, cvp9_graph_t *g)
 {
     if (!e || !g || !g->exec) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
-    /* the recorded batches run on queue 0, after everything queued so far */
-    if (e->order(1, 0) != hipSuccess || hipGraphLaunch(g->exec, e->stream) != hipSuccess)
+    /* the recorded batches run on queue 0, after everything queued so far;
+     * later first phases (which write the sets the replay reads) wait for it */
+    if (e->order_all_into_0() != hipSuccess || hipGraphLaunch(g->exec, e->stream) != hipSuccess ||
+        hipEventRecord(e->gev, e->stream) != hipSuccess)
         return CVP9_ERR_GPU;
+    e->graph_pending[0] = e->graph_pending[1] = true;
     for (int8_t &q : e->sq) q = q >= 0 ? 0 : -1;
     e->last_q = 0;
     return CVP9_OK;

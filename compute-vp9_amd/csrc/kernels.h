@@ -56,9 +56,12 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
  * while they need at most CVP9_RES_SMALL_MAX_WGS workgroups; 1 otherwise.
  * cvp9_launch_stages runs stages [stage, stage + span) with that launch. */
 #define CVP9_RES_SMALL_MAX_WGS 2048
-int cvp9_stage_span(int stage, const StageCounts &n);
+int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n);
 hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageCounts &n,
                               hipStream_t s);
+/* The 128-fill of jobs with holes in parts: 1 the owner map (NONE), 2 the
+ * frame (128), 0 both (STAGE_FILL). */
+hipError_t cvp9_launch_fill(const DBatch &b, const StageCounts &n, int part, hipStream_t s);
 
 /* Fused persistent kernel: every stage of the batch in one launch of `wgs`
  * co-resident workgroups (at most cvp9_fused_max_wgs()). */

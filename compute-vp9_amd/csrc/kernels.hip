@@ -20,6 +20,30 @@
 #include "dev_types.h"
 #include "kernels.h"
 
+/* Global-memory views of the descriptor's pointers. The descriptors hold
+ * generic pointers, through which the compiler emits FLAT accesses; those
+ * count against the LDS/scalar wait counter too, so every LDS or scalar
+ * wait would also wait for all memory loads in flight. Accesses through
+ * these views are plain global loads and stores. */
+#define GMEM __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GMEM T *gm(T *p)
+{
+    return (GMEM T *)p;
+}
+
+/* a 16- or 8-byte record through a global view, as 32-bit words */
+__device__ __forceinline__ uint4 gload4(const void *p)
+{
+    const GMEM uint32_t *q = (const GMEM uint32_t *)gm(p);
+    return make_uint4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ uint2 gload2(const void *p)
+{
+    const GMEM uint32_t *q = (const GMEM uint32_t *)gm(p);
+    return make_uint2(q[0], q[1]);
+}
+
 __constant__ double c_M4[16];
 __constant__ double c_M8[64];
 __constant__ double c_M16[256];
@@ -42,7 +66,7 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *pre, uint32_t n, ui
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= w) lo = mid; else hi = mid;
+        if (gm(pre)[mid] <= w) lo = mid; else hi = mid;
     }
     return lo;
 }
@@ -54,15 +78,15 @@ __device__ __forceinline__ uint32_t find_job_wave(const uint32_t *pre, uint32_t 
 {
     const uint32_t lane = __lane_id();
     if (n <= 64) {
-        const bool le = lane < n && pre[lane] <= w;
+        const bool le = lane < n && gm(pre)[lane] <= w;
         return (uint32_t)__builtin_popcountll(__ballot(le)) - 1;
     }
     const uint32_t step = (n + 63) >> 6;
     const uint32_t i = lane * step;
-    const bool le = i < n && pre[i] <= w;
+    const bool le = i < n && gm(pre)[i] <= w;
     const uint32_t base = ((uint32_t)__builtin_popcountll(__ballot(le)) - 1) * step;
     const uint32_t k = base + lane;
-    const bool le2 = lane < step && k < n && pre[k] <= w;
+    const bool le2 = lane < step && k < n && gm(pre)[k] <= w;
     return base + (uint32_t)__builtin_popcountll(__ballot(le2)) - 1;
 }
 
@@ -85,7 +109,8 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p)
 /* 128 fill of the output frame and NONE fill of the owner map, for jobs    */
 /* whose blocks do not cover the frame (vp9_frame.c:27-30 initial value).   */
 
-__device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w)
+/* part: 0 both, 1 the owner map only, 2 the frame only */
+__device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w, int part = 0)
 {
     const uint32_t j = find_job_wave(B.pre_fill, B.njobs, w);
     const DJob &J = B.jobs[j];
@@ -97,6 +122,7 @@ __device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w)
     uint8_t *base;
     uint64_t len;
     uint4 v;
+    if ((chunk < fchunks) ? part == 1 : part == 2) return;
     if (chunk < fchunks) {
         base = J.out.y + ((uint64_t)chunk << 16);
         len = fbytes - ((uint64_t)chunk << 16);
@@ -108,13 +134,18 @@ __device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w)
         v = make_uint4(CVP9_OWNER_NONE, CVP9_OWNER_NONE, CVP9_OWNER_NONE, CVP9_OWNER_NONE);
     }
     if (len > 65536) len = 65536;
-    uint4 *p = (uint4 *)base;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(len >> 4); i += 256) p[i] = v;
+    GMEM uint32_t *p = (GMEM uint32_t *)gm(base);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(len >> 4); i += 256) {
+        p[4 * i] = v.x;
+        p[4 * i + 1] = v.y;
+        p[4 * i + 2] = v.z;
+        p[4 * i + 3] = v.w;
+    }
 }
 
-__global__ __launch_bounds__(256) void k_fill(DBatch B)
+__global__ __launch_bounds__(256) void k_fill(DBatch B, int part)
 {
-    fill_item(B, blockIdx.x);
+    fill_item(B, blockIdx.x, part);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -131,7 +162,7 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
     const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
     const uint32_t ncx = ((k.x + k.bw - 1) >> 2) - cx0 + 1;
     const uint32_t ncy = ((k.y + k.bh - 1) >> 2) - cy0 + 1;
-    uint32_t *const owner = J.owner; /* held in registers across the stores */
+    GMEM uint32_t *const owner = gm(J.owner); /* held in registers across the stores */
     const uint32_t ow = J.ow;
     for (uint32_t i = lane; i < ncx * ncy; i += 16) {
         const uint32_t r = i / ncx, c = i - r * ncx;
@@ -168,12 +199,12 @@ __global__ __launch_bounds__(256) void k_owner(DBatch B)
 __device__ __forceinline__ uint64_t mc_src(const uint8_t *__restrict__ ref, uint32_t rs, int rw,
                                            int rh, int sx, int y)
 {
-    const uint8_t *row = ref + (size_t)clampi(y, 0, rh - 1) * rs;
+    const GMEM uint8_t *row = gm(ref) + (size_t)clampi(y, 0, rh - 1) * rs;
     if (sx >= 0 && sx + 4 <= rw - 1) {
         /* rows start 64-byte aligned; the 8-byte window may pass the row end
          * by 3 bytes, inside the row pitch or the frame's tail slack */
         const uint32_t o = (uint32_t)sx & 3;
-        const uint32_t *q = (const uint32_t *)(row + (sx - (int)o));
+        const GMEM uint32_t *q = (const GMEM uint32_t *)(row + (sx - (int)o));
         return (((uint64_t)q[1] << 32) | q[0]) >> (8 * o);
     }
     uint64_t v = 0;
@@ -207,10 +238,11 @@ __device__ __forceinline__ uint32_t mc_vpass(uint32_t t, uint32_t b, uint32_t fy
 
 __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
 {
+    GMEM uint8_t *g = gm(d);
     if (n >= 4 && ((uintptr_t)d & 3) == 0) {
-        *(uint32_t *)d = v;
+        *(GMEM uint32_t *)g = v;
     } else {
-        for (int e = 0; e < n && e < 4; e++) d[e] = (uint8_t)(v >> (8 * e));
+        for (int e = 0; e < n && e < 4; e++) g[e] = (uint8_t)(v >> (8 * e));
     }
 }
 
@@ -249,7 +281,9 @@ __device__ __forceinline__ void mc_run(const uint8_t *__restrict__ ref, uint32_t
  * (m = min(n, CVP9_MC_RUN)), then the filtered rows stored. */
 struct McRun {
     uint64_t src[CVP9_MC_RUN + 1];
+    uint64_t res[CVP9_MC_RUN]; /* luma residual of the run's rows, 4 int16 each */
     int m;
+    bool add;                  /* res holds a residual to add */
 };
 
 __device__ __forceinline__ void mc_load(McRun &q, const uint8_t *__restrict__ ref, uint32_t rs,
@@ -259,6 +293,33 @@ __device__ __forceinline__ void mc_load(McRun &q, const uint8_t *__restrict__ re
 #pragma unroll
     for (int t = 0; t <= CVP9_MC_RUN; t++)
         if (ref && t <= q.m) q.src[t] = mc_src(ref, rs, rw, rh, ix, y0 + t);
+}
+
+/* The residual under a luma run (transform stage in resid_all mode): the 4
+ * int16 of each row r .. r + q.m - 1 at quad column c0 of a block whose
+ * first transform's coefficients start at coeff, transforms of 4 << (lt-2)
+ * samples, nx to a row; rows / columns past the transform grid (rrows,
+ * rcols) have none (vp9_reconstruct.c:336-363). */
+__device__ __forceinline__ void mc_res_load(McRun &q, const int16_t *resid, uint32_t coeff, int lt,
+                                            int nx, int rrows, int rcols, int r, int c0)
+{
+    const int T = 1 << lt;
+#pragma unroll
+    for (int t = 0; t < CVP9_MC_RUN; t++) {
+        q.res[t] = 0;
+        const int rr = r + t;
+        if (t < q.m && rr < rrows && c0 < rcols) {
+            const GMEM int16_t *p = gm(resid) + coeff +
+                                    (size_t)(((rr >> lt) * nx + (c0 >> lt)) << (2 * lt)) +
+                                    ((rr & (T - 1)) << lt) + (c0 & (T - 1));
+            if (((uintptr_t)p & 7) == 0) {
+                q.res[t] = *(const GMEM uint64_t *)p;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) q.res[t] |= (uint64_t)(uint16_t)p[e] << (16 * e);
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ void mc_emit(const McRun &q, bool flat, uint32_t fx, uint32_t fy,
@@ -273,6 +334,16 @@ __device__ __forceinline__ void mc_emit(const McRun &q, bool flat, uint32_t fx, 
                 const uint32_t bot = mc_hpass(q.src[t + 1], fx);
                 v = mc_vpass(top, bot, fy);
                 top = bot;
+            }
+            if (q.add) {
+                uint32_t o = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int p = (int)((v >> (8 * e)) & 0xff);
+                    const int rv = (int16_t)(uint16_t)(q.res[t] >> (16 * e));
+                    o |= (uint32_t)clampi(p + rv, 0, 255) << (8 * e);
+                }
+                v = o;
             }
             store_quad(d + (size_t)t * ds, v, nv);
         }
@@ -296,13 +367,19 @@ __device__ __forceinline__ bool mc_lane(int li, int ncol, int nrows, int &col, i
 /* One motion-compensation unit of job J on 16 lanes (li = 0..15): e is its
  * unit-table entry, R the reference frame of its block (R.y == nullptr: the
  * slot is empty or the block names none). */
-__device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R, int li)
+__device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R, int li,
+                                        const int16_t *resid)
 {
     const int bx = (int)(e.x & 0xffff), by = (int)(e.x >> 16);
     const int bw = (int)(e.y & 0xff), bh = (int)((e.y >> 8) & 0xff);
     const int mvx = (int16_t)(e.z & 0xffff), mvy = (int16_t)(e.z >> 16);
     const int rows = (int)cvp9_mc_unit_rows((uint32_t)bw);
-    const int r0 = (int)(e.y >> 24) * rows;
+    const int r0 = (int)((e.y >> 20) & 15) * rows;
+    /* the block's transforms, for the residual add (resid != nullptr) */
+    const uint32_t tf = e.y >> 24;
+    const int lt = (int)(tf & 3) + 2, rnx = (int)(tf >> 3);
+    const bool radd = resid && !(tf & 4);
+    const int rrows = radd ? (bh >> lt) << lt : 0, rcols = rnx << lt;
     const int r1 = min(r0 + rows, bh);
     const int cbx = bx >> 1, cby = by >> 1, cbw = (bw + 1) >> 1;
     const int cr0 = (r0 + 1) >> 1, cr1 = (r1 + 1) >> 1; /* chroma rows under [r0, r1) */
@@ -327,12 +404,16 @@ __device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R,
     uint8_t *cd = (cp ? out.v : out.u) + (size_t)(cby + cr0 + cra) * out.suv + cbx + 4 * cq;
     McRun lq, cqr;
     lq.m = cqr.m = 0;
+    lq.add = radd;
+    cqr.add = false; /* the reference adds no chroma residual */
     if (lon) mc_load(lq, R.y, R.sy, (int)R.w, (int)R.h, lx, ly, ln);
+    if (lon && radd) mc_res_load(lq, resid, e.w, lt, rnx, rrows, rcols, r0 + lra, 4 * lcol);
     if (con) mc_load(cqr, cref, R.suv, (int)R.w / 2, (int)R.h / 2, cx, cy, cn);
     if (lon) mc_emit(lq, !R.y, lpx8 & 7, lpy8 & 7, ld, out.sy, bw - 4 * lcol);
     if (con) mc_emit(cqr, !cref, cpx8 & 7, cpy8 & 7, cd, out.suv, cbw - 4 * cq);
-    /* rows past the first CVP9_MC_RUN of a run (none with blocks of at most
-     * 64x64, whose runs are at most four rows) */
+    /* rows past the first CVP9_MC_RUN of a run: none with blocks of at most
+     * 64x64, whose runs are at most four rows (and the residual add relies
+     * on that) */
     if (lon && ln > CVP9_MC_RUN)
         mc_run(R.y, R.sy, (int)R.w, (int)R.h, lx, ly + CVP9_MC_RUN, lpx8 & 7, lpy8 & 7,
                ln - CVP9_MC_RUN, ld + (size_t)CVP9_MC_RUN * out.sy, out.sy, bw - 4 * lcol);
@@ -348,11 +429,11 @@ __device__ __forceinline__ void mc_unit_at(const DBatch &B, uint32_t u, int li)
     const DJob &J = B.jobs[j];
     const uint32_t lu = u - B.pre_inter[j];
     if (lu >= J.n_mc) return; /* padding of the job's count */
-    const uint4 e = J.mc_tab[lu];
-    const uint32_t ref = (e.y >> 16) & 0xff;
+    const uint4 e = gload4(J.mc_tab + lu);
+    const uint32_t ref = (e.y >> 16) & 15;
     DFrame R = {};
     if (ref < 8) R = J.ref[ref];
-    mc_unit(J, e, R, li);
+    mc_unit(J, e, R, li, B.resid_all ? J.resid : nullptr);
 }
 
 /* A workgroup takes CVP9_MC_WG_UNITS units of one job (job counts are
@@ -371,13 +452,13 @@ __device__ __forceinline__ void mc_wg(const DBatch &B, uint32_t wg, uint32_t nwg
     const uint32_t lu = u0 - B.pre_inter[j] + threadIdx.x / CVP9_MC_UNIT;
     const bool live = lu < J.n_mc;
     uint4 e = make_uint4(0, 0, 0, 0);
-    if (live) e = J.mc_tab[lu];
+    if (live) e = gload4(J.mc_tab + lu);
     __syncthreads();
     if (!live) return;
-    const uint32_t ref = (e.y >> 16) & 0xff;
+    const uint32_t ref = (e.y >> 16) & 15;
     DFrame R = {};
     if (ref < 8) R = s_ref[ref];
-    mc_unit(J, e, R, threadIdx.x % CVP9_MC_UNIT);
+    mc_unit(J, e, R, threadIdx.x % CVP9_MC_UNIT, B.resid_all ? J.resid : nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_mc(DBatch B)
@@ -431,10 +512,10 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
     uint32_t pred[RD];
     const bool inter = coef && !rout;
     if (inter) {
-        const uint8_t *row = dst + (size_t)j * stride + i0;
+        const GMEM uint8_t *row = gm(dst) + (size_t)j * stride + i0;
         if constexpr (R >= 4) {
 #pragma unroll
-            for (int c4 = 0; c4 < RD; c4++) pred[c4] = *(const uint32_t *)(row + 4 * c4);
+            for (int c4 = 0; c4 < RD; c4++) pred[c4] = *(const GMEM uint32_t *)(row + 4 * c4);
         } else {
             pred[0] = 0;
 #pragma unroll
@@ -448,7 +529,7 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
 
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        const int x = coef ? (int)coef[k * N + j] : 0;
+        const int x = coef ? (int)gm(coef)[k * N + j] : 0;
         col_nz |= (x != 0);
         if (wave_ballot(x != 0)) {
             const double xd = (double)x;
@@ -489,11 +570,11 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
 
     if (coef && rout) {
         /* intra transform: keep the int16 residual for the intra stage */
-        int16_t *o = rout + r * N + i0;
+        GMEM int16_t *o = gm(rout) + r * N + i0;
 #pragma unroll
         for (int c = 0; c < R; c++) o[c] = (int16_t)round_clamp16(acc[c]);
     } else if (inter) {
-        uint8_t *row = dst + (size_t)r * stride + i0;
+        GMEM uint8_t *row = gm(dst) + (size_t)r * stride + i0;
         if constexpr (R >= 4) {
 #pragma unroll
             for (int c4 = 0; c4 < R; c4 += 4) {
@@ -504,7 +585,7 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
                     const int p = (int)((px >> (8 * e)) & 0xff);
                     o |= (uint32_t)clampi(p + round_clamp16(acc[c4 + e]), 0, 255) << (8 * e);
                 }
-                *(uint32_t *)(row + c4) = o;
+                *(GMEM uint32_t *)(row + c4) = o;
             }
         } else {
 #pragma unroll
@@ -518,16 +599,16 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
 /* lane's transform in a group of a job's size class c: coefficient pointer
  * and destination origin, or coef = nullptr past the end */
 template <int N>
-__device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi,
+__device__ __forceinline__ void locate_tx(const DJob &J, int cls, uint32_t txi, bool resid_all,
                                           const int16_t *&coef, uint8_t *&dst, int16_t *&rout)
 {
     coef = nullptr;
     dst = nullptr;
     rout = nullptr;
     if (txi >= J.cls_ntx[cls]) return;
-    const uint2 t = ((const uint2 *)J.tx_tab[cls])[txi];
+    const uint2 t = gload2((const uint2 *)J.tx_tab[cls] + txi);
     coef = J.coeffs + t.x;
-    if (t.y >> 31)
+    if ((t.y >> 31) || resid_all)
         rout = J.resid + t.x;
     else
         dst = J.out.y + (size_t)((t.y >> 14) & 0x3fff) * J.out.sy + (t.y & 0x3fff);
@@ -561,7 +642,7 @@ __device__ __forceinline__ void residual_unit(const DBatch &B, uint32_t unit, do
         const uint32_t j = find_job_wave(pre, B.njobs, grp); /* grp is wave-uniform */
         const DJob &J = B.jobs[j];
         stride = J.out.sy;
-        locate_tx<N>(J, C, (grp - pre[j]) * G + lane / N, coef, dst, rout);
+        locate_tx<N>(J, C, (grp - pre[j]) * G + lane / N, B.resid_all != 0, coef, dst, rout);
     }
     idct_group<N, S>(coef, dst, stride, rout, lds + gi * (G * N * (N + 1)), sub, lane);
 }
@@ -642,11 +723,12 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
     const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
-    uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+    GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
     /* fields held in registers across the record stores */
     const DFrame F = J.out;
-    const uint32_t *const owner = J.owner;
+    const GMEM uint32_t *const owner = gm(J.owner);
     const DBlock *const blocks = J.blocks;
+    const GMEM uint8_t *const fy = gm(F.y), *const fu = gm(F.u), *const fv = gm(F.v);
     const uint32_t ow = J.ow;
     uint64_t mask = 0;
     for (int c = (int)li; c <= ncx + ncy; c += 32) {
@@ -665,27 +747,32 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
             if (o == CVP9_OWNER_NONE || o >= b) {
                 val = 0x8080808080808080ull; /* still the initial 128 */
             } else {
-                const DBlock n = blocks[o];
+                DBlock n;
+                {
+                    const GMEM uint64_t *q = (const GMEM uint64_t *)gm(blocks + o);
+                    const uint64_t w[3] = {q[0], q[1], q[2]};
+                    memcpy(&n, w, sizeof(n));
+                }
                 if (!(n.flags & DB_INTRA)) {
                     if (kind == 0) {
                         /* rows are 64-byte aligned and x + 4q < W: the dword and
                          * the chroma pair stay inside their rows */
-                        val = *(const uint32_t *)(F.y + (size_t)(y - 1) * F.sy + x + 4 * q);
+                        val = *(const GMEM uint32_t *)(fy + (size_t)(y - 1) * F.sy + x + 4 * q);
                         const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
-                        val |= (uint64_t)*(const uint16_t *)(F.u + co) << 32;
-                        val |= (uint64_t)*(const uint16_t *)(F.v + co) << 48;
+                        val |= (uint64_t)*(const GMEM uint16_t *)(fu + co) << 32;
+                        val |= (uint64_t)*(const GMEM uint16_t *)(fv + co) << 48;
                     } else if (kind == 1) {
                         for (int e = 0; e < 4 && 4 * q + e < bh; e++)
-                            val |= (uint64_t)F.y[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
+                            val |= (uint64_t)fy[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
                         for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
                             const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
-                            val |= (uint64_t)F.u[co] << (32 + 8 * e);
-                            val |= (uint64_t)F.v[co] << (48 + 8 * e);
+                            val |= (uint64_t)fu[co] << (32 + 8 * e);
+                            val |= (uint64_t)fv[co] << (48 + 8 * e);
                         }
                     } else {
                         const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
-                        val = (uint64_t)F.y[(size_t)(y - 1) * F.sy + x - 1] << 24 |
-                              (uint64_t)F.u[co] << 40 | (uint64_t)F.v[co] << 56;
+                        val = (uint64_t)fy[(size_t)(y - 1) * F.sy + x - 1] << 24 |
+                              (uint64_t)fu[co] << 40 | (uint64_t)fv[co] << 56;
                     }
                 } else {
                     const int ox = n.x, oy = n.y;
@@ -718,7 +805,7 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     if (li == 0) rec[CVP9_NB_MASK] = mask;
     if (li == 1) rec[CVP9_NB_INDEX] = b;
     if (li >= 2 && li < 2 + CVP9_NB_WORDS - CVP9_NB_BLOCK)
-        rec[CVP9_NB_BLOCK + li - 2] = ((const uint64_t *)(blocks + b))[li - 2];
+        rec[CVP9_NB_BLOCK + li - 2] = ((const GMEM uint64_t *)gm(blocks + b))[li - 2];
 }
 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
@@ -743,10 +830,10 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
     const uint32_t o = (uint32_t)d;
     const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
     const uint32_t c = (uint32_t)(d >> 40) & 31;
-    const uint64_t *rec = edge + (size_t)o * CVP9_EDGE_WORDS;
-    const uint64_t *py = rec + (side ? 16 : 0) + lw;
-    const uint64_t *pu = rec + (side ? 40 : 32) + (c >> 2);
-    const uint64_t *pv = rec + (side ? 56 : 48) + (c >> 2);
+    const GMEM uint64_t *rec = gm(edge) + (size_t)o * CVP9_EDGE_WORDS;
+    const GMEM uint64_t *py = rec + (side ? 16 : 0) + lw;
+    const GMEM uint64_t *pu = rec + (side ? 40 : 32) + (c >> 2);
+    const GMEM uint64_t *pv = rec + (side ? 56 : 48) + (c >> 2);
     uint64_t wy = __hip_atomic_load(py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint64_t wu = __hip_atomic_load(pu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint64_t wv = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -781,7 +868,7 @@ __device__ __forceinline__ uint64_t edge_try(const uint64_t *edge, uint32_t epoc
     const uint32_t o = (uint32_t)d;
     const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
     const uint32_t c = (uint32_t)(d >> 40) & 31;
-    const uint64_t *rec = edge + (size_t)o * CVP9_EDGE_WORDS;
+    const GMEM uint64_t *rec = gm(edge) + (size_t)o * CVP9_EDGE_WORDS;
     const uint64_t wy = __hip_atomic_load(rec + (side ? 16 : 0) + lw, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t wu = __hip_atomic_load(rec + (side ? 40 : 32) + (c >> 2), __ATOMIC_RELAXED,
@@ -843,13 +930,13 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
      * compiler would otherwise reload them (a store could alias the
      * descriptor as far as it knows), one memory round trip each */
     const DFrame out = J.out;
-    int16_t *const jres = J.resid;
+    const GMEM int16_t *const jres = gm(J.resid);
     uint64_t *const jedge = J.edge;
     uint32_t *const jerr = J.err;
     const uint32_t epoch = J.epoch;
     /* block index, block record, dependency mask and this thread's cell
      * all come from the neighbour record, in one round of loads */
-    const uint64_t *rec = J.nbr + (size_t)tk * CVP9_NB_WORDS;
+    const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
     const uint64_t cell = tid < CVP9_NB_CELLS ? rec[tid] : 0;
     const uint64_t dmask = rec[CVP9_NB_MASK];
     const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
@@ -870,14 +957,14 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const bool res = !(k.flags & DB_SKIP);
     const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
     const int nq = (bw + 3) >> 2;
-    auto resid_quad = [&](int i) -> const int16_t * {
+    auto resid_quad = [&](int i) -> const GMEM int16_t * {
         const int r = i / nq, c0 = (i - r * nq) * 4;
         if (r >= rrows || c0 >= rcols) return nullptr;
         return jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
                ((r & (T - 1)) << lt) + (c0 & (T - 1));
     };
     int16_t rq0[4] = {0, 0, 0, 0};
-    const int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
+    const GMEM int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
     if (rs0) {
 #pragma unroll
         for (int e = 0; e < 4; e++) rq0[e] = rs0[e];
@@ -984,7 +1071,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     #pragma unroll
                 for (int e = 0; e < 4; e++)
                     if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
-                __hip_atomic_store(jedge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -1007,7 +1094,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             for (int u = 0; u < 4; u++) {
                 const int i = i0 + u * W;
                 const bool first = i == tid; /* loaded before the gather */
-                const int16_t *rs = first ? rs0 : (i < nitems ? resid_quad(i) : nullptr);
+                const GMEM int16_t *rs = first ? rs0 : (i < nitems ? resid_quad(i) : nullptr);
                 has[u] = rs != nullptr;
 #pragma unroll
                 for (int e = 0; e < 4; e++) rq[u][e] = first ? rq0[e] : (rs ? rs[e] : (int16_t)0);
@@ -1074,7 +1161,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     #pragma unroll
                 for (int e = 0; e < 4; e++)
                     if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
-                __hip_atomic_store(jedge + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
+                __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -1231,7 +1318,7 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
     const int a0 = c0 & ~3, nd = (c1 - a0 + 3) >> 2;
     for (int i = tid; i < nr * nd; i += 256) {
         const int r = i / nd, p = a0 + 4 * (i - r * nd);
-        const uint32_t v = *(const uint32_t *)(Y + (size_t)(r0 + r) * sy + p);
+        const uint32_t v = *(const GMEM uint32_t *)(gm(Y) + (size_t)(r0 + r) * sy + p);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int c = p + e - c0;
@@ -1271,9 +1358,9 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
                 mask |= 1u << e;
             }
         }
-        uint8_t *d = Y + (size_t)(r0 + r) * sy + p;
+        GMEM uint8_t *d = gm(Y) + (size_t)(r0 + r) * sy + p;
         if (mask == 15) {
-            *(uint32_t *)d = v;
+            *(GMEM uint32_t *)d = v;
         } else {
 #pragma unroll
             for (int e = 0; e < 4; e++)
@@ -1509,9 +1596,11 @@ static inline uint3 res_small_starts(const StageCounts &n, uint32_t *total)
     return make_uint3(a, b, c);
 }
 
-int cvp9_stage_span(int stage, const StageCounts &n)
+int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n)
 {
-    if (stage == STAGE_OWNER && n.inter) return 2; /* owner map + motion compensation */
+    /* owner map + motion compensation (with resid_all, motion compensation
+     * follows the transforms instead) */
+    if (stage == STAGE_OWNER && n.inter && !b.resid_all) return 2;
     if (stage == STAGE_RES4) {
         uint32_t total;
         res_small_starts(n, &total);
@@ -1537,11 +1626,17 @@ hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageC
     return span == 1 ? cvp9_launch_stage(stage, b, n, s) : hipErrorInvalidValue;
 }
 
+hipError_t cvp9_launch_fill(const DBatch &b, const StageCounts &n, int part, hipStream_t s)
+{
+    if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b, part);
+    return hipGetLastError();
+}
+
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s)
 {
     switch (stage) {
     case STAGE_FILL:
-        if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b);
+        if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b, 0);
         break;
     case STAGE_OWNER:
         hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 16) : 1), dim3(256), 0, s, b);

@@ -43,6 +43,16 @@ void      *cvp9_engine_hip_stream(cvp9_engine_t *e);
  * streams) overlap on the GPU, and each stream's frames stay in order.
  * Graph capture, fused mode and per-stage timing use one queue. */
 cvp9_err_t cvp9_engine_set_queues(cvp9_engine_t *e, int n);
+/* Frame pipelining (default on). A frame's owner map and transforms do not
+ * depend on the reference frames; with pipelining they run first, writing
+ * every transform's residual as int16, on a queue paired with the batch's
+ * queue, while the previous batch's motion compensation, intra and
+ * deblocking still run; motion compensation then adds the inter residuals.
+ * Each stream keeps two sets of owner map and residual buffers. Off: the
+ * stages run in the order of the reference's frame loop on one queue.
+ * Graph capture, fused mode and per-stage timing never overlap batches.
+ * Output never depends on it. */
+cvp9_err_t cvp9_engine_set_pipeline(cvp9_engine_t *e, int on);
 /* HBM held by a stream (frame pool, scratch). */
 uint64_t   cvp9_engine_stream_bytes(cvp9_engine_t *e, int stream_id);
 

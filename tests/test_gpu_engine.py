@@ -77,7 +77,8 @@ def test_autotune_installs_a_valid_shape(pkg, gpu_ready):
         assert eng.get_tune().as_dict() == t.as_dict()
 
 
-def test_two_queues_alternating_halves(pkg, orc, gpu_ready):
+@pytest.mark.parametrize("pipeline", [True, False])
+def test_two_queues_alternating_halves(pkg, orc, gpu_ready, pipeline):
     """Batches of two disjoint stream halves, alternated, run on the engine's
     two queues; a later batch holding streams of both halves waits for the
     other queue. Every stream's frames equal the stream decoded alone."""
@@ -86,9 +87,11 @@ def test_two_queues_alternating_halves(pkg, orc, gpu_ready):
     seqs = [[pkg.SynthFrame(pkg.synth_params(1 + (s & 1), w, h, 300 + s), i) for i in range(n)]
             for s, (w, h) in enumerate(sizes)]
     want = [orc.run_oracle(q) for q in seqs]
-    print("[case] two queues, alternating halves then a mixed batch", flush=True)
+    print(f"[case] two queues, alternating halves then a mixed batch, pipeline {pipeline}",
+          flush=True)
     with pkg.Engine(0) as eng:
         eng.set_queues(2)
+        eng.set_pipeline(pipeline)
         sids = [eng.open_stream() for _ in sizes]
         preps = [[eng.prepare(f) for f in q] for q in seqs]
         halves = [[0, 2], [1, 3]]

@@ -9,7 +9,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_server_mix_equals_alone(pkg, orc, gpu_ready):
+@pytest.mark.parametrize("pipeline", [True, False])
+def test_server_mix_equals_alone(pkg, orc, gpu_ready, pipeline):
     rng = np.random.default_rng(7)
     sizes = [(64, 64), (136, 72), (200, 120), (96, 64), (136, 72), (64, 64), (200, 120), (96, 64)]
     lengths = [5, 7, 4, 6, 3, 8, 5, 6]
@@ -20,8 +21,9 @@ def test_server_mix_equals_alone(pkg, orc, gpu_ready):
         seqs.append(fr)
         want.append([orc.nv12_pack(img, w, h) for img in orc.run_oracle(fr)])
     budget = 3 * max(pkg.stream_cost(w, h) for w, h in sizes)
-    print("[case] server mix, budget for 3 streams", flush=True)
+    print(f"[case] server mix, budget for 3 streams, pipeline {pipeline}", flush=True)
     with pkg.Engine(0) as eng:
+        eng.set_pipeline(pipeline)
         preps = [[eng.prepare(f) for f in fr] for fr in seqs]
         with pkg.Server(eng, budget, max_batch=64, max_inflight=2) as srv:
             handle = {}
