@@ -136,6 +136,8 @@ struct DJob {
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
     uint32_t       *err;           /* error word (spin timeout) */
     uint32_t n_blocks, n_inter, n_intra, n_mc;
+    uint32_t n_mc_a;               /* motion-compensation units [0, n_mc_a) run in the
+                                      early launch, the rest in the late one */
     uint32_t cls_ntx[4];
     uint32_t ow, oh;
     uint32_t epoch;
@@ -149,7 +151,9 @@ struct DBatch {
     const DJob     *jobs;
     uint32_t        njobs;
     const uint32_t *pre_blocks;    /* owner-map scatter: one wave per block */
-    const uint32_t *pre_inter;     /* motion compensation: one wave / strip */
+    const uint32_t *pre_inter;     /* motion compensation, early units (per job padded to
+                                      whole workgroups) */
+    const uint32_t *pre_inter2;    /* motion compensation, late units */
     const uint32_t *pre_grp[4];    /* residual: groups of 64/N transforms */
     const uint32_t *pre_intra;     /* intra blocks per job */
     const uint32_t *pre_lf;        /* deblocking windows */

@@ -154,7 +154,9 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
         else {
             L.n_inter++;
             const uint32_t rows = cvp9_mc_unit_rows((uint32_t)c.bw);
-            L.n_mc += ((uint32_t)c.bh + rows - 1) / rows;
+            const uint32_t units = ((uint32_t)c.bh + rows - 1) / rows;
+            L.n_mc += units;
+            if (b.ref_frame[0] != 1) L.n_mc_not0 += units; /* slot ref_frame - 1 */
         }
     }
     L.fill = covered != (uint64_t)ow * oh;
@@ -253,7 +255,8 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
     const int W = (int)L.W, H = (int)L.H;
     DBlock *blocks = (DBlock *)(dst + L.off_blocks);
     uint32_t *mc_tab = (uint32_t *)(dst + L.off_mc_tab); /* 4 words per unit */
-    uint32_t nm = 0;
+    /* units of blocks predicted from slot 0 go after all others */
+    uint32_t nm_not0 = 0, nm_0 = L.n_mc_not0;
     uint32_t *intra = (uint32_t *)(dst + L.off_intra);
     uint32_t *txt[4], cn[4] = {0, 0, 0, 0};
     for (int k = 0; k < 4; k++) txt[k] = (uint32_t *)(dst + L.off_tx_tab[k]);
@@ -296,6 +299,7 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         else {
             const uint32_t rows = cvp9_mc_unit_rows(d.bw);
             const uint32_t units = (d.bh + rows - 1) / rows;
+            uint32_t &nm = d.ref == 0 ? nm_0 : nm_not0;
             for (uint32_t u = 0; u < units; u++, nm++) {
                 uint32_t *m = mc_tab + 4 * (size_t)nm;
                 const uint32_t skip = (d.flags & DB_SKIP) ? 1u : 0u;
@@ -372,17 +376,20 @@ bool StreamState::scratch_fits(const PrepLayout &L) const
            (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_[par_];
 }
 
-hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free)
+hipError_t StreamState::acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free,
+                                bool avoid_prev)
 {
     uint32_t sy, suv;
     const size_t need = frame_bytes(w, h, &sy, &suv);
     FrameBuf *pick = nullptr;
     for (FrameBuf *f : all_)
-        if (f->refcnt == 0 && f->cap >= need && (!pick || f->cap < pick->cap)) pick = f;
+        if (f->refcnt == 0 && f->cap >= need && (!pick || f->cap < pick->cap) &&
+            !(avoid_prev && read_by_prev(f)))
+            pick = f;
     if (!pick) {
         /* drop idle buffers of other sizes before allocating */
         for (size_t i = 0; may_free && i < all_.size();) {
-            if (all_[i]->refcnt == 0) {
+            if (all_[i]->refcnt == 0 && !(avoid_prev && read_by_prev(all_[i]))) {
                 hipFree(all_[i]->dev);
                 delete all_[i];
                 all_[i] = all_.back();
@@ -415,6 +422,7 @@ void StreamState::release(FrameBuf *f)
 
 void StreamState::update_refs(bool key, FrameBuf *out)
 {
+    last_written_ = key ? 1 : 0;
     for (int i = 0; i < 8; i++) {
         if (!key && i != 0) continue;
         FrameBuf *old = refs_[i];
@@ -430,6 +438,7 @@ hipError_t StreamState::reset(hipStream_t s)
         release(refs_[i]);
         refs_[i] = nullptr;
     }
+    last_written_ = 1; /* a frame still in flight may read any slot's old frame */
     return err_ ? hipMemsetAsync(err_, 0, 256, s) : hipSuccess;
 }
 
@@ -502,6 +511,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.out = out->dframe();
     for (int i = 0; i < 8; i++) {
         if (refs_[i]) j.ref[i] = refs_[i]->dframe();
+        prev_read_[i] = refs_[i];
     }
     j.blocks = (const DBlock *)(dev + L.off_blocks);
     j.coeffs = (const int16_t *)(dev + L.off_coeffs);
@@ -522,6 +532,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.n_blocks = L.n_blocks;
     j.n_inter = L.n_inter;
     j.n_mc = L.n_mc;
+    j.n_mc_a = L.n_mc; /* all early unless the engine splits them */
     j.n_intra = L.n_intra;
     j.ow = (L.W + 3) / 4;
     j.oh = (L.H + 3) / 4;
@@ -557,7 +568,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
     uint32_t *pre = (uint32_t *)(host + pre_off);
     const uint32_t stride = njobs + 1;
-    enum { P_FILL, P_BLK, P_INTER, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
+    enum { P_FILL, P_BLK, P_INTER, P_INTER2, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
+    static_assert(P_N + 1 <= 12, "prefix rows and the ticket row fit the descriptor");
     for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
 
     for (uint32_t i = 0; i < njobs; i++) {
@@ -571,7 +583,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
         }
         c[P_FILL] = fill;
         c[P_BLK] = J.n_blocks;
-        c[P_INTER] = (J.n_mc + CVP9_MC_WG_UNITS - 1) / CVP9_MC_WG_UNITS * CVP9_MC_WG_UNITS;
+        c[P_INTER] = (J.n_mc_a + CVP9_MC_WG_UNITS - 1) / CVP9_MC_WG_UNITS * CVP9_MC_WG_UNITS;
+        c[P_INTER2] = (J.n_mc - J.n_mc_a + CVP9_MC_WG_UNITS - 1) / CVP9_MC_WG_UNITS * CVP9_MC_WG_UNITS;
         for (int k = 0; k < 4; k++) c[P_G0 + k] = div_up(J.cls_ntx[k], 64u >> (2 + k));
         c[P_INTRA] = J.n_intra;
         c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;
@@ -584,6 +597,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     b_.pre_fill = dpre + P_FILL * stride;
     b_.pre_blocks = dpre + P_BLK * stride;
     b_.pre_inter = dpre + P_INTER * stride;
+    b_.pre_inter2 = dpre + P_INTER2 * stride;
     for (int k = 0; k < 4; k++) b_.pre_grp[k] = dpre + (P_G0 + k) * stride;
     b_.pre_intra = dpre + P_INTRA * stride;
     b_.pre_lf = dpre + P_LF * stride;
@@ -598,6 +612,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     n_.fill = pre[P_FILL * stride + njobs];
     n_.blocks = pre[P_BLK * stride + njobs];
     n_.inter = pre[P_INTER * stride + njobs];
+    n_.inter2 = pre[P_INTER2 * stride + njobs];
     for (int k = 0; k < 4; k++) n_.grp[k] = pre[(P_G0 + k) * stride + njobs];
     n_.intra = pre[P_INTRA * stride + njobs];
     n_.lf = pre[P_LF * stride + njobs];

@@ -28,6 +28,9 @@ struct PrepLayout {
     uint32_t cls_nblk[4] = {0, 0, 0, 0}, cls_ntx[4] = {0, 0, 0, 0};
     uint32_t num_coeffs = 0;
     uint32_t n_mc = 0;              /* motion-compensation units (16 lanes each) */
+    uint32_t n_mc_not0 = 0;         /* units of blocks not predicted from slot 0: first in
+                                       the unit table, the ones the previous frame (unless
+                                       KEY) leaves alone */
     uint64_t off_blocks = 0, off_intra = 0, off_mc_tab = 0;
     uint64_t off_tx_tab[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
@@ -63,8 +66,20 @@ public:
     StreamState(const StreamState &) = delete;
     StreamState &operator=(const StreamState &) = delete;
 
-    /* frame for the next output; hipSuccess or an allocation error */
-    hipError_t acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free = true);
+    /* frame for the next output; hipSuccess or an allocation error. With
+     * avoid_prev, never a frame the previous frame reads as a reference
+     * (that frame's later stages may still run while this one's first
+     * phase writes its output) */
+    hipError_t acquire(uint32_t w, uint32_t h, FrameBuf **out, bool may_free = true,
+                       bool avoid_prev = false);
+    /* motion-compensation units of a frame with layout L that do not read a
+     * slot the previous frame writes: all without a previous frame, none
+     * after a KEY frame (it writes every slot), else those not predicted
+     * from slot 0 */
+    uint32_t mc_independent(const PrepLayout &L) const
+    {
+        return last_written_ < 0 ? L.n_mc : (last_written_ == 1 ? 0 : L.n_mc_not0);
+    }
     /* scratch already large enough for L (no allocation needed) */
     bool scratch_fits(const PrepLayout &L) const;
     /* Frame pipelining: the owner map and the residual buffer come in two
@@ -113,6 +128,15 @@ private:
     bool two_sets_ = false;
     int par_ = 0;
     uint64_t reader_[2] = {0, 0};
+    FrameBuf *prev_read_[8] = {nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, nullptr}; /* refs of the last frame */
+    int last_written_ = -1; /* slots the last frame wrote: -1 none yet, 0 slot 0, 1 all */
+    bool read_by_prev(const FrameBuf *f) const
+    {
+        for (const FrameBuf *p : prev_read_)
+            if (p == f) return true;
+        return false;
+    }
 };
 
 /* Packs job descriptors + per-stage prefix sums for one batch into a host

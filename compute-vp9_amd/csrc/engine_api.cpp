@@ -35,15 +35,17 @@ struct BatchSlot {
 };
 
 /* Stage order without frame pipelining, then the two phases with it: the
- * first depends only on the frame's own data (owner map, transforms to
- * int16 residuals), the second on the reference frames (motion
- * compensation with the residual add, neighbours, intra, deblocking). */
+ * first depends only on the frame's own data and on reference slots the
+ * previous frame does not write (fill, owner map, transforms to int16
+ * residuals, motion compensation of the early units), the second on the
+ * previous frame (motion compensation of the late units, with the residual
+ * add like the early ones, neighbours, intra, deblocking). */
 constexpr int kOrderPlain[] = {STAGE_FILL, STAGE_OWNER, STAGE_MC, STAGE_RES4, STAGE_RES8,
                                STAGE_RES16, STAGE_RES32, STAGE_NBR, STAGE_INTRA, STAGE_LF,
                                STAGE_NV12};
 constexpr int kOrderA[] = {STAGE_FILL, STAGE_OWNER, STAGE_RES4, STAGE_RES8, STAGE_RES16,
-                           STAGE_RES32};
-constexpr int kOrderB[] = {STAGE_FILL, STAGE_MC, STAGE_NBR, STAGE_INTRA, STAGE_LF, STAGE_NV12};
+                           STAGE_RES32, STAGE_MC};
+constexpr int kOrderB[] = {STAGE_MC, STAGE_NBR, STAGE_INTRA, STAGE_LF, STAGE_NV12};
 
 struct TimedLaunch {
     int stage;
@@ -451,8 +453,10 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
                 readers.push_back(ev);
         }
         FrameBuf *out = nullptr;
-        if ((he = s->acquire(L.W, L.H, &out, !e->capture)) != hipSuccess) return to_err(he);
+        if ((he = s->acquire(L.W, L.H, &out, !e->capture, overlap)) != hipSuccess) return to_err(he);
+        const uint32_t early = overlap ? s->mc_independent(L) : L.n_mc;
         jobs[i] = s->make_job(L, preps[i]->dev, out, nv12 ? nv12[i] : nullptr, serial);
+        jobs[i].n_mc_a = early;
         s->update_refs(L.key != 0, out);
     }
 
@@ -515,11 +519,14 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             e->timed.push_back(t);
         }
     }
-    /* stages in order; fill_part 1/2: the owner-map / frame part of the fill */
-    auto run_stages = [&](const int *order, int count, hipStream_t qx, int fill_part) -> hipError_t {
+    /* stages in order; late: the second phase (its motion compensation takes
+     * the late units) */
+    auto run_stages = [&](const int *order, int count, hipStream_t qx, bool late) -> hipError_t {
         for (int i = 0; i < count; i++) {
             const int st = order[i];
-            const bool timed = timing && stage_has_work(st, bb.counts());
+            const bool timed =
+                timing && (late && st == STAGE_MC ? bb.counts().inter2 != 0
+                                                  : stage_has_work(st, bb.counts()));
             TimedLaunch t{st, nullptr, nullptr};
             if (timed) {
                 t.a = e->take_event();
@@ -529,8 +536,8 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
             }
             hipError_t r;
             int span = 1;
-            if (st == STAGE_FILL && fill_part) {
-                r = cvp9_launch_fill(bb.batch(), bb.counts(), fill_part, qx);
+            if (late && st == STAGE_MC) {
+                r = cvp9_launch_mc_late(bb.batch(), bb.counts(), qx);
             } else {
                 /* outside per-stage timing, independent stages share launches */
                 span = timing ? 1 : cvp9_stage_span(st, bb.batch(), bb.counts());
@@ -547,15 +554,15 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
     };
     if (!e->fused) {
         if (!resid_all) {
-            he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, 0);
+            he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, false);
         } else {
-            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, 1);
+            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, false);
             if (he == hipSuccess && overlap) {
                 he = hipEventRecord(e->aev[q], qa);
                 if (he == hipSuccess) he = hipStreamWaitEvent(qs, e->aev[q], 0);
             }
             if (he == hipSuccess)
-                he = run_stages(kOrderB, (int)(sizeof(kOrderB) / sizeof(int)), qs, 2);
+                he = run_stages(kOrderB, (int)(sizeof(kOrderB) / sizeof(int)), qs, true);
         }
         if (he != hipSuccess) return to_err(he);
     }

@@ -35,7 +35,8 @@ struct Tune {
 /* Total work items of each stage over the whole batch (the last element of
  * each prefix array in DBatch), plus launch parameters. */
 struct StageCounts {
-    uint32_t fill, blocks, inter, grp[4], intra, lf, nv12;
+    uint32_t fill, blocks, inter, inter2, grp[4], intra, lf, nv12; /* inter: early motion-
+                                      compensation units, inter2: late ones (padded) */
     uint32_t levels;          /* leading levels launched one by one */
     const uint32_t *level_n;  /* host arrays, per leading level over the batch: large and */
     const uint32_t *level_ns; /* small (at most 16x16) intra blocks */
@@ -59,9 +60,9 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
 int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n);
 hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageCounts &n,
                               hipStream_t s);
-/* The 128-fill of jobs with holes in parts: 1 the owner map (NONE), 2 the
- * frame (128), 0 both (STAGE_FILL). */
-hipError_t cvp9_launch_fill(const DBatch &b, const StageCounts &n, int part, hipStream_t s);
+/* Motion compensation of each job's late units [n_mc_a, n_mc) (STAGE_MC
+ * runs the early ones). */
+hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s);
 
 /* Fused persistent kernel: every stage of the batch in one launch of `wgs`
  * co-resident workgroups (at most cvp9_fused_max_wgs()). */

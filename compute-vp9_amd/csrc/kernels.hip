@@ -109,8 +109,7 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p)
 /* 128 fill of the output frame and NONE fill of the owner map, for jobs    */
 /* whose blocks do not cover the frame (vp9_frame.c:27-30 initial value).   */
 
-/* part: 0 both, 1 the owner map only, 2 the frame only */
-__device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w, int part = 0)
+__device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w)
 {
     const uint32_t j = find_job_wave(B.pre_fill, B.njobs, w);
     const DJob &J = B.jobs[j];
@@ -122,7 +121,6 @@ __device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w, int part 
     uint8_t *base;
     uint64_t len;
     uint4 v;
-    if ((chunk < fchunks) ? part == 1 : part == 2) return;
     if (chunk < fchunks) {
         base = J.out.y + ((uint64_t)chunk << 16);
         len = fbytes - ((uint64_t)chunk << 16);
@@ -143,9 +141,9 @@ __device__ __forceinline__ void fill_item(const DBatch &B, uint32_t w, int part 
     }
 }
 
-__global__ __launch_bounds__(256) void k_fill(DBatch B, int part)
+__global__ __launch_bounds__(256) void k_fill(DBatch B)
 {
-    fill_item(B, blockIdx.x, part);
+    fill_item(B, blockIdx.x);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -428,7 +426,7 @@ __device__ __forceinline__ void mc_unit_at(const DBatch &B, uint32_t u, int li)
     const uint32_t j = find_job(B.pre_inter, B.njobs, u);
     const DJob &J = B.jobs[j];
     const uint32_t lu = u - B.pre_inter[j];
-    if (lu >= J.n_mc) return; /* padding of the job's count */
+    if (lu >= J.n_mc_a) return; /* padding of the job's count */
     const uint4 e = gload4(J.mc_tab + lu);
     const uint32_t ref = (e.y >> 16) & 15;
     DFrame R = {};
@@ -439,18 +437,21 @@ __device__ __forceinline__ void mc_unit_at(const DBatch &B, uint32_t u, int li)
 /* A workgroup takes CVP9_MC_WG_UNITS units of one job (job counts are
  * padded to whole workgroups): the job descriptor is read with scalar loads
  * and its reference table is staged in LDS once. */
-__device__ __forceinline__ void mc_wg(const DBatch &B, uint32_t wg, uint32_t nwg)
+/* late: the units [n_mc_a, n_mc) of each job, else [0, n_mc_a) */
+__device__ __forceinline__ void mc_wg(const DBatch &B, uint32_t wg, uint32_t nwg, bool late)
 {
     static_assert(CVP9_MC_WG_UNITS * CVP9_MC_UNIT == 256, "one unit per 16 lanes");
     __shared__ DFrame s_ref[8];
+    const uint32_t *const pre = late ? B.pre_inter2 : B.pre_inter;
     const uint32_t u0 = xcd_block(wg, nwg) * CVP9_MC_WG_UNITS;
-    const uint32_t j = __builtin_amdgcn_readfirstlane(find_job_wave(B.pre_inter, B.njobs, u0));
+    const uint32_t j = __builtin_amdgcn_readfirstlane(find_job_wave(pre, B.njobs, u0));
     const DJob &J = B.jobs[j];
     constexpr uint32_t ref_words = sizeof(s_ref) / 4;
     if (threadIdx.x < ref_words)
         ((uint32_t *)s_ref)[threadIdx.x] = ((const uint32_t *)J.ref)[threadIdx.x];
-    const uint32_t lu = u0 - B.pre_inter[j] + threadIdx.x / CVP9_MC_UNIT;
-    const bool live = lu < J.n_mc;
+    const uint32_t lo = late ? J.n_mc_a : 0, hi = late ? J.n_mc : J.n_mc_a;
+    const uint32_t lu = lo + u0 - pre[j] + threadIdx.x / CVP9_MC_UNIT;
+    const bool live = lu < hi;
     uint4 e = make_uint4(0, 0, 0, 0);
     if (live) e = gload4(J.mc_tab + lu);
     __syncthreads();
@@ -461,9 +462,9 @@ __device__ __forceinline__ void mc_wg(const DBatch &B, uint32_t wg, uint32_t nwg
     mc_unit(J, e, R, threadIdx.x % CVP9_MC_UNIT, B.resid_all ? J.resid : nullptr);
 }
 
-__global__ __launch_bounds__(256) void k_mc(DBatch B)
+__global__ __launch_bounds__(256) void k_mc(DBatch B, int late)
 {
-    mc_wg(B, blockIdx.x, gridDim.x);
+    mc_wg(B, blockIdx.x, gridDim.x, late != 0);
 }
 
 /* The owner map and motion compensation in one launch (they are
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(256) void k_mc(DBatch B)
 __global__ __launch_bounds__(256) void k_front(DBatch B, uint32_t owner_wgs)
 {
     if (blockIdx.x < owner_wgs) owner_wg(B, blockIdx.x);
-    else mc_wg(B, blockIdx.x - owner_wgs, gridDim.x - owner_wgs);
+    else mc_wg(B, blockIdx.x - owner_wgs, gridDim.x - owner_wgs, false);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1626,23 +1627,25 @@ hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageC
     return span == 1 ? cvp9_launch_stage(stage, b, n, s) : hipErrorInvalidValue;
 }
 
-hipError_t cvp9_launch_fill(const DBatch &b, const StageCounts &n, int part, hipStream_t s)
+hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s)
 {
-    if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b, part);
+    if (n.inter2)
+        hipLaunchKernelGGL(k_mc, dim3(n.inter2 / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, 1);
     return hipGetLastError();
 }
+
 
 hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s)
 {
     switch (stage) {
     case STAGE_FILL:
-        if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b, 0);
+        if (n.fill) hipLaunchKernelGGL(k_fill, dim3(n.fill), dim3(256), 0, s, b);
         break;
     case STAGE_OWNER:
         hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 16) : 1), dim3(256), 0, s, b);
         break;
     case STAGE_MC:
-        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b);
+        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, 0);
         break;
     case STAGE_RES4:
         return launch_residual<4>(b, n, 0, s);

@@ -106,3 +106,33 @@ def test_two_queues_alternating_halves(pkg, orc, gpu_ready, pipeline):
         eng.sync()
         for s, (w, h) in enumerate(sizes):
             assert np.array_equal(eng.read(sids[s], w, h), want[s][n - 1]), f"stream {s}"
+
+
+@pytest.mark.parametrize("pipeline", [True, False])
+def test_pipelined_frames_with_key_frames(pkg, orc, gpu_ready, pipeline):
+    """Frames of two streams enqueued back to back with no synchronisation,
+    KEY frames every third frame: with pipelining a frame's first phase
+    (including motion compensation from slots the previous frame does not
+    write; none after a KEY frame) overlaps the previous frame's second
+    phase. The last frame of each stream, which depends on every frame
+    before it, equals the oracle's."""
+    sizes = [(320, 184), (264, 136)]
+    n = 9
+    seqs = []
+    for s, (w, h) in enumerate(sizes):
+        p = pkg.synth_params(1, w, h, 900 + s)
+        p.key_interval = 3
+        seqs.append([pkg.SynthFrame(p, i) for i in range(n)])
+    want = [orc.run_oracle(q) for q in seqs]
+    assert any(f.pf.hdr.frame_type == pkg.VP9_FRAME_KEY for f in seqs[0][1:])
+    print(f"[case] pipelined frames, KEY every 3, pipeline {pipeline}", flush=True)
+    with pkg.Engine(0) as eng:
+        eng.set_pipeline(pipeline)
+        sids = [eng.open_stream() for _ in sizes]
+        preps = [[eng.prepare(f) for f in q] for q in seqs]
+        for i in range(n):
+            for s in range(len(sizes)):
+                eng.run([sids[s]], [preps[s][i]])
+        eng.sync()
+        for s, (w, h) in enumerate(sizes):
+            assert np.array_equal(eng.read(sids[s], w, h), want[s][n - 1]), f"stream {s}"
