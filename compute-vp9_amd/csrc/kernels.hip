@@ -105,6 +105,20 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p)
     return __ballot(p);
 }
 
+/* Sum of x over the 64 lanes of a whole wave (wave-uniform result): an
+ * inclusive scan inside each row of 16 lanes with DPP row shifts (lanes
+ * whose source lies outside the row add 0), then the four row totals. */
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true); /* row_shr:1 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true); /* row_shr:2 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true); /* row_shr:4 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true); /* row_shr:8 */
+    return (uint32_t)(__builtin_amdgcn_readlane(v, 15) + __builtin_amdgcn_readlane(v, 31) +
+                      __builtin_amdgcn_readlane(v, 47) + __builtin_amdgcn_readlane(v, 63));
+}
+
 /* ------------------------------------------------------------------------ */
 /* 128 fill of the output frame and NONE fill of the owner map, for jobs    */
 /* whose blocks do not cover the frame (vp9_frame.c:27-30 initial value).   */
@@ -985,7 +999,6 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
      *    the neighbour stage; cells of earlier intra blocks from their
      *    edge records. ncx + ncy + 1 <= 33 cells: one per thread. */
-    if (tid < 3) S.sum[tid] = 0;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
     if (tid < 64) { /* the worker's first wave, whole */
         const int c = tid;
@@ -998,6 +1011,30 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         /* a neighbour still running: poll the cell's words until it has
          * published them */
         if (stale) v = cell_from_edges(jedge, epoch, jerr, cell, B.spin_limit);
+        /* DC sums over the w above and h left samples of each plane
+         * (vp9_reconstruct.c:67-90), reduced across the wave: Y | U << 16
+         * and V (at most 128 * 255 and 64 * 255) */
+        uint32_t syu = 0, sv = 0;
+        if (c <= ncx + ncy && kind != 2) {
+            const int q = kind == 0 ? c : c - ncx;
+            const int ly = kind == 0 ? bw : bh, lc = kind == 0 ? cbw : cbh;
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (4 * q + e < ly) syu += (uint32_t)(v >> (8 * e)) & 0xff;
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+                if (2 * q + e < lc) {
+                    syu += ((uint32_t)(v >> (32 + 8 * e)) & 0xff) << 16;
+                    sv += (uint32_t)(v >> (48 + 8 * e)) & 0xff;
+                }
+        }
+        syu = wave_sum(syu);
+        sv = wave_sum(sv);
+        if (c == 0) {
+            S.sum[0] = syu & 0xffff;
+            S.sum[1] = syu >> 16;
+            S.sum[2] = sv;
+        }
         if (c > ncx + ncy) {
             /* no cell */
         } else if (kind == 2) {
@@ -1017,21 +1054,9 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     }
     worker_sync<W>();
 
-    /* 2. DC sums over w above + h left samples */
-    for (int i = tid; i < 3 * 128; i += W) {
-        const int p = i >> 7, e = i & 127;
-        if ((p == 0 ? k.ymode : k.uvmode) != 0) continue;
-        const int w = p ? cbw : bw, h = p ? cbh : bh;
-        uint32_t v = 0;
-        if (e < 64) { if (e < w) v = S.nb[p][0][e]; }
-        else if (e - 64 < h) v = S.nb[p][1][e - 64];
-        if (v) atomicAdd(&S.sum[p], v);
-    }
-    worker_sync<W>();
-
     const int dc0 = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
     if constexpr (EARLY) {
-        /* 3. with EARLY, the edge samples first (bottom row and right
+        /* 2. with EARLY, the edge samples first (bottom row and right
          *    column of Y, U and V), published at once, so blocks waiting on
          *    this one go on while its interior is written */
         const int ne = bw + bh + 2 * (cbw + cbh);
@@ -1061,7 +1086,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         }
         worker_sync<W>();
 
-        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
+        /* 3. publish the edge record: Y bottom, Y right, U bottom, U right,
          *    V bottom, V right */
         for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
             int p, dir, wi, len;
@@ -1078,7 +1103,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         }
     }
 
-    /* 5. the whole block: prediction + residual, 4 luma samples per item,
+    /* 4. the whole block: prediction + residual, 4 luma samples per item,
      *    four items' residuals loaded before any is used (so a 64-thread
      *    worker on a large block waits once per four items, not per item);
      *    chroma prediction only (the reference adds no chroma residual), 4
@@ -1151,7 +1176,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     if constexpr (!EARLY) {
         worker_sync<W>();
 
-        /* 4. publish the edge record: Y bottom, Y right, U bottom, U right,
+        /* 5. publish the edge record: Y bottom, Y right, U bottom, U right,
          *    V bottom, V right */
         for (int t = tid; t < CVP9_EDGE_WORDS; t += W) {
             int p, dir, wi, len;
