@@ -3,10 +3,12 @@
  *
  * Mirrors the Vulkan backend's contract (src/backend/vulkan/vulkan_backend.c:
  * ring :1037-1062, :1391-1426; get :1428-1455; DMA-BUF :1457-1486; export
- * buffers :1488-1544) on one HIP stream:
- *   decode: validate + stage (pinned) -> H2D -> stage kernels -> device
- *           reference-pool update -> pack to the slot's VRAM I420 buffer ->
- *           D2H into the slot's pinned output -> event
+ * buffers :1488-1544) on a one-stream engine (engine_api.cpp: frame
+ * pipelining, shared launches, the stream's reference pool):
+ *   decode: validate + stage (pinned) -> engine batch (H2D on its queue,
+ *           stage kernels, reference-pool update) -> pack to the slot's VRAM
+ *           I420 buffer -> D2H into the slot's pinned output -> event, on
+ *           the queue the batch finishes on
  *   get:    poll / wait the oldest slot's event, hand out pinned pointers.
  */
 #include <hip/hip_runtime.h>
@@ -27,7 +29,6 @@ using namespace cvp9;
 
 namespace {
 
-constexpr uint32_t kSpinLimit = 1u << 21;   /* ~0.1 s of s_sleep per wait */
 
 struct Slot {
     hipEvent_t done = nullptr;
@@ -45,15 +46,14 @@ struct Slot {
 
 struct Backend {
     int device = 0;
-    hipStream_t stream = nullptr;
-    StreamState *st = nullptr;
+    cvp9_engine_t *eng = nullptr;
+    int sid = -1;                       /* the engine stream this context decodes */
     Slot slots[CVP9_HIP_INFLIGHT];
     uint32_t head = 0, count = 0;
     uint8_t *pending_target = nullptr;   /* device address of the armed NV12 target */
     uint64_t pending_target_size = 0;
     struct ExportPriv *pending_priv = nullptr;
     uint32_t debug = 0;
-    Tune tune = cvp9_default_tune(256);
     bool dmabuf = false;
 };
 
@@ -171,17 +171,11 @@ cvp9_err_t hip_backend_init(void **ctx)
     Backend *b = new (std::nothrow) Backend();
     if (!b) return CVP9_ERR_NOMEM;
     b->device = dev;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-        b->tune = cvp9_default_tune(prop.multiProcessorCount);
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
-        return CVP9_ERR_GPU;
-    }
-    b->st = new (std::nothrow) StreamState(dev);
-    if (!b->st) {
+    cvp9_err_t ce = cvp9_engine_create(dev, &b->eng);
+    if (ce == CVP9_OK) ce = cvp9_engine_stream_open(b->eng, &b->sid);
+    if (ce != CVP9_OK) {
         hip_backend_destroy(b);
-        return CVP9_ERR_NOMEM;
+        return ce;
     }
     for (Slot &s : b->slots) {
         if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
@@ -213,10 +207,9 @@ void hip_backend_destroy(void *ctx)
     Backend *b = (Backend *)ctx;
     if (!b) return;
     hipSetDevice(b->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->eng) cvp9_engine_sync(b->eng);
     for (Slot &s : b->slots) free_slot(s);
-    delete b->st;
-    if (b->stream) hipStreamDestroy(b->stream);
+    cvp9_engine_destroy(b->eng);
     delete b;
 }
 
@@ -233,8 +226,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     hipSetDevice(b->device);
 
     Slot &s = b->slots[(b->head + b->count) % CVP9_HIP_INFLIGHT];
-    const size_t batch_off = L.bytes;
-    const size_t need = batch_off + BatchBuilder::bytes_for(1);
+    const size_t need = L.bytes;
     hipError_t e;
     if ((e = grow_pinned(&s.stage_host, &s.stage_host_cap, need)) != hipSuccess) return to_err(e);
     if ((e = grow_device(&s.stage_dev, &s.stage_dev_cap, need)) != hipSuccess) return to_err(e);
@@ -250,34 +242,26 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     }
 
     plan_write(pf, L, s.stage_host);
-    if ((e = b->st->ensure_scratch(L, b->stream)) != hipSuccess) return to_err(e);
-    FrameBuf *out = nullptr;
-    if ((e = b->st->acquire(L.W, L.H, &out)) != hipSuccess) return to_err(e);
-    DJob job = b->st->make_job(L, s.stage_dev, out, b->pending_target);
-    if (b->debug & CVP9_HIP_DEBUG_NO_DEBLOCK) job.lf_thr = 0;
-    BatchBuilder bb;
-    const PrepLayout *lay = &L;
-    bb.build(&job, &lay, 1, s.stage_host + batch_off, s.stage_dev + batch_off, b->tune, kSpinLimit);
-
-    if ((e = hipMemcpyAsync(s.stage_dev, s.stage_host, batch_off + bb.used(), hipMemcpyHostToDevice,
-                            b->stream)) != hipSuccess)
-        return to_err(e);
-    if ((e = BatchBuilder::launch(bb.batch(), bb.counts(), b->stream)) != hipSuccess) return to_err(e);
-    b->st->update_refs(L.key != 0, out);
+    if (b->debug & CVP9_HIP_DEBUG_NO_DEBLOCK) L.lf_thr = 0;
+    StagedResult r;
+    if ((err = engine_run_staged(b->eng, b->sid, L, s.stage_host, s.stage_dev, b->pending_target,
+                                 &r)) != CVP9_OK)
+        return err;
+    const hipStream_t q = r.queue;
 
     /* packed I420 into the slot's VRAM buffer, then to the pinned output */
-    const DFrame f = out->dframe();
+    const DFrame f = r.out;
     const uint32_t cw = (L.W + 1) / 2, ch = (L.H + 1) / 2;
     uint8_t *dy = s.out_dev, *du = dy + (size_t)L.W * L.H, *dv = du + (size_t)cw * ch;
-    if ((e = hipMemcpy2DAsync(dy, L.W, f.y, f.sy, L.W, L.H, hipMemcpyDeviceToDevice, b->stream)) ||
-        (e = hipMemcpy2DAsync(du, cw, f.u, f.suv, cw, ch, hipMemcpyDeviceToDevice, b->stream)) ||
-        (e = hipMemcpy2DAsync(dv, cw, f.v, f.suv, cw, ch, hipMemcpyDeviceToDevice, b->stream)) ||
-        (e = hipMemcpyAsync(s.out_host, s.out_dev, ob, hipMemcpyDeviceToHost, b->stream)) ||
+    if ((e = hipMemcpy2DAsync(dy, L.W, f.y, f.sy, L.W, L.H, hipMemcpyDeviceToDevice, q)) ||
+        (e = hipMemcpy2DAsync(du, cw, f.u, f.suv, cw, ch, hipMemcpyDeviceToDevice, q)) ||
+        (e = hipMemcpy2DAsync(dv, cw, f.v, f.suv, cw, ch, hipMemcpyDeviceToDevice, q)) ||
+        (e = hipMemcpyAsync(s.out_host, s.out_dev, ob, hipMemcpyDeviceToHost, q)) ||
         (b->pending_priv && b->pending_priv->shadow &&
          (e = hipMemcpyAsync(b->pending_priv->map, b->pending_priv->dev, ob, hipMemcpyDeviceToHost,
-                             b->stream))) ||
-        (e = hipMemcpyAsync(s.err_host, b->st->err_word(), 4, hipMemcpyDeviceToHost, b->stream)) ||
-        (e = hipEventRecord(s.done, b->stream)))
+                             q))) ||
+        (e = hipMemcpyAsync(s.err_host, r.err, 4, hipMemcpyDeviceToHost, q)) ||
+        (e = hipEventRecord(s.done, q)))
         return to_err(e);
 
     s.pending = true;
@@ -389,7 +373,7 @@ void hip_export_buffer_free(void *ctx, cvp9_export_buffer_t *buf)
     Backend *b = (Backend *)ctx;
     if (!b || !buf || !buf->priv_buf) return;
     hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
+    cvp9_engine_sync(b->eng);
     ExportPriv *p = (ExportPriv *)buf->priv_buf;
     if (b->pending_priv == p) {
         b->pending_target = nullptr;
@@ -490,7 +474,7 @@ cvp9_err_t cvp9_hip_export_read(void *ctx, const cvp9_export_buffer_t *buf, uint
     const ExportPriv *p = (const ExportPriv *)buf->priv_buf;
     if (n > p->size) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(b->device);
-    if (hipStreamSynchronize(b->stream) != hipSuccess) return CVP9_ERR_GPU;
+    if (cvp9_engine_sync(b->eng) != CVP9_OK) return CVP9_ERR_GPU;
     return hipMemcpy(dst, p->dev, n, hipMemcpyDeviceToHost) == hipSuccess ? CVP9_OK : CVP9_ERR_GPU;
 }
 

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "cvp9_hip/abi_types.h"
+#include "cvp9_hip/engine.h"
 #include "dev_types.h"
 #include "kernels.h"
 
@@ -161,6 +162,19 @@ private:
     size_t used_ = 0;
     std::vector<uint32_t> level_n_, level_ns_;
 };
+
+/* One frame of engine stream sid from a prepared image (layout L) staged in
+ * host memory at host, copied to dev on the batch's first queue: the drop-in
+ * backend's path through the engine. On success res names the output frame
+ * (the stream's new slot 0), the HIP queue the batch finishes on and the
+ * stream's error word. */
+struct StagedResult {
+    DFrame out;
+    hipStream_t queue;
+    uint32_t *err;
+};
+cvp9_err_t engine_run_staged(cvp9_engine_t *e, int sid, const PrepLayout &L, const uint8_t *host,
+                             uint8_t *dev, uint8_t *nv12, StagedResult *res);
 
 /* Once per device: upload the IDCT matrices (computed with the C library's
  * cos exactly as the reference builds them). */

@@ -399,8 +399,51 @@ uint64_t cvp9_engine_prep_bytes(const cvp9_prep_t *p)
     return p ? p->algo : 0;
 }
 
+} // extern "C"
+
+namespace {
+/* a prepared image still in host memory, copied to its device place on the
+ * batch's first queue before anything reads it */
+struct Upload {
+    const uint8_t *host;
+    uint8_t *dev;
+    size_t bytes;
+};
+
+cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *const *preps,
+                     uint8_t *const *nv12, const Upload *up);
+} // namespace
+
 cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *const *preps,
                            uint8_t *const *nv12)
+{
+    return run_batch(e, n, streams, preps, nv12, nullptr);
+}
+
+namespace cvp9 {
+cvp9_err_t engine_run_staged(cvp9_engine_t *e, int sid, const PrepLayout &L, const uint8_t *host,
+                             uint8_t *dev, uint8_t *nv12, StagedResult *res)
+{
+    if (!e || !host || !dev || !res) return CVP9_ERR_INVALID_DATA;
+    cvp9_prep p;
+    p.L = L;
+    p.dev = dev;
+    cvp9_prep_t *pp = &p;
+    const Upload up{host, dev, (size_t)L.bytes};
+    const cvp9_err_t err = run_batch(e, 1, &sid, &pp, nv12 ? &nv12 : nullptr, &up);
+    p.dev = nullptr; /* not the engine's to free */
+    if (err != CVP9_OK) return err;
+    StreamState *s = e->streams[sid];
+    res->out = s->ref(0)->dframe(); /* every frame becomes slot 0 */
+    res->queue = e->queue(e->last_q);
+    res->err = s->err_word();
+    return CVP9_OK;
+}
+} // namespace cvp9
+
+namespace {
+cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *const *preps,
+                     uint8_t *const *nv12, const Upload *up)
 {
     if (!e || n <= 0 || n > CVP9_MAX_BATCH || !streams || !preps) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(e->device);
@@ -502,6 +545,9 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
         if ((he = hipStreamWaitEvent(qa, e->gev, 0)) != hipSuccess) return to_err(he);
         e->graph_pending[q] = false;
     }
+    if (up && (he = hipMemcpyAsync(up->dev, up->host, up->bytes, hipMemcpyHostToDevice, qa)) !=
+                  hipSuccess)
+        return to_err(he);
     if ((he = hipMemcpyAsync(ddev, dhost, bb.used(), hipMemcpyHostToDevice, qa)) != hipSuccess)
         return to_err(he);
     const bool timing = e->timing && !e->capture;
@@ -575,6 +621,9 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
     e->last_q = q;
     return CVP9_OK;
 }
+} // namespace
+
+extern "C" {
 
 cvp9_err_t cvp9_engine_set_fused(cvp9_engine_t *e, int on)
 {
