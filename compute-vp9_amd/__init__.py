@@ -195,6 +195,7 @@ SIGNATURES = {
     "cvp9_engine_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_set_queues": (C.c_int, [C.c_void_p, C.c_int]),
     "cvp9_engine_set_pipeline": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvp9_engine_set_spin_limit": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cvp9_engine_capture_begin": (C.c_int, [C.c_void_p]),
     "cvp9_engine_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "cvp9_engine_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -453,6 +454,9 @@ class Engine:
     def close_stream(self, sid):
         lib().cvp9_engine_stream_close(self.e, sid)
 
+    def reset_stream(self, sid):
+        check(lib().cvp9_engine_stream_reset(self.e, sid), "stream_reset")
+
     def stream_bytes(self, sid):
         return int(lib().cvp9_engine_stream_bytes(self.e, sid))
 
@@ -521,6 +525,9 @@ class Engine:
 
     def set_pipeline(self, on):
         check(lib().cvp9_engine_set_pipeline(self.e, int(on)), "set_pipeline")
+
+    def set_spin_limit(self, polls):
+        check(lib().cvp9_engine_set_spin_limit(self.e, polls), "set_spin_limit")
 
     def capture_begin(self):
         check(lib().cvp9_engine_capture_begin(self.e), "capture_begin")

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "cvp9_hip/engine.h"
+#include "cvp9_hip/testing.h"
 #include "engine.hpp"
 
 static_assert(CVP9_ENGINE_STAGES == STAGE_COUNT, "stage list of the ABI and the kernels differ");
@@ -128,6 +129,7 @@ struct cvp9_engine {
      * phase waits for the batch that last read its set. Capture, fused mode,
      * per-stage timing and one queue run both phases in order on one queue. */
     bool pipeline = true;
+    uint32_t spin_limit = kSpinLimit; /* bound on every dependency wait */
     hipStream_t astream[2] = {nullptr, nullptr};
     hipEvent_t aev[2] = {nullptr, nullptr};
     int last_mode = -1;            /* overlap of the previous batch (0/1), -1 none */
@@ -344,6 +346,13 @@ cvp9_err_t cvp9_engine_set_queues(cvp9_engine_t *e, int n)
     return CVP9_OK;
 }
 
+cvp9_err_t cvp9_engine_set_spin_limit(cvp9_engine_t *e, uint32_t polls)
+{
+    if (!e) return CVP9_ERR_INVALID_DATA;
+    e->spin_limit = polls ? polls : kSpinLimit;
+    return CVP9_OK;
+}
+
 cvp9_err_t cvp9_engine_set_pipeline(cvp9_engine_t *e, int on)
 {
     if (!e) return CVP9_ERR_INVALID_DATA;
@@ -537,7 +546,7 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
     for (int i = 0; i < n; i++) lays[i] = &preps[i]->L;
     Tune tune = e->tune;
     if (e->fused) tune.intra_level_min = 0; /* the fused kernel runs persistent workers only */
-    bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, kSpinLimit);
+    bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, e->spin_limit);
     bb.set_resid_all(resid_all);
     for (hipEvent_t ev : readers)
         if ((he = hipStreamWaitEvent(qa, ev, 0)) != hipSuccess) return to_err(he);

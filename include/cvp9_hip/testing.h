@@ -66,6 +66,13 @@ double cvp9_hip_copy_probe(uint64_t bytes, int iters);
 #define CVP9_HIP_DEBUG_NO_DEBLOCK 1u
 cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags);
 
+/* Bound on every dependency wait of the engine's later batches, in polls
+ * (0: the default, about 2 s). A tiny bound makes waits give up, so tests
+ * can check that a timed-out wait is reported as CVP9_ERR_GPU while every
+ * kernel still ends. The engine type is declared in cvp9_hip/engine.h. */
+struct cvp9_engine;
+cvp9_err_t cvp9_engine_set_spin_limit(struct cvp9_engine *e, uint32_t polls);
+
 #ifdef __cplusplus
 }
 #endif

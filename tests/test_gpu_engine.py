@@ -136,3 +136,33 @@ def test_pipelined_frames_with_key_frames(pkg, orc, gpu_ready, pipeline):
         eng.sync()
         for s, (w, h) in enumerate(sizes):
             assert np.array_equal(eng.read(sids[s], w, h), want[s][n - 1]), f"stream {s}"
+
+
+@pytest.mark.parametrize("pipeline", [True, False])
+def test_dependency_timeout_is_reported(pkg, orc, gpu_ready, pipeline):
+    """A dependency wait that gives up is an error, not a hang or a silent
+    wrong frame: with a bound of one poll per wait, a KEY frame's chained
+    intra blocks (all of them persistent-worker work here) time out and the
+    stream reports CVP9_ERR_GPU while every kernel ends. After a stream
+    reset with the default bound the same sequence decodes bit-exact."""
+    w, h = 320, 184
+    p = pkg.synth_params(1, w, h, 77)
+    seq = [pkg.SynthFrame(p, i) for i in range(3)]
+    want = orc.run_oracle(seq)
+    print(f"[case] dependency timeout, pipeline {pipeline}", flush=True)
+    with pkg.Engine(0) as eng:
+        eng.set_pipeline(pipeline)
+        eng.set_tune([1, 1, 2, 4], 256, 512, 0)
+        sid = eng.open_stream()
+        preps = [eng.prepare(f) for f in seq]
+        eng.set_spin_limit(1)
+        eng.run([sid], [preps[0]])
+        with pytest.raises(pkg.Cvp9Error) as ei:
+            eng.sync()
+        assert ei.value.code == pkg.CVP9_ERR_GPU
+        eng.set_spin_limit(0)
+        eng.reset_stream(sid)
+        for f in preps:
+            eng.run([sid], [f])
+        eng.sync()
+        assert np.array_equal(eng.read(sid, w, h), want[-1])
