@@ -58,10 +58,11 @@
  * followed by the block's local index and a copy of its DBlock, so the
  * ordered stage reads everything it needs about a block in one round. */
 #define CVP9_NB_CELLS 33
-#define CVP9_NB_MASK  33
-#define CVP9_NB_INDEX 34
-#define CVP9_NB_BLOCK 35   /* 3 words */
-#define CVP9_NB_WORDS 38
+#define CVP9_NB_MASK  33   /* cells of earlier intra blocks (edge-record pointers) */
+#define CVP9_NB_IMASK 34   /* cells of inter blocks (samples read from the frame) */
+#define CVP9_NB_INDEX 35
+#define CVP9_NB_BLOCK 36   /* 3 words */
+#define CVP9_NB_WORDS 39
 
 /* Motion compensation splits an inter block into units of
  * cvp9_mc_unit_rows(bw) luma rows, four units per wave. A unit is 16 lanes;

@@ -361,10 +361,10 @@ StreamState::~StreamState()
     }
     for (int p = 0; p < 2; p++) {
         hipFree(owner_[p]);
+        hipFree(nbr_[p]);
         hipFree(resid_[p]);
     }
     hipFree(edge_);
-    hipFree(nbr_);
     hipFree(err_);
 }
 
@@ -372,7 +372,7 @@ bool StreamState::scratch_fits(const PrepLayout &L) const
 {
     const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
     return err_ && cells <= owner_cap_[par_] && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
-           (L.n_intra ? L.n_intra : 1) <= nbr_cap_ &&
+           (L.n_intra ? L.n_intra : 1) <= nbr_cap_[par_] &&
            (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_[par_];
 }
 
@@ -476,17 +476,17 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
         edge_cap_ = cap;
     }
     const size_t ni = L.n_intra ? L.n_intra : 1;
-    if (ni > nbr_cap_) {
-        size_t cap = nbr_cap_ ? nbr_cap_ : 1024;
+    if (ni > nbr_cap_[p]) {
+        size_t cap = nbr_cap_[p] ? nbr_cap_[p] : 1024;
         while (cap < ni) cap *= 2;
-        if (nbr_) {
+        if (nbr_[p]) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            hipFree(nbr_);
-            nbr_ = nullptr;
+            hipFree(nbr_[p]);
+            nbr_[p] = nullptr;
         }
         /* rewritten by the neighbour stage before every read */
-        if ((e = hipMalloc(&nbr_, cap * CVP9_NB_WORDS * sizeof(uint64_t))) != hipSuccess) return e;
-        nbr_cap_ = cap;
+        if ((e = hipMalloc(&nbr_[p], cap * CVP9_NB_WORDS * sizeof(uint64_t))) != hipSuccess) return e;
+        nbr_cap_[p] = cap;
     }
     const size_t nc = L.num_coeffs ? L.num_coeffs : 1;
     if (nc > resid_cap_[p]) {
@@ -523,7 +523,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     }
     j.owner = owner_[par_];
     j.edge = edge_;
-    j.nbr = nbr_;
+    j.nbr = nbr_[par_];
     j.resid = resid_[par_];
     reader_[par_] = serial;
     if (two_sets_) par_ ^= 1;
@@ -548,7 +548,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
 size_t StreamState::resident_bytes() const
 {
     size_t b = (owner_cap_[0] + owner_cap_[1]) * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 +
-               nbr_cap_ * CVP9_NB_WORDS * 8 + (resid_cap_[0] + resid_cap_[1]) * 2 + (err_ ? 256 : 0);
+               (nbr_cap_[0] + nbr_cap_[1]) * CVP9_NB_WORDS * 8 +
+               (resid_cap_[0] + resid_cap_[1]) * 2 + (err_ ? 256 : 0);
     for (const FrameBuf *f : all_) b += f->cap;
     return b;
 }

@@ -83,10 +83,11 @@ public:
     }
     /* scratch already large enough for L (no allocation needed) */
     bool scratch_fits(const PrepLayout &L) const;
-    /* Frame pipelining: the owner map and the residual buffer come in two
-     * sets, alternated frame by frame, so the owner map and transforms of
-     * one frame can run while the previous frame still reads its set. With
-     * two_sets off every frame uses set 0. */
+    /* Frame pipelining: the owner map, the neighbour records and the
+     * residual buffer come in two sets, alternated frame by frame, so a
+     * frame's owner map, neighbour records and transforms can run while the
+     * previous frame still reads its set. With two_sets off every frame uses
+     * set 0. */
     void set_two_sets(bool on) { two_sets_ = on; if (!on) par_ = 0; }
     int parity() const { return par_; }
     /* serial of the batch whose later stages last read set p (0: none) */
@@ -120,8 +121,8 @@ private:
     size_t owner_cap_[2] = {0, 0};
     uint64_t *edge_ = nullptr;
     size_t edge_cap_ = 0;
-    uint64_t *nbr_ = nullptr;
-    size_t nbr_cap_ = 0;      /* intra blocks */
+    uint64_t *nbr_[2] = {nullptr, nullptr};
+    size_t nbr_cap_[2] = {0, 0}; /* intra blocks */
     int16_t *resid_[2] = {nullptr, nullptr};
     size_t resid_cap_[2] = {0, 0};
     uint32_t *err_ = nullptr;

@@ -37,16 +37,16 @@ struct BatchSlot {
 
 /* Stage order without frame pipelining, then the two phases with it: the
  * first depends only on the frame's own data and on reference slots the
- * previous frame does not write (fill, owner map, transforms to int16
- * residuals, motion compensation of the early units), the second on the
- * previous frame (motion compensation of the late units, with the residual
- * add like the early ones, neighbours, intra, deblocking). */
+ * previous frame does not write (fill, owner map, neighbour records,
+ * transforms to int16 residuals, motion compensation of the early units),
+ * the second on the previous frame (motion compensation of the late units,
+ * with the residual add like the early ones, intra, deblocking). */
 constexpr int kOrderPlain[] = {STAGE_FILL, STAGE_OWNER, STAGE_MC, STAGE_RES4, STAGE_RES8,
                                STAGE_RES16, STAGE_RES32, STAGE_NBR, STAGE_INTRA, STAGE_LF,
                                STAGE_NV12};
-constexpr int kOrderA[] = {STAGE_FILL, STAGE_OWNER, STAGE_RES4, STAGE_RES8, STAGE_RES16,
-                           STAGE_RES32, STAGE_MC};
-constexpr int kOrderB[] = {STAGE_MC, STAGE_NBR, STAGE_INTRA, STAGE_LF, STAGE_NV12};
+constexpr int kOrderA[] = {STAGE_FILL, STAGE_OWNER, STAGE_NBR, STAGE_RES4, STAGE_RES8,
+                           STAGE_RES16, STAGE_RES32, STAGE_MC};
+constexpr int kOrderB[] = {STAGE_MC, STAGE_INTRA, STAGE_LF, STAGE_NV12};
 
 struct TimedLaunch {
     int stage;

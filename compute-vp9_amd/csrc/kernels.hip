@@ -715,12 +715,12 @@ __global__ __launch_bounds__(256) void k_res_small(DBatch B, uint3 wg)
 /* job's intra list by level, so their tickets were handed out earlier, to  */
 /* workers that are running, and every wait ends.                           */
 
-/* Neighbour stage: one wave per intra block, one lane per neighbour cell.
- * A cell owned by no block, by a later block or lying in an inter block is
- * final before the intra stage starts (inter blocks are complete after the
- * residual stage), so its samples are resolved here, off the ordered path;
- * a cell of an earlier intra block becomes a pointer into that block's edge
- * record. A sample of block b owned by an earlier intra block o always lies
+/* Neighbour stage: one wave per intra block, one lane per neighbour cell,
+ * from the owner map alone (so it can run before motion compensation). A
+ * cell owned by no block or by a later block holds the initial 128; a cell
+ * of an inter block is marked (CVP9_NB_IMASK) for the intra stage to read
+ * from the frame, where it is final by then; a cell of an earlier intra
+ * block becomes a pointer into that block's edge record. A sample of block b owned by an earlier intra block o always lies
  * on o's bottom row or right column: o covers the sample and not the
  * sample's neighbour inside b, and blocks are disjoint rectangles. The
  * chroma samples next to b lie in the same 4x4 luma cells as the luma ones
@@ -736,18 +736,16 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     const uint32_t b = J.intra[tk];
     const DBlock k = J.blocks[b];
     const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
-    const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
+    const int cbx = x >> 1, cby = y >> 1;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
     GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
     /* fields held in registers across the record stores */
-    const DFrame F = J.out;
     const GMEM uint32_t *const owner = gm(J.owner);
     const DBlock *const blocks = J.blocks;
-    const GMEM uint8_t *const fy = gm(F.y), *const fu = gm(F.u), *const fv = gm(F.v);
     const uint32_t ow = J.ow;
-    uint64_t mask = 0;
+    uint64_t mask = 0, imask = 0;
     for (int c = (int)li; c <= ncx + ncy; c += 32) {
-        bool dep;
+        bool dep, inter = false;
         int kind, q = 0;
         bool present;
         if (c < ncx) { kind = 0; q = c; present = y > 0; }
@@ -769,26 +767,7 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
                     memcpy(&n, w, sizeof(n));
                 }
                 if (!(n.flags & DB_INTRA)) {
-                    if (kind == 0) {
-                        /* rows are 64-byte aligned and x + 4q < W: the dword and
-                         * the chroma pair stay inside their rows */
-                        val = *(const GMEM uint32_t *)(fy + (size_t)(y - 1) * F.sy + x + 4 * q);
-                        const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
-                        val |= (uint64_t)*(const GMEM uint16_t *)(fu + co) << 32;
-                        val |= (uint64_t)*(const GMEM uint16_t *)(fv + co) << 48;
-                    } else if (kind == 1) {
-                        for (int e = 0; e < 4 && 4 * q + e < bh; e++)
-                            val |= (uint64_t)fy[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
-                        for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
-                            const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
-                            val |= (uint64_t)fu[co] << (32 + 8 * e);
-                            val |= (uint64_t)fv[co] << (48 + 8 * e);
-                        }
-                    } else {
-                        const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
-                        val = (uint64_t)fy[(size_t)(y - 1) * F.sy + x - 1] << 24 |
-                              (uint64_t)fu[co] << 40 | (uint64_t)fv[co] << 56;
-                    }
+                    inter = true; /* read by the intra stage */
                 } else {
                     const int ox = n.x, oy = n.y;
                     const bool bottom_row = y - 1 == oy + n.bh - 1, right_col = x - 1 == ox + n.bw - 1;
@@ -813,14 +792,19 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
         }
         rec[c] = val;
         if (dep) mask |= 1ull << c;
+        if (inter) imask |= 1ull << c;
     }
-    /* OR of the 32 lanes' dependency bits */
+    /* OR of the 32 lanes' dependency and inter bits */
 #pragma unroll
-    for (int off = 1; off < 32; off <<= 1) mask |= __shfl_xor(mask, off, 32);
+    for (int off = 1; off < 32; off <<= 1) {
+        mask |= __shfl_xor(mask, off, 32);
+        imask |= __shfl_xor(imask, off, 32);
+    }
     if (li == 0) rec[CVP9_NB_MASK] = mask;
-    if (li == 1) rec[CVP9_NB_INDEX] = b;
-    if (li >= 2 && li < 2 + CVP9_NB_WORDS - CVP9_NB_BLOCK)
-        rec[CVP9_NB_BLOCK + li - 2] = ((const GMEM uint64_t *)gm(blocks + b))[li - 2];
+    if (li == 1) rec[CVP9_NB_IMASK] = imask;
+    if (li == 2) rec[CVP9_NB_INDEX] = b;
+    if (li >= 3 && li < 3 + CVP9_NB_WORDS - CVP9_NB_BLOCK)
+        rec[CVP9_NB_BLOCK + li - 3] = ((const GMEM uint64_t *)gm(blocks + b))[li - 3];
 }
 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
@@ -872,6 +856,38 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
     }
     const uint32_t sh = 8 * (c & 3);
     return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
+}
+
+/* A neighbour cell lying in an inter block, read from the frame (final
+ * once motion compensation and the inter residuals are done): 4 Y samples
+ * and 2 each of U and V in the layout of a resolved cell; kind 0 above
+ * (cell q), 1 left (cell q), 2 above-left. Rows are 64-byte aligned and
+ * x + 4q < W, so the dword and the chroma pair of an above cell stay
+ * inside their rows. */
+__device__ __forceinline__ uint64_t inter_cell(const DFrame &F, int x, int y, int bh, int kind, int q)
+{
+    const GMEM uint8_t *const fy = gm(F.y), *const fu = gm(F.u), *const fv = gm(F.v);
+    const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
+    uint64_t val = 0;
+    if (kind == 0) {
+        val = *(const GMEM uint32_t *)(fy + (size_t)(y - 1) * F.sy + x + 4 * q);
+        const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
+        val |= (uint64_t)*(const GMEM uint16_t *)(fu + co) << 32;
+        val |= (uint64_t)*(const GMEM uint16_t *)(fv + co) << 48;
+    } else if (kind == 1) {
+        for (int e = 0; e < 4 && 4 * q + e < bh; e++)
+            val |= (uint64_t)fy[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
+        for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
+            const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
+            val |= (uint64_t)fu[co] << (32 + 8 * e);
+            val |= (uint64_t)fv[co] << (48 + 8 * e);
+        }
+    } else {
+        const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
+        val = (uint64_t)fy[(size_t)(y - 1) * F.sy + x - 1] << 24 | (uint64_t)fu[co] << 40 |
+              (uint64_t)fv[co] << 56;
+    }
+    return val;
 }
 
 /* The cell of an intra neighbour from its edge record: one load of each
@@ -954,6 +970,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
     const uint64_t cell = tid < CVP9_NB_CELLS ? rec[tid] : 0;
     const uint64_t dmask = rec[CVP9_NB_MASK];
+    const uint64_t imask = rec[CVP9_NB_IMASK];
     const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
     DBlock k;
     {
@@ -1008,6 +1025,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         uint64_t v = present ? cell : 0x7f7f7f7f7f7f7f7full;
         bool stale = false;
         if (pend) v = edge_try(jedge, epoch, cell, stale);
+        else if (present && c <= ncx + ncy && ((imask >> c) & 1))
+            v = inter_cell(out, x, y, bh, kind, kind == 0 ? c : c - ncx);
         /* a neighbour still running: poll the cell's words until it has
          * published them */
         if (stale) v = cell_from_edges(jedge, epoch, jerr, cell, B.spin_limit);

@@ -98,12 +98,12 @@ uint64_t cvp9_server_stream_cost(uint32_t w, uint32_t h)
 {
     /* frames: KEY frame, previous, current, and the frame the previous one
      * read (not reused while that frame may still run: frame pipelining);
-     * two sets of owner map and residual scratch (frame pipelining), edge
-     * records (512 B per block) and neighbour records (304 B per intra
-     * block) for 8x8 blocks, NV12 ring */
+     * two sets (frame pipelining) of owner map, neighbour records (312 B
+     * per intra block, 8x8 blocks) and residual scratch; edge records
+     * (512 B per block, 8x8 blocks); NV12 ring */
     const uint64_t px = (uint64_t)w * h;
-    return 4ull * cvp9::frame_bytes(w, h, nullptr, nullptr) + 2 * (px / 4 + 2 * px) + 8 * px +
-           5 * px + CVP9_SERVER_NV12_RING * nv12_bytes(w, h);
+    return 4ull * cvp9::frame_bytes(w, h, nullptr, nullptr) + 2 * (px / 4 + 5 * px + 2 * px) +
+           8 * px + CVP9_SERVER_NV12_RING * nv12_bytes(w, h);
 }
 
 cvp9_err_t cvp9_server_create(cvp9_engine_t *e, const cvp9_server_config_t *cfg, cvp9_server_t **out)

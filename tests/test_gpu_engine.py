@@ -166,3 +166,31 @@ def test_dependency_timeout_is_reported(pkg, orc, gpu_ready, pipeline):
             eng.run([sid], [f])
         eng.sync()
         assert np.array_equal(eng.read(sid, w, h), want[-1])
+
+
+def test_pipelined_resolution_changes(pkg, orc, gpu_ready):
+    """One stream whose frame size changes at KEY frames (small, large,
+    middle), every frame enqueued back to back with pipelining: per-stream
+    scratch and frame buffers grow while earlier frames may still run. Each
+    segment's last frame, read after the whole sequence, equals the
+    oracle's decode of that segment."""
+    segs = [(136, 72, 3), (320, 184, 3), (200, 120, 3)]
+    seqs = []
+    for s, (w, h, n) in enumerate(segs):
+        p = pkg.synth_params(1, w, h, 950 + s)
+        seqs.append([pkg.SynthFrame(p, i) for i in range(n)])
+    want = [orc.run_oracle(q)[-1] for q in seqs]
+    print("[case] pipelined resolution changes", flush=True)
+    with pkg.Engine(0) as eng:
+        sid = eng.open_stream()
+        sid2 = eng.open_stream()  # the same sequence, read segment by segment
+        for k, q in enumerate(seqs):
+            preps = [eng.prepare(f) for f in q]
+            for pr in preps:
+                eng.run([sid], [pr])
+                eng.run([sid2], [pr])
+            w, h, _ = segs[k]
+            assert np.array_equal(eng.read(sid2, w, h), want[k]), f"segment {k}"
+        w, h, _ = segs[-1]
+        eng.sync()
+        assert np.array_equal(eng.read(sid, w, h), want[-1])
