@@ -186,7 +186,8 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
 __device__ __forceinline__ void owner_wg(const DBatch &B, uint32_t wg)
 {
     const uint32_t total = B.pre_blocks[B.njobs];
-    const uint32_t w0 = wg * 16 + (threadIdx.x >> 6) * 4; /* the wave's first block */
+    /* the wave's first block, wave-uniform */
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(wg * 16 + (threadIdx.x >> 6) * 4);
     if (w0 >= total) return;
     const uint32_t jh = find_job_wave(B.pre_blocks, B.njobs, w0);
     const uint32_t wv = w0 + ((threadIdx.x & 63) >> 4);
@@ -207,33 +208,46 @@ __global__ __launch_bounds__(256) void k_owner(DBatch B)
 /* it. A source window of 5 samples inside the reference row is read with   */
 /* two aligned dword loads; others take each tap with the edge clamp.       */
 
-/* the 5 source samples [sx, sx+5) of reference row y (clamped), packed */
-__device__ __forceinline__ uint64_t mc_src(const uint8_t *__restrict__ ref, uint32_t rs, int rw,
-                                           int rh, int sx, int y)
+/* Source windows of one lane's quad column in one plane, for the rows of
+ * its run: reference row y0 + t (clamped to the plane) for t = 0 .. m, each
+ * as the 8 bytes at the 4-aligned column b, and the byte of that window each
+ * of the 5 taps sx + e takes, edge clamp included (vp9_reconstruct.c:94-131).
+ * b is sx clamped to [0, max(rw - 5, 0)] and aligned down, so the window
+ * holds every clamped tap position; it may pass the row end by 3 bytes,
+ * inside the row pitch (rows are 64-byte aligned) or the frame's tail slack.
+ * Rows past m repeat row y0 + m. The loads are issued with no branch around
+ * them and used only after all of them (and the residual's) are in flight:
+ * a load inside a branch is waited for before the branch ends, which made
+ * each row one memory round trip. */
+struct McWin {
+    uint64_t w[CVP9_MC_RUN + 1];
+    uint32_t sh[5];
+};
+
+__device__ __forceinline__ void mc_win(McWin &q, const uint8_t *ref, uint32_t rs, int rw, int rh,
+                                       int sx, int y0, int m)
 {
-    const GMEM uint8_t *row = gm(ref) + (size_t)clampi(y, 0, rh - 1) * rs;
-    if (sx >= 0 && sx + 4 <= rw - 1) {
-        /* rows start 64-byte aligned; the 8-byte window may pass the row end
-         * by 3 bytes, inside the row pitch or the frame's tail slack */
-        const uint32_t o = (uint32_t)sx & 3;
-        const GMEM uint32_t *q = (const GMEM uint32_t *)(row + (sx - (int)o));
-        return (((uint64_t)q[1] << 32) | q[0]) >> (8 * o);
-    }
-    uint64_t v = 0;
+    const int b = clampi(sx, 0, max(rw - 5, 0)) & ~3;
 #pragma unroll
-    for (int e = 0; e < 5; e++) v |= (uint64_t)row[clampi(sx + e, 0, rw - 1)] << (8 * e);
-    return v;
+    for (int e = 0; e < 5; e++) q.sh[e] = 8u * (uint32_t)(clampi(sx + e, 0, rw - 1) - b);
+    const GMEM uint8_t *base = gm(ref) + b;
+#pragma unroll
+    for (int t = 0; t <= CVP9_MC_RUN; t++) {
+        const int y = clampi(y0 + min(t, m), 0, rh - 1);
+        const GMEM uint32_t *p = (const GMEM uint32_t *)(base + (size_t)y * rs);
+        q.w[t] = (uint64_t)p[1] << 32 | p[0];
+    }
 }
 
-/* horizontal pass of a packed source window: 4 samples, 8 bits each */
-__device__ __forceinline__ uint32_t mc_hpass(uint64_t v, uint32_t fx)
+/* horizontal pass of window row t: 4 samples, 8 bits each */
+__device__ __forceinline__ uint32_t mc_hpass(const McWin &q, int t, uint32_t fx)
 {
+    uint32_t a[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) a[e] = (uint32_t)(q.w[t] >> q.sh[e]) & 0xff;
     uint32_t h = 0;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const uint32_t a = (uint32_t)(v >> (8 * e)) & 0xff, b = (uint32_t)(v >> (8 * e + 8)) & 0xff;
-        h |= ((a * (8 - fx) + b * fx + 4) >> 3) << (8 * e);
-    }
+    for (int e = 0; e < 4; e++) h |= ((a[e] * (8 - fx) + a[e + 1] * fx + 4) >> 3) << (8 * e);
     return h;
 }
 
@@ -258,107 +272,32 @@ __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
     }
 }
 
-/* n output rows of one quad column: prediction rows y0 .. y0+n-1 of the
- * plane at 1/8-sample position (ix + fx/8, y0 + fy/8) (ix: the quad's first
- * sample), stored at d with pitch ds, nv samples of the quad valid.
- * ref == nullptr: an empty slot predicts flat 127 (:258-263, :328-333). */
-__device__ __forceinline__ void mc_run(const uint8_t *__restrict__ ref, uint32_t rs, int rw, int rh,
-                                       int ix, int y0, uint32_t fx, uint32_t fy, int n,
-                                       uint8_t *d, uint32_t ds, int nv)
+/* The m filtered rows of a run (flat 127 for an empty slot,
+ * vp9_reconstruct.c:258-263, :328-333), the residual rows res added when
+ * add, stored at d with pitch ds, nv samples of the quad valid; nothing is
+ * stored by a lane with on false, whose values are computed all the same
+ * (so that the loads they use stay ahead of every branch). */
+__device__ __forceinline__ void mc_emit(const McWin &q, bool on, int m, bool flat, uint32_t fx,
+                                        uint32_t fy, const uint64_t *res, bool add, uint8_t *d,
+                                        uint32_t ds, int nv)
 {
-    if (!ref) {
-        for (int t = 0; t < n; t++) store_quad(d + (size_t)t * ds, 0x7f7f7f7fu, nv);
-        return;
-    }
-    for (int t0 = 0; t0 < n; t0 += CVP9_MC_RUN) {
-        const int m = min(n - t0, CVP9_MC_RUN);
-        uint64_t src[CVP9_MC_RUN + 1];
-#pragma unroll
-        for (int t = 0; t <= CVP9_MC_RUN; t++)
-            if (t <= m) src[t] = mc_src(ref, rs, rw, rh, ix, y0 + t0 + t);
-        uint32_t top = mc_hpass(src[0], fx);
-#pragma unroll
-        for (int t = 0; t < CVP9_MC_RUN; t++) {
-            if (t < m) {
-                const uint32_t bot = mc_hpass(src[t + 1], fx);
-                store_quad(d + (size_t)(t0 + t) * ds, mc_vpass(top, bot, fy), nv);
-                top = bot;
-            }
-        }
-    }
-}
-
-/* The first CVP9_MC_RUN rows of a run in two steps, so that a lane's loads
- * for luma and chroma are in flight together: the m + 1 source rows
- * (m = min(n, CVP9_MC_RUN)), then the filtered rows stored. */
-struct McRun {
-    uint64_t src[CVP9_MC_RUN + 1];
-    uint64_t res[CVP9_MC_RUN]; /* luma residual of the run's rows, 4 int16 each */
-    int m;
-    bool add;                  /* res holds a residual to add */
-};
-
-__device__ __forceinline__ void mc_load(McRun &q, const uint8_t *__restrict__ ref, uint32_t rs,
-                                        int rw, int rh, int ix, int y0, int n)
-{
-    q.m = min(n, CVP9_MC_RUN);
-#pragma unroll
-    for (int t = 0; t <= CVP9_MC_RUN; t++)
-        if (ref && t <= q.m) q.src[t] = mc_src(ref, rs, rw, rh, ix, y0 + t);
-}
-
-/* The residual under a luma run (transform stage in resid_all mode): the 4
- * int16 of each row r .. r + q.m - 1 at quad column c0 of a block whose
- * first transform's coefficients start at coeff, transforms of 4 << (lt-2)
- * samples, nx to a row; rows / columns past the transform grid (rrows,
- * rcols) have none (vp9_reconstruct.c:336-363). */
-__device__ __forceinline__ void mc_res_load(McRun &q, const int16_t *resid, uint32_t coeff, int lt,
-                                            int nx, int rrows, int rcols, int r, int c0)
-{
-    const int T = 1 << lt;
+    uint32_t top = mc_hpass(q, 0, fx);
 #pragma unroll
     for (int t = 0; t < CVP9_MC_RUN; t++) {
-        q.res[t] = 0;
-        const int rr = r + t;
-        if (t < q.m && rr < rrows && c0 < rcols) {
-            const GMEM int16_t *p = gm(resid) + coeff +
-                                    (size_t)(((rr >> lt) * nx + (c0 >> lt)) << (2 * lt)) +
-                                    ((rr & (T - 1)) << lt) + (c0 & (T - 1));
-            if (((uintptr_t)p & 7) == 0) {
-                q.res[t] = *(const GMEM uint64_t *)p;
-            } else {
+        const uint32_t bot = mc_hpass(q, t + 1, fx);
+        uint32_t v = flat ? 0x7f7f7f7fu : mc_vpass(top, bot, fy);
+        top = bot;
+        if (add) {
+            uint32_t o = 0;
 #pragma unroll
-                for (int e = 0; e < 4; e++) q.res[t] |= (uint64_t)(uint16_t)p[e] << (16 * e);
+            for (int e = 0; e < 4; e++) {
+                const int p = (int)((v >> (8 * e)) & 0xff);
+                const int rv = (int16_t)(uint16_t)(res[t] >> (16 * e));
+                o |= (uint32_t)clampi(p + rv, 0, 255) << (8 * e);
             }
+            v = o;
         }
-    }
-}
-
-__device__ __forceinline__ void mc_emit(const McRun &q, bool flat, uint32_t fx, uint32_t fy,
-                                        uint8_t *d, uint32_t ds, int nv)
-{
-    uint32_t top = flat ? 0 : mc_hpass(q.src[0], fx);
-#pragma unroll
-    for (int t = 0; t < CVP9_MC_RUN; t++) {
-        if (t < q.m) {
-            uint32_t v = 0x7f7f7f7fu; /* empty slot: flat 127 */
-            if (!flat) {
-                const uint32_t bot = mc_hpass(q.src[t + 1], fx);
-                v = mc_vpass(top, bot, fy);
-                top = bot;
-            }
-            if (q.add) {
-                uint32_t o = 0;
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int p = (int)((v >> (8 * e)) & 0xff);
-                    const int rv = (int16_t)(uint16_t)(q.res[t] >> (16 * e));
-                    o |= (uint32_t)clampi(p + rv, 0, 255) << (8 * e);
-                }
-                v = o;
-            }
-            store_quad(d + (size_t)t * ds, v, nv);
-        }
+        if (on && t < m) store_quad(d + (size_t)t * ds, v, nv);
     }
 }
 
@@ -378,7 +317,10 @@ __device__ __forceinline__ bool mc_lane(int li, int ncol, int nrows, int &col, i
 
 /* One motion-compensation unit of job J on 16 lanes (li = 0..15): e is its
  * unit-table entry, R the reference frame of its block (R.y == nullptr: the
- * slot is empty or the block names none). */
+ * slot is empty or the block names none). A lane's luma and chroma runs are
+ * at most CVP9_MC_RUN rows: with blocks of at most 64x64, a unit of bw wide
+ * has 64 / next_pow2(ceil(bw/4)) luma rows (4 from bw > 32 on) spread over
+ * floor(16 / ceil(bw/4)) lane groups, and its chroma rows likewise. */
 __device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R, int li,
                                         const int16_t *resid)
 {
@@ -399,39 +341,68 @@ __device__ __forceinline__ void mc_unit(const DJob &J, uint4 e, const DFrame &R,
     /* held in registers: a store could alias the descriptor as far as the
      * compiler knows, and would make it reload it */
     const DFrame out = J.out;
-    /* luma and chroma runs of this lane; all their source rows are loaded
-     * before either is filtered */
-    int lcol, lra, ln, ccol, cra, cn;
+    int lcol = 0, lra = 0, ln = 0, ccol = 0, cra = 0, cn = 0;
     const bool lon = mc_lane(li, nq, r1 - r0, lcol, lra, ln);
     const bool con = mc_lane(li, 2 * ncq, cr1 - cr0, ccol, cra, cn);
+    const int lm = min(ln, CVP9_MC_RUN), cm = min(cn, CVP9_MC_RUN);
     const int lpx8 = bx * 8 + mvx, lpy8 = by * 8 + mvy;
     /* chroma: MV halved toward zero, bounds from half the reference luma
      * size; the unit's U columns, then its V columns */
     const int cpx8 = cbx * 8 + mvx / 2, cpy8 = cby * 8 + mvy / 2;
     const int cp = ccol >= ncq, cq = ccol - cp * ncq;
-    const uint8_t *cref = R.y ? (cp ? R.v : R.u) : nullptr;
+    /* an empty slot predicts flat 127; its loads read the output frame's
+     * first row and are discarded */
+    const bool flat = !R.y;
+    const uint8_t *lref = flat ? out.y : R.y;
+    const uint8_t *cref = flat ? out.u : (cp ? R.v : R.u);
+    const uint32_t lrs = flat ? 0 : R.sy, crs = flat ? 0 : R.suv;
+    const int lrw = flat ? 1 : (int)R.w, lrh = flat ? 1 : (int)R.h;
+    const int crw = flat ? 1 : (int)R.w / 2, crh = flat ? 1 : (int)R.h / 2;
     const int lx = (lpx8 >> 3) + 4 * lcol, ly = (lpy8 >> 3) + r0 + lra;
     const int cx = (cpx8 >> 3) + 4 * cq, cy = (cpy8 >> 3) + cr0 + cra;
     uint8_t *ld = out.y + (size_t)(by + r0 + lra) * out.sy + bx + 4 * lcol;
     uint8_t *cd = (cp ? out.v : out.u) + (size_t)(cby + cr0 + cra) * out.suv + cbx + 4 * cq;
-    McRun lq, cqr;
-    lq.m = cqr.m = 0;
-    lq.add = radd;
-    cqr.add = false; /* the reference adds no chroma residual */
-    if (lon) mc_load(lq, R.y, R.sy, (int)R.w, (int)R.h, lx, ly, ln);
-    if (lon && radd) mc_res_load(lq, resid, e.w, lt, rnx, rrows, rcols, r0 + lra, 4 * lcol);
-    if (con) mc_load(cqr, cref, R.suv, (int)R.w / 2, (int)R.h / 2, cx, cy, cn);
-    if (lon) mc_emit(lq, !R.y, lpx8 & 7, lpy8 & 7, ld, out.sy, bw - 4 * lcol);
-    if (con) mc_emit(cqr, !cref, cpx8 & 7, cpy8 & 7, cd, out.suv, cbw - 4 * cq);
-    /* rows past the first CVP9_MC_RUN of a run: none with blocks of at most
-     * 64x64, whose runs are at most four rows (and the residual add relies
-     * on that) */
-    if (lon && ln > CVP9_MC_RUN)
-        mc_run(R.y, R.sy, (int)R.w, (int)R.h, lx, ly + CVP9_MC_RUN, lpx8 & 7, lpy8 & 7,
-               ln - CVP9_MC_RUN, ld + (size_t)CVP9_MC_RUN * out.sy, out.sy, bw - 4 * lcol);
-    if (con && cn > CVP9_MC_RUN)
-        mc_run(cref, R.suv, (int)R.w / 2, (int)R.h / 2, cx, cy + CVP9_MC_RUN, cpx8 & 7, cpy8 & 7,
-               cn - CVP9_MC_RUN, cd + (size_t)CVP9_MC_RUN * out.suv, out.suv, cbw - 4 * cq);
+
+    McWin lw, cw;
+    mc_win(lw, lref, lrs, lrw, lrh, lx, ly, lm);
+    mc_win(cw, cref, crs, crw, crh, cx, cy, cm);
+    /* the luma residual under the run (transform stage in resid_all mode):
+     * the 4 int16 of each row r0 + lra + t at quad column 4 * lcol of a block
+     * whose first transform's coefficients start at e.w, transforms of
+     * 4 << (lt - 2) samples, rnx to a row; rows / columns past the transform
+     * grid have none (vp9_reconstruct.c:336-363). A row is one 8-byte word
+     * when e.w is a multiple of 4 (the transforms' offsets then all are);
+     * otherwise it is read at use. */
+    const bool ral = (e.w & 3) == 0;
+    const int16_t *rbase = resid ? resid : (const int16_t *)out.y;
+    uint64_t res[CVP9_MC_RUN];
+    bool rok[CVP9_MC_RUN];
+    const GMEM int16_t *rptr[CVP9_MC_RUN];
+#pragma unroll
+    for (int t = 0; t < CVP9_MC_RUN; t++) {
+        const int rr = r0 + lra + t, c0 = 4 * lcol;
+        const int T = 1 << lt;
+        rok[t] = lon && radd && t < lm && rr < rrows && c0 < rcols;
+        rptr[t] = gm(rbase) + e.w + (size_t)(((rr >> lt) * rnx + (c0 >> lt)) << (2 * lt)) +
+                  ((rr & (T - 1)) << lt) + (c0 & (T - 1));
+        const GMEM uint64_t *pw =
+            rok[t] && ral ? (const GMEM uint64_t *)rptr[t] : (const GMEM uint64_t *)gm(out.y);
+        res[t] = *pw;
+    }
+#pragma unroll
+    for (int t = 0; t < CVP9_MC_RUN; t++) {
+        if (!rok[t]) {
+            res[t] = 0;
+        } else if (!ral) {
+            uint64_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) v |= (uint64_t)(uint16_t)rptr[t][k] << (16 * k);
+            res[t] = v;
+        }
+    }
+    mc_emit(lw, lon, lm, flat, lpx8 & 7, lpy8 & 7, res, radd, ld, out.sy, bw - 4 * lcol);
+    /* the reference adds no chroma residual */
+    mc_emit(cw, con, cm, flat, cpx8 & 7, cpy8 & 7, res, false, cd, out.suv, cbw - 4 * cq);
 }
 
 /* unit u of the batch list (the fused kernel's grid-stride form) */
@@ -537,6 +508,18 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
             for (int c = 0; c < R; c++) pred[0] |= (uint32_t)row[c] << (8 * c);
         }
     }
+    /* the lane's whole input column, every load issued before the first is
+     * used (a load per k-step behind the step's skip test would make the
+     * transform one memory round trip per row) */
+    int xs[N];
+    if (coef) {
+        const GMEM int16_t *cp = gm(coef) + j;
+#pragma unroll
+        for (int k = 0; k < N; k++) xs[k] = cp[k * N];
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) xs[k] = 0;
+    }
     double acc[R];
 #pragma unroll
     for (int i = 0; i < R; i++) acc[i] = 0.0;
@@ -544,7 +527,7 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
 
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        const int x = coef ? (int)gm(coef)[k * N + j] : 0;
+        const int x = xs[k];
         col_nz |= (x != 0);
         if (wave_ballot(x != 0)) {
             const double xd = (double)x;
@@ -573,13 +556,23 @@ __device__ __forceinline__ void idct_group(const int16_t *coef, uint8_t *dst, ui
     const int r = j;
 #pragma unroll
     for (int c = 0; c < R; c++) acc[c] = 0.0;
+    /* temp row r in chunks of KC columns, each chunk's LDS reads issued
+     * together */
+    constexpr int KC = N < 8 ? N : 8;
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-        if ((m >> k) & 1) {
-            const double u = tt[r * (N + 1) + k];
+    for (int k0 = 0; k0 < N; k0 += KC) {
+        if (((m >> k0) & ((1ull << KC) - 1)) == 0) continue;
+        double u[KC];
 #pragma unroll
-            for (int c = 0; c < R; c++)
-                acc[c] = __dadd_rn(acc[c], __dmul_rn(u, mat<N>(k * N + i0 + c)));
+        for (int kk = 0; kk < KC; kk++) u[kk] = tt[r * (N + 1) + k0 + kk];
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+            const int k = k0 + kk;
+            if ((m >> k) & 1) {
+#pragma unroll
+                for (int c = 0; c < R; c++)
+                    acc[c] = __dadd_rn(acc[c], __dmul_rn(u[kk], mat<N>(k * N + i0 + c)));
+            }
         }
     }
 
@@ -643,7 +636,9 @@ __device__ __forceinline__ void residual_unit(const DBatch &B, uint32_t unit, do
 {
     using RS = ResShape<N, S>;
     constexpr int C = RS::C, G = RS::G, GPW = RS::GPW;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    /* the wave index made wave-uniform, so the matrix columns a wave owns come
+     * through scalar loads */
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gi = wave / S, sub = wave - gi * S;
     const uint32_t grp = unit * GPW + gi;
     const uint32_t *pre = B.pre_grp[C];
@@ -810,7 +805,8 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
 __global__ __launch_bounds__(256) void k_nbr(DBatch B)
 {
     const uint32_t total = B.pre_intra[B.njobs];
-    const uint32_t w0 = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 6) * 2;
+    const uint32_t w0 =
+        __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 6) * 2);
     if (w0 >= total) return;
     const uint32_t jh = find_job_wave(B.pre_intra, B.njobs, w0); /* the wave's first block */
     const uint32_t wv = w0 + ((threadIdx.x & 63) >> 5);
@@ -858,59 +854,7 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
     return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
 }
 
-/* A neighbour cell lying in an inter block, read from the frame (final
- * once motion compensation and the inter residuals are done): 4 Y samples
- * and 2 each of U and V in the layout of a resolved cell; kind 0 above
- * (cell q), 1 left (cell q), 2 above-left. Rows are 64-byte aligned and
- * x + 4q < W, so the dword and the chroma pair of an above cell stay
- * inside their rows. */
-__device__ __forceinline__ uint64_t inter_cell(const DFrame &F, int x, int y, int bh, int kind, int q)
-{
-    const GMEM uint8_t *const fy = gm(F.y), *const fu = gm(F.u), *const fv = gm(F.v);
-    const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
-    uint64_t val = 0;
-    if (kind == 0) {
-        val = *(const GMEM uint32_t *)(fy + (size_t)(y - 1) * F.sy + x + 4 * q);
-        const size_t co = (size_t)(cby - 1) * F.suv + cbx + 2 * q;
-        val |= (uint64_t)*(const GMEM uint16_t *)(fu + co) << 32;
-        val |= (uint64_t)*(const GMEM uint16_t *)(fv + co) << 48;
-    } else if (kind == 1) {
-        for (int e = 0; e < 4 && 4 * q + e < bh; e++)
-            val |= (uint64_t)fy[(size_t)(y + 4 * q + e) * F.sy + x - 1] << (8 * e);
-        for (int e = 0; e < 2 && 2 * q + e < cbh; e++) {
-            const size_t co = (size_t)(cby + 2 * q + e) * F.suv + cbx - 1;
-            val |= (uint64_t)fu[co] << (32 + 8 * e);
-            val |= (uint64_t)fv[co] << (48 + 8 * e);
-        }
-    } else {
-        const size_t co = (size_t)(cby - 1) * F.suv + cbx - 1;
-        val = (uint64_t)fy[(size_t)(y - 1) * F.sy + x - 1] << 24 | (uint64_t)fu[co] << 40 |
-              (uint64_t)fv[co] << 56;
-    }
-    return val;
-}
 
-/* The cell of an intra neighbour from its edge record: one load of each
- * of its Y, U and V words; stale when a word does not carry the frame's
- * epoch yet (the neighbour has not published it). */
-__device__ __forceinline__ uint64_t edge_try(const uint64_t *edge, uint32_t epoch, uint64_t d,
-                                             bool &stale)
-{
-    const uint32_t o = (uint32_t)d;
-    const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
-    const uint32_t c = (uint32_t)(d >> 40) & 31;
-    const GMEM uint64_t *rec = gm(edge) + (size_t)o * CVP9_EDGE_WORDS;
-    const uint64_t wy = __hip_atomic_load(rec + (side ? 16 : 0) + lw, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t wu = __hip_atomic_load(rec + (side ? 40 : 32) + (c >> 2), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t wv = __hip_atomic_load(rec + (side ? 56 : 48) + (c >> 2), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    stale = (uint32_t)(wy >> 32) != epoch || (uint32_t)(wu >> 32) != epoch ||
-            (uint32_t)(wv >> 32) != epoch;
-    const uint32_t sh = 8 * (c & 3);
-    return (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
-}
 
 __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc)
 {
@@ -918,6 +862,91 @@ __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc
     if (mode == 1) return a;
     if (mode == 2) return l;
     return clampi(a + l - tl, 0, 255);
+}
+
+/* One neighbour cell of an intra block (thread c of the gather; kind 0
+ * above, cell q, 1 left, cell q, 2 above-left), in the layout of a resolved
+ * cell (4 Y, 2 U, 2 V samples):
+ *   pend  (an earlier intra block): from the owner's edge record, one word
+ *         each of Y, U and V; stale when a word does not carry the frame's
+ *         epoch yet (the owner has not published it);
+ *   inter (an inter block): from the frame, final once motion compensation
+ *         and the inter residuals are done;
+ *   else  the resolved value d.
+ * Every load is issued for every thread, at an address inside the edge
+ * records or the frame, and the selection follows the loads: a load inside
+ * a branch would be waited for before the branch ends, making the gather
+ * one memory round trip per kind of cell instead of one in all. Block
+ * positions are multiples of 4, so each sample lies in the dword named
+ * below; rows are 64-byte aligned, so the dword lies inside the row pitch. */
+__device__ __forceinline__ uint64_t cell_gather(const uint64_t *edge, uint32_t epoch, uint32_t self,
+                                                const DFrame &F, int x, int y, int bh, int kind,
+                                                int q, bool pend, bool inter, uint64_t d,
+                                                bool &stale)
+{
+    /* edge words (a thread without an edge cell reads the block's own
+     * record, and discards it) */
+    const uint64_t ed = pend ? d : (uint64_t)self;
+    const uint32_t o = (uint32_t)ed;
+    const uint32_t side = (uint32_t)(ed >> 32) & 1, lw = (uint32_t)(ed >> 33) & 15;
+    const uint32_t cc = (uint32_t)(ed >> 40) & 31;
+    const GMEM uint64_t *rec = gm(edge) + (size_t)o * CVP9_EDGE_WORDS;
+    const uint64_t wy = __hip_atomic_load(rec + (side ? 16 : 0) + lw, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t wu = __hip_atomic_load(rec + (side ? 40 : 32) + (cc >> 2), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t wv = __hip_atomic_load(rec + (side ? 56 : 48) + (cc >> 2), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+
+    /* frame dwords, four of luma and two of each chroma plane, and the byte
+     * each contributes (shift, and whether it is one of the cell's samples):
+     *   above (kind 0): row y-1, the cell's 4 Y samples and 2 U / V samples,
+     *     every luma dword the same one, every chroma dword likewise;
+     *   left (kind 1): rows of the cell, the sample left of the block in each
+     *     (rows past the block's end have none: 0);
+     *   above-left (kind 2): row y-1, the single samples in bytes 3, 5, 7.
+     * All loads are used the same way whatever the kind, so none is moved
+     * into a branch of its own. A thread without an inter cell reads row 0,
+     * column 0 and keeps nothing. */
+    const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
+    const int colY = kind == 0 ? x + 4 * q : x - 4;
+    const int cC = kind == 0 ? cbx + 2 * q : cbx - 1;
+    const int colC = cC & ~3, shC = 8 * (cC & 3);
+    uint32_t fy[4], fu[2], fv[2];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int r = kind == 1 ? y + min(4 * q + e, bh - 1) : y - 1;
+        const size_t o = inter ? (size_t)r * F.sy + colY : 0;
+        fy[e] = *(const GMEM uint32_t *)(gm(F.y) + o);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const int r = kind == 1 ? cby + min(2 * q + e, cbh - 1) : cby - 1;
+        const size_t o = inter ? (size_t)r * F.suv + colC : 0;
+        fu[e] = *(const GMEM uint32_t *)(gm(F.u) + o);
+        fv[e] = *(const GMEM uint32_t *)(gm(F.v) + o);
+    }
+    uint64_t v_inter = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint32_t sh = kind == 0 ? 8 * e : 24;
+        const bool keep = kind == 0 || (kind == 1 ? 4 * q + e < bh : e == 3);
+        v_inter |= (uint64_t)(keep ? (fy[e] >> sh) & 0xff : 0u) << (8 * e);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const uint32_t sh = kind == 0 ? shC + 8 * e : shC;
+        const bool keep = kind == 0 || (kind == 1 ? 2 * q + e < cbh : e == 1);
+        v_inter |= (uint64_t)(keep ? (fu[e] >> sh) & 0xff : 0u) << (32 + 8 * e);
+        v_inter |= (uint64_t)(keep ? (fv[e] >> sh) & 0xff : 0u) << (48 + 8 * e);
+    }
+
+    const uint32_t sh = 8 * (cc & 3);
+    const uint64_t v_edge =
+        (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
+    stale = pend && ((uint32_t)(wy >> 32) != epoch || (uint32_t)(wu >> 32) != epoch ||
+                     (uint32_t)(wv >> 32) != epoch);
+    return pend ? v_edge : (inter ? v_inter : d);
 }
 
 /* Synchronises the threads of one intra worker: a whole workgroup, or a
@@ -968,7 +997,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     /* block index, block record, dependency mask and this thread's cell
      * all come from the neighbour record, in one round of loads */
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
-    const uint64_t cell = tid < CVP9_NB_CELLS ? rec[tid] : 0;
+    /* (threads past the cells read the last one, and never use it) */
+    const uint64_t cell = rec[min(tid, CVP9_NB_CELLS - 1)];
     const uint64_t dmask = rec[CVP9_NB_MASK];
     const uint64_t imask = rec[CVP9_NB_IMASK];
     const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
@@ -995,12 +1025,13 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         return jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
                ((r & (T - 1)) << lt) + (c0 & (T - 1));
     };
-    int16_t rq0[4] = {0, 0, 0, 0};
+    /* loads without a branch around them (a thread without residual reads
+     * the neighbour record and ignores it): a branch would wait for them */
     const GMEM int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
-    if (rs0) {
+    const GMEM int16_t *rp0 = rs0 ? rs0 : (const GMEM int16_t *)rec;
+    int16_t rq0[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) rq0[e] = rs0[e];
-    }
+    for (int e = 0; e < 4; e++) rq0[e] = rp0[e];
     /* residual at one luma sample (0 where the block has none) */
     auto resid_at = [&](int r, int c) -> int {
         if (r >= rrows || c >= rcols) return 0;
@@ -1009,9 +1040,14 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     };
     /* and this thread's first luma edge sample: bottom row, then right
      * column */
-    const int eres0 = !EARLY ? 0
-                             : tid < bw ? resid_at(bh - 1, tid)
-                                        : (tid < bw + bh ? resid_at(tid - bw, bw - 1) : 0);
+    const int er = tid < bw ? bh - 1 : tid - bw, ec = tid < bw ? tid : bw - 1;
+    const bool eres_ok = EARLY && tid < bw + bh && er < rrows && ec < rcols;
+    const int eres0 =
+        !EARLY ? 0
+               : (int)(eres_ok ? jres + k.coeff +
+                                     (size_t)(((er >> lt) * (int)k.nx + (ec >> lt)) << (2 * lt)) +
+                                     ((er & (T - 1)) << lt) + (ec & (T - 1))
+                               : (const GMEM int16_t *)rec)[0];
 
     /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
      *    the neighbour stage; cells of earlier intra blocks from their
@@ -1021,12 +1057,12 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         const int c = tid;
         const int kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
         const bool present = kind == 0 ? y > 0 : (kind == 1 ? x > 0 : (x > 0 && y > 0));
-        const bool pend = c <= ncx + ncy && present && ((dmask >> c) & 1);
-        uint64_t v = present ? cell : 0x7f7f7f7f7f7f7f7full;
-        bool stale = false;
-        if (pend) v = edge_try(jedge, epoch, cell, stale);
-        else if (present && c <= ncx + ncy && ((imask >> c) & 1))
-            v = inter_cell(out, x, y, bh, kind, kind == 0 ? c : c - ncx);
+        const bool incell = c <= ncx + ncy && present;
+        const bool pend = incell && ((dmask >> c) & 1);
+        const bool inter = incell && !pend && ((imask >> c) & 1);
+        bool stale;
+        uint64_t v = cell_gather(jedge, epoch, b, out, x, y, bh, kind, kind == 0 ? c : c - ncx, pend,
+                                 inter, present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
         /* a neighbour still running: poll the cell's words until it has
          * published them */
         if (stale) v = cell_from_edges(jedge, epoch, jerr, cell, B.spin_limit);
@@ -1093,7 +1129,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             if (p == 0) {
                 const int r = dir ? e : bh - 1, c = dir ? bw - 1 : e;
                 v = intra_pred(k.ymode, S.nb[0][0][c], S.nb[0][1][r], S.tl[0], dc0);
-                const int rv = i == tid ? eres0 : resid_at(r, c);
+                const int rv = i == tid ? (eres_ok ? eres0 : 0) : resid_at(r, c);
                 v = clampi(v + rv, 0, 255);
             } else {
                 const int r = dir ? e : cbh - 1, c = dir ? cbw - 1 : e;
@@ -1361,13 +1397,29 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
      * are 64-byte aligned and every dword that meets [c0, c1) lies inside
      * the row's pitch) */
     const int a0 = c0 & ~3, nd = (c1 - a0 + 3) >> 2;
-    for (int i = tid; i < nr * nd; i += 256) {
-        const int r = i / nd, p = a0 + 4 * (i - r * nd);
-        const uint32_t v = *(const GMEM uint32_t *)(gm(Y) + (size_t)(r0 + r) * sy + p);
+    /* at most 64 rows of 17 dwords: five per thread, all loads issued
+     * before the first is used */
+    constexpr int LF_ITEMS = 5;
+    static_assert(LF_ITEMS * 256 >= 64 * 17, "window dwords per thread");
+    /* (past the window's end a thread loads the window's last dword again
+     * and discards it: unconditional loads need no wait before the join) */
+    uint32_t lv[LF_ITEMS];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int c = p + e - c0;
-            if (c >= 0 && c < nc) tile[r][c] = (uint8_t)(v >> (8 * e));
+    for (int u = 0; u < LF_ITEMS; u++) {
+        const int i = min(tid + 256 * u, nr * nd - 1);
+        const int r = i / nd, p = a0 + 4 * (i - r * nd);
+        lv[u] = *(const GMEM uint32_t *)(gm(Y) + (size_t)(r0 + r) * sy + p);
+    }
+#pragma unroll
+    for (int u = 0; u < LF_ITEMS; u++) {
+        const int i = tid + 256 * u;
+        if (i < nr * nd) {
+            const int r = i / nd, p = a0 + 4 * (i - r * nd);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int c = p + e - c0;
+                if (c >= 0 && c < nc) tile[r][c] = (uint8_t)(lv[u] >> (8 * e));
+            }
         }
     }
     __syncthreads();
