@@ -76,18 +76,20 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *pre, uint32_t n, ui
  * rounds of loads replace the binary search's dependent chain. */
 __device__ __forceinline__ uint32_t find_job_wave(const uint32_t *pre, uint32_t n, uint32_t w)
 {
+    /* loads with no branch around them (lanes past n read entry n - 1 and
+     * ignore it), so nothing waits before the ballot */
     const uint32_t lane = __lane_id();
     if (n <= 64) {
-        const bool le = lane < n && gm(pre)[lane] <= w;
-        return (uint32_t)__builtin_popcountll(__ballot(le)) - 1;
+        const uint32_t v = gm(pre)[min(lane, n - 1)];
+        return (uint32_t)__builtin_popcountll(__ballot(lane < n && v <= w)) - 1;
     }
     const uint32_t step = (n + 63) >> 6;
     const uint32_t i = lane * step;
-    const bool le = i < n && gm(pre)[i] <= w;
-    const uint32_t base = ((uint32_t)__builtin_popcountll(__ballot(le)) - 1) * step;
+    const uint32_t v = gm(pre)[min(i, n - 1)];
+    const uint32_t base = ((uint32_t)__builtin_popcountll(__ballot(i < n && v <= w)) - 1) * step;
     const uint32_t k = base + lane;
-    const bool le2 = lane < step && k < n && gm(pre)[k] <= w;
-    return base + (uint32_t)__builtin_popcountll(__ballot(le2)) - 1;
+    const uint32_t v2 = gm(pre)[min(k, n - 1)];
+    return base + (uint32_t)__builtin_popcountll(__ballot(lane < step && k < n && v2 <= w)) - 1;
 }
 
 /* Workgroup b of n runs on XCD b % 8 (the dispatcher deals workgroups out
@@ -1025,8 +1027,9 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         return jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
                ((r & (T - 1)) << lt) + (c0 & (T - 1));
     };
-    /* loads without a branch around them (a thread without residual reads
-     * the neighbour record and ignores it): a branch would wait for them */
+    /* this thread's first luma quad of residual, loaded with no branch
+     * around it (a thread without residual reads the neighbour record and
+     * ignores it): a branch would wait for it */
     const GMEM int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
     const GMEM int16_t *rp0 = rs0 ? rs0 : (const GMEM int16_t *)rec;
     int16_t rq0[4];
@@ -1389,7 +1392,7 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
     const int c0 = max(64 * wx - 2, 0), c1 = min(64 * wx + 62, W);
     const int nr = r1 - r0, nc = c1 - c0;
     const int thr = J.lf_thr;
-    uint8_t *Y = J.out.y;
+    GMEM uint8_t *const Y = gm(J.out.y);
     const uint32_t sy = J.out.sy;
     const int tid = threadIdx.x;
 
@@ -1398,17 +1401,17 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
      * the row's pitch) */
     const int a0 = c0 & ~3, nd = (c1 - a0 + 3) >> 2;
     /* at most 64 rows of 17 dwords: five per thread, all loads issued
-     * before the first is used */
+     * before the first is used (past the window's end a thread loads the
+     * window's last dword again and discards it: unconditional loads need no
+     * wait before the join) */
     constexpr int LF_ITEMS = 5;
     static_assert(LF_ITEMS * 256 >= 64 * 17, "window dwords per thread");
-    /* (past the window's end a thread loads the window's last dword again
-     * and discards it: unconditional loads need no wait before the join) */
     uint32_t lv[LF_ITEMS];
 #pragma unroll
     for (int u = 0; u < LF_ITEMS; u++) {
         const int i = min(tid + 256 * u, nr * nd - 1);
         const int r = i / nd, p = a0 + 4 * (i - r * nd);
-        lv[u] = *(const GMEM uint32_t *)(gm(Y) + (size_t)(r0 + r) * sy + p);
+        lv[u] = *(const GMEM uint32_t *)(Y + (size_t)(r0 + r) * sy + p);
     }
 #pragma unroll
     for (int u = 0; u < LF_ITEMS; u++) {
@@ -1455,7 +1458,7 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
                 mask |= 1u << e;
             }
         }
-        GMEM uint8_t *d = gm(Y) + (size_t)(r0 + r) * sy + p;
+        GMEM uint8_t *d = Y + (size_t)(r0 + r) * sy + p;
         if (mask == 15) {
             *(GMEM uint32_t *)d = v;
         } else {
@@ -1466,10 +1469,14 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
     }
 }
 
+/* Windows next to each other share the cache lines of their overlapping
+ * columns (a window's rows start 4 bytes left of its first column, and its
+ * first rows are the last ones of the window above), so each XCD takes one
+ * contiguous run of the list and neighbours meet in its L2. */
 __global__ __launch_bounds__(256) void k_lf(DBatch B)
 {
     __shared__ uint8_t tile[64][65];
-    lf_item(B, blockIdx.x, tile);
+    lf_item(B, xcd_block(blockIdx.x, gridDim.x), tile);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1502,7 +1509,7 @@ __device__ __forceinline__ void nv12_item(const DBatch &B, uint32_t w)
 
 __global__ __launch_bounds__(256) void k_nv12(DBatch B)
 {
-    nv12_item(B, blockIdx.x);
+    nv12_item(B, xcd_block(blockIdx.x, gridDim.x)); /* neighbouring tiles share row lines */
 }
 
 /* ------------------------------------------------------------------------ */
