@@ -12,7 +12,10 @@ Default run (N=1):
   * roofline of the dominant kernel from HIP events on the engine's stream
     over the timed region, against MI355X HBM peak.
   * cpu_baseline: the reference's own CPU reconstruction (oracle/_ref, one
-    thread) on a bounded sample of the same 1080p sequence.
+    thread) on a bounded sample of the same 1080p sequence, with per-frame
+    latency percentiles; multi_stream.cpu_baseline_all_cores: the same code
+    on the first frames of every stream of the mix, streams dealt over every
+    host core the process may use (at most 16).
 Prints ONE JSON line.
 """
 import argparse
@@ -434,17 +437,54 @@ def config2(P, args):
 
 
 def cpu_baseline(P, frames, n):
+    """The reference's own C (oracle/_ref, else the restatement) on the first
+    n frames of the 1080p sequence, one thread, one call per frame on one
+    decoder (its reference pool carried over): frames/s and the per-frame
+    latency percentiles."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     sample = frames[:n]
     kind = "reference" if pyoracle.have_ref() else "port"
-    run = pyoracle.run_reference if kind == "reference" else pyoracle.run_oracle
-    t0 = time.perf_counter()
-    run(sample)
-    dt = time.perf_counter() - t0
+    lat = pyoracle.time_per_frame(sample, kind == "reference")
+    dt = float(np.sum(lat))
     return {"value": round(len(sample) / dt, 3), "unit": "frames/s", "cores": 1, "kind": kind,
+            "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 2),
             "sample": f"first {len(sample)} frames of the 1920x1080 preset-1 sequence, "
                       f"1 thread, {dt:.1f} s"}
+
+
+def cpu_baseline_mix(P, args):
+    """The 64-stream mix on every host core the process may use (at most 16):
+    streams dealt round robin over threads, each thread decoding its streams
+    one after another with the reference's own C (one decoder per stream; the
+    reference has no parallelism inside a frame). A bounded sample: the first
+    args.cpu_mix_frames frames of every stream."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    per_res = args.streams // 4
+    F = args.cpu_mix_frames
+    layout = [(w, h) for (w, h) in MIX for _ in range(per_res)]
+    specs = [(3, w, h, 7000 + sid, i) for sid, (w, h) in enumerate(layout) for i in range(F)]
+    frames = gen_frames(P, specs)
+    streams = [frames[k * F:(k + 1) * F] for k in range(len(layout))]
+    kind = "reference" if pyoracle.have_ref() else "port"
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    run = pyoracle.run_reference if kind == "reference" else pyoracle.run_oracle
+
+    def worker(t):
+        for k in range(t, len(streams), cores):
+            run(streams[k])
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(worker, range(cores)))
+    dt = time.perf_counter() - t0
+    px = sum(w * h for w, h in layout) * F
+    return {"value": round(px / (1920 * 1080) / dt, 2), "unit": "1080p-equivalent frames/s",
+            "cores": cores, "kind": kind,
+            "sample": f"first {F} frames of each of the {len(layout)} streams, "
+                      f"{cores} threads, {dt:.1f} s"}
 
 
 def main():
@@ -461,6 +501,8 @@ def main():
                     help="batches per multi-stream step (two disjoint stream groups overlap on "
                          "the engine's two queues; more groups are not supported)")
     ap.add_argument("--cpu-frames", type=int, default=100)
+    ap.add_argument("--cpu-mix-frames", type=int, default=20,
+                    help="frames per stream of the all-core CPU baseline of the 64-stream mix")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-stream phase (for profiling the others); value is null")
     ap.add_argument("--mtune", type=int, nargs=7, default=None,
@@ -544,6 +586,7 @@ def main():
     if serve:
         log("serving:", json.dumps(serve))
     cpu = None if args.no_cpu or ss is None else cpu_baseline(P, ss["frames"], args.cpu_frames)
+    cpu_mix = None if args.no_cpu or not multi else cpu_baseline_mix(P, args)
 
     line = {
         "metric": METRIC,
@@ -581,7 +624,8 @@ def main():
                 round(multi["groups_p1080_equiv_per_s"], 1),
             "stages_one_batch": multi["stages"],
             "tune": multi["tune"],
-            "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2) if value else None},
+            "vs_single_stream": round(multi["p1080_equiv_per_s"] / value, 2) if value else None,
+            "cpu_baseline_all_cores": cpu_mix},
         "config2_2160p": c2,
         "serving": serve,
         "cpu_baseline": cpu,

@@ -100,6 +100,29 @@ def _run(L, prefix, frames):
     return res
 
 
+def time_per_frame(frames, use_ref=True):
+    """Seconds per frame of a sequence decoded one call per frame on one
+    decoder (the reference's own C, or the restatement), the reference pool
+    carried from frame to frame."""
+    import time
+    L, prefix = (reference(), "ref") if use_ref else (oracle(), "orc")
+    d = getattr(L, prefix + "_decoder_new")()
+    run = getattr(L, prefix + "_decoder_run")
+    lat = []
+    try:
+        for f in frames:
+            out = np.zeros(i420_size(f.pf.hdr.width, f.pf.hdr.height), np.uint8)
+            arr = (C.c_void_p * 1)(C.cast(f.ptr, C.c_void_p))
+            t0 = time.perf_counter()
+            done = run(d, arr, 1, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+            lat.append(time.perf_counter() - t0)
+            if done != 1:
+                raise RuntimeError(f"{prefix} decode failed")
+    finally:
+        getattr(L, prefix + "_decoder_free")(d)
+    return lat
+
+
 def run_oracle(frames):
     """Packed I420 output of every frame, CPU restatement, fresh pool."""
     return _run(oracle(), "orc", frames)
