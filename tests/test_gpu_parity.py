@@ -63,6 +63,20 @@ def test_small_sequences(pkg, orc, gpu_ready, preset, w, h, n):
     compare(pkg, orc, frames, f"preset{preset} {w}x{h} n={n}")
 
 
+# Frames smaller than a block: chroma planes narrower than the 5 samples one
+# motion-compensation quad reads (its source window is clamped into the row),
+# planes one sample high, widths that are not multiples of 4, and motion
+# vectors that point far outside such planes (preset 2: +-256 px).
+TINY = [(2, 8, 8, 4), (0, 6, 10, 4), (1, 18, 2, 3), (2, 2, 2, 3), (1, 10, 6, 4), (2, 26, 14, 4)]
+
+
+@pytest.mark.parametrize("preset,w,h,n", TINY)
+def test_tiny_frames(pkg, orc, gpu_ready, preset, w, h, n):
+    p = pkg.synth_params(preset, w, h, 3000 + preset * 7 + w)
+    frames = [pkg.SynthFrame(p, i) for i in range(n)]
+    compare(pkg, orc, frames, f"tiny preset{preset} {w}x{h} n={n}")
+
+
 def test_skip_blocks_and_sharpness(pkg, orc, gpu_ready):
     for sharp in (0, 3, 6):
         p = pkg.synth_params(1, 200, 136, 55 + sharp, skip_permille=300, sharpness=sharp)
