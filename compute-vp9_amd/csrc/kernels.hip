@@ -76,6 +76,9 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *pre, uint32_t n, ui
  * rounds of loads replace the binary search's dependent chain. */
 __device__ __forceinline__ uint32_t find_job_wave(const uint32_t *pre, uint32_t n, uint32_t w)
 {
+    /* a batch of one job (a single stream) needs no lookup: its kernels
+     * start on the job's descriptor one memory round trip earlier */
+    if (n == 1) return 0;
     /* loads with no branch around them (lanes past n read entry n - 1 and
      * ignore it), so nothing waits before the ballot */
     const uint32_t lane = __lane_id();
