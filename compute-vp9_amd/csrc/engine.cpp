@@ -587,6 +587,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
         c[P_INTER] = (J.n_mc_a + CVP9_MC_WG_UNITS - 1) / CVP9_MC_WG_UNITS * CVP9_MC_WG_UNITS;
         c[P_INTER2] = (J.n_mc - J.n_mc_a + CVP9_MC_WG_UNITS - 1) / CVP9_MC_WG_UNITS * CVP9_MC_WG_UNITS;
         for (int k = 0; k < 4; k++) c[P_G0 + k] = div_up(J.cls_ntx[k], 64u >> (2 + k));
+        /* 4x4: whole waves of four groups (64 transforms, one per lane) per job */
+        c[P_G0] = div_up(c[P_G0], 4u) * 4u;
         c[P_INTRA] = J.n_intra;
         c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;
         c[P_NV12] = J.nv12 ? div_up(J.out.w, 64) * div_up(J.out.h, 64) : 0;

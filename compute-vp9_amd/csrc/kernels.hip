@@ -669,6 +669,78 @@ __global__ __launch_bounds__(256) void k_residual(DBatch B)
     residual_unit<N, S>(B, blockIdx.x, lds);
 }
 
+/* 4x4 transforms, one per lane: a wave takes four groups of 16 (each job's
+ * group count is padded to a multiple of four, so a wave's groups belong to
+ * one job), a lane its whole transform. Both passes run in registers and
+ * take every term, in the reference's order (the same sums the skipping
+ * form computes, see above); there is no LDS and no barrier, and a wave
+ * holds four times the transforms of the column-per-lane form, so it pays
+ * its memory round trips once for 64 transforms. */
+__global__ __launch_bounds__(256) void k_res4_lanes(DBatch B)
+{
+    const uint32_t *pre = B.pre_grp[0];
+    const uint32_t g0 =
+        __builtin_amdgcn_readfirstlane((blockIdx.x * 4 + (threadIdx.x >> 6)) * 4);
+    if (g0 >= pre[B.njobs]) return; /* uniform per wave */
+    const uint32_t j = find_job_wave(pre, B.njobs, g0);
+    const DJob &J = B.jobs[j];
+    const uint32_t txi = (g0 - pre[j]) * 16 + __lane_id();
+    if (txi >= J.cls_ntx[0]) return; /* no barrier below */
+    const uint2 t = gload2((const uint2 *)J.tx_tab[0] + txi);
+    const bool to_res = (t.y >> 31) || B.resid_all;
+    const GMEM int16_t *cp = gm(J.coeffs) + t.x;
+    int in[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) in[i] = cp[i];
+    /* the prediction an inter transform adds to, loaded with the
+     * coefficients by every lane (a transform's rows lie inside the frame,
+     * rows are 64-byte aligned and x is a multiple of 4; a transform whose
+     * residual is kept ignores it) */
+    GMEM uint8_t *dst = gm(J.out.y) + (size_t)((t.y >> 14) & 0x3fff) * J.out.sy + (t.y & 0x3fff);
+    const uint32_t sy = J.out.sy;
+    uint32_t pred[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) pred[r] = *(const GMEM uint32_t *)(dst + r * sy);
+    /* pass 1: temp[i][c] = sum_k M[k][i] * in[k][c] */
+    double tmp[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                acc = __dadd_rn(acc, __dmul_rn(mat<4>(k * 4 + i), (double)in[k * 4 + c]));
+            tmp[i * 4 + c] = acc;
+        }
+    /* pass 2: out[r][c] = sum_k temp[r][k] * M[k][c] */
+    int out[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc = __dadd_rn(acc, __dmul_rn(tmp[r * 4 + k], mat<4>(k * 4 + c)));
+            out[r * 4 + c] = round_clamp16(acc);
+        }
+    if (to_res) {
+        GMEM int16_t *o = gm(J.resid) + t.x;
+#pragma unroll
+        for (int i = 0; i < 16; i++) o[i] = (int16_t)out[i];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                w |= (uint32_t)clampi((int)((pred[r] >> (8 * c)) & 0xff) + out[r * 4 + c], 0, 255)
+                     << (8 * c);
+            *(GMEM uint32_t *)(dst + r * sy) = w;
+        }
+    }
+}
+
 /* Every transform size in one launch, for batches too small to fill the
  * device with one size at a time: workgroups [0, wg.x) take 4x4, then
  * [wg.x, wg.y) 8x8, [wg.y, wg.z) 16x16 and the rest 32x32, each with the
@@ -1686,6 +1758,10 @@ static hipError_t launch_residual(const DBatch &b, const StageCounts &n, int c, 
 {
     const uint32_t g = n.grp[c];
     if (!g) return hipSuccess;
+    if constexpr (N == 4) { /* one transform per lane, 16 groups per workgroup */
+        hipLaunchKernelGGL(k_res4_lanes, dim3(div_up(g, 16)), dim3(256), 0, s, b);
+        return hipGetLastError();
+    }
     switch (n.tune.res_split[c]) {
     case 1: hipLaunchKernelGGL((k_residual<N, 1>), dim3(div_up(g, 4)), dim3(256), 0, s, b); break;
     case 2: hipLaunchKernelGGL((k_residual<N, 2>), dim3(div_up(g, 2)), dim3(256), 0, s, b); break;
