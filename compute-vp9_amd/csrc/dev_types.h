@@ -160,8 +160,10 @@ struct DBatch {
     const uint32_t *pre_lf;        /* deblocking windows */
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
-    uint32_t       *jticket;       /* per-job intra ticket counters, set by the descriptor copy
-                                      to where the persistent workers start */
+    uint32_t       *jticket;       /* per-job intra ticket counters (zeroed by the descriptor
+                                      copy), for jobs no persistent worker starts on */
+    const uint32_t *jstart;        /* per job, the intra-list entry where the persistent
+                                      workers start (after the level launches) */
     uint32_t       *bar;           /* grid-barrier counter of the fused kernel (zeroed likewise) */
     /* leading levels of the intra stage, per dependency level l (rows of
      * njobs + 1): exclusive prefixes over jobs of the small (at most 16x16)

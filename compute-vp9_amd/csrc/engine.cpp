@@ -205,7 +205,7 @@ static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint
     }
     const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
     thread_local std::vector<uint32_t> map; /* level + 1 per 4x4 cell, zero between frames */
-    thread_local std::vector<uint32_t> lev, cnt, tmp;
+    thread_local std::vector<uint32_t> lev, cnt, tmp, hgt, pos;
     if (map.size() < (size_t)ow * oh) map.assign((size_t)ow * oh, 0);
     lev.resize(ni);
     uint32_t depth = 0;
@@ -225,13 +225,46 @@ static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint
         for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = m + 1;
         for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = m + 1;
     }
-    /* clear what this frame wrote, then a stable counting sort by level */
+    /* clear what this frame wrote */
     for (uint32_t i = 0; i < ni; i++) {
         const DBlock &k = blocks[intra[i]];
         const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
         const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
         for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = 0;
         for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = 0;
+    }
+    /* height: the longest chain of later intra blocks that depend on a
+     * block (1 without any), from a pass in reverse list order. A later
+     * block leaves its height on the cells it reads (the row above it from
+     * the above-left cell on, and the column left of it); a block's
+     * dependents are exactly the later blocks that read its bottom row or
+     * right column, and blocks earlier in the list write their marks only
+     * after it has been scanned. */
+    hgt.resize(ni);
+    for (uint32_t i = ni; i-- > 0;) {
+        const DBlock &k = blocks[intra[i]];
+        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
+        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
+        uint32_t h = 0;
+        for (uint32_t cx = cx0; cx <= cx1; cx++) h = std::max(h, map[(size_t)cy1 * ow + cx]);
+        for (uint32_t cy = cy0; cy <= cy1; cy++) h = std::max(h, map[(size_t)cy * ow + cx1]);
+        hgt[i] = h + 1;
+        if (cy0 > 0) {
+            uint32_t *row = &map[(size_t)(cy0 - 1) * ow];
+            for (uint32_t cx = cx0 ? cx0 - 1 : 0; cx <= cx1; cx++) row[cx] = std::max(row[cx], h + 1);
+        }
+        if (cx0 > 0)
+            for (uint32_t cy = cy0; cy <= cy1; cy++)
+                map[(size_t)cy * ow + cx0 - 1] = std::max(map[(size_t)cy * ow + cx0 - 1], h + 1);
+    }
+    for (uint32_t i = 0; i < ni; i++) { /* clear the marks */
+        const DBlock &k = blocks[intra[i]];
+        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
+        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
+        if (cy0 > 0)
+            for (uint32_t cx = cx0 ? cx0 - 1 : 0; cx <= cx1; cx++) map[(size_t)(cy0 - 1) * ow + cx] = 0;
+        if (cx0 > 0)
+            for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx0 - 1] = 0;
     }
     /* stable counting sort by (level, size class): small blocks of a level
      * first, so a level launch gives them 64-thread workers four to a
@@ -245,8 +278,22 @@ static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint
     levels.resize(depth + 1);
     for (uint32_t l = 0; l <= depth; l++) levels[l] = cnt[2 * l];
     tmp.resize(ni);
-    for (uint32_t i = 0; i < ni; i++)
-        tmp[cnt[2 * lev[i] + (intra_small(blocks[intra[i]]) ? 0 : 1)]++] = intra[i];
+    pos.resize(ni);
+    for (uint32_t i = 0; i < ni; i++) {
+        const uint32_t at = cnt[2 * lev[i] + (intra_small(blocks[intra[i]]) ? 0 : 1)]++;
+        tmp[at] = intra[i];
+        pos[at] = i;
+    }
+    /* within a (level, size class) bucket, the blocks heading the longest
+     * chains first (then list order): tickets go out in list order, so the
+     * dependency chain that bounds the stage starts at once instead of
+     * behind the level's other blocks */
+    for (uint32_t k = 0; k < 2 * depth; k++) {
+        const uint32_t a = k ? cnt[k - 1] : 0, e = cnt[k];
+        std::stable_sort(pos.begin() + a, pos.begin() + e,
+                         [&](uint32_t u, uint32_t v) { return hgt[u] > hgt[v]; });
+    }
+    for (uint32_t i = 0; i < ni; i++) tmp[i] = intra[pos[i]];
     memcpy(intra, tmp.data(), (size_t)ni * 4);
 }
 
@@ -558,7 +605,7 @@ size_t StreamState::resident_bytes() const
 
 size_t BatchBuilder::bytes_for(uint32_t njobs)
 {
-    return align256((uint64_t)njobs * sizeof(DJob)) + align256(12ull * (njobs + 1) * 4) + 256 +
+    return align256((uint64_t)njobs * sizeof(DJob)) + align256(13ull * (njobs + 1) * 4) + 256 +
            3ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
 }
 
@@ -570,7 +617,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     uint32_t *pre = (uint32_t *)(host + pre_off);
     const uint32_t stride = njobs + 1;
     enum { P_FILL, P_BLK, P_INTER, P_INTER2, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
-    static_assert(P_N + 1 <= 12, "prefix rows and the ticket row fit the descriptor");
+    static_assert(P_N + 2 <= 13, "prefix rows, the ticket row and the start row fit the descriptor");
     for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
 
     for (uint32_t i = 0; i < njobs; i++) {
@@ -606,7 +653,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     b_.pre_lf = dpre + P_LF * stride;
     b_.pre_nv12 = dpre + P_NV12 * stride;
     b_.jticket = (uint32_t *)(dev + pre_off) + P_N * stride;
-    const uint64_t tk_off = pre_off + align256(12ull * stride * 4);
+    b_.jstart = (const uint32_t *)(dev + pre_off) + (P_N + 1) * stride;
+    const uint64_t tk_off = pre_off + align256(13ull * stride * 4);
     b_.bar = (uint32_t *)(dev + tk_off);
     memset(host + tk_off, 0, 256); /* the barrier counter starts at zero */
     b_.spin_limit = spin_limit;
@@ -626,7 +674,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     /* Leading levels at least tune.intra_level_min blocks wide (over the
      * batch) are launched one by one; the persistent workers take the rest,
      * each job's tickets starting where its first remaining level starts. */
-    uint32_t *tick = pre + P_N * stride; /* row P_N: per-job intra tickets */
+    uint32_t *tick = pre + P_N * stride;        /* row P_N: per-job intra tickets */
+    uint32_t *jst = pre + (P_N + 1) * stride;   /* row P_N + 1: where they start */
     uint32_t head = 0;
     level_n_.clear();
     level_ns_.clear();
@@ -648,8 +697,9 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     uint32_t tail = 0;
     for (uint32_t i = 0; i < njobs; i++) {
         const std::vector<uint32_t> &lv = layouts[i]->levels;
-        tick[i] = lv.empty() ? 0 : lv[std::min<size_t>(head, lv.size() - 1)];
-        tail += jobs[i].n_intra - tick[i];
+        jst[i] = lv.empty() ? 0 : lv[std::min<size_t>(head, lv.size() - 1)];
+        tick[i] = 0;
+        tail += jobs[i].n_intra - jst[i];
     }
     n_.levels = head;
     n_.level_n = head ? level_n_.data() : nullptr;

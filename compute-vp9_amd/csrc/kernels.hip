@@ -1327,11 +1327,23 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     }
 }
 
-/* A job of the batch with intra tickets left, looked for by one whole wave
- * from job j + 1 on, round robin; B.njobs when every job's tickets are
- * taken. Each lane reads one job's ticket counter: a plain load, so workers
- * that run out of work do not queue up on the counters' atomics. */
-__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
+/* Persistent workers are spread over the batch's jobs in proportion to
+ * their intra blocks: worker w starts on the job holding intra block
+ * floor(w * total / nwk) of the batch, so job j's workers are
+ * [first(j), first(j + 1)) with first(j) = ceil(pre_intra[j] * nwk / total). */
+__device__ __forceinline__ uint32_t job_first_worker(const DBatch &B, uint32_t j, uint32_t nwk,
+                                                     uint32_t total)
+{
+    return (uint32_t)(((uint64_t)B.pre_intra[j] * nwk + total - 1) / total);
+}
+
+/* A job of the batch that no worker started on and that has intra tickets
+ * left, looked for by one whole wave from job j + 1 on, round robin;
+ * B.njobs when there is none. Each lane reads one job's ticket counter: a
+ * plain load, so workers that run out of work do not queue up on the
+ * counters' atomics. */
+__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, uint32_t nwk,
+                                                  uint32_t total)
 {
     const uint32_t lane = __lane_id();
     for (uint32_t k = 0; k < B.njobs; k += 64) {
@@ -1339,8 +1351,10 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
         while (idx >= B.njobs) idx -= B.njobs;
         bool open = false;
         if (k + lane < B.njobs)
-            open = __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                   B.jobs[idx].n_intra;
+            open = job_first_worker(B, idx, nwk, total) == job_first_worker(B, idx + 1, nwk, total) &&
+                   B.jstart[idx] +
+                           __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       B.jobs[idx].n_intra;
         const uint64_t m = __ballot(open);
         if (m) return __shfl(idx, __builtin_ctzll(m));
     }
@@ -1349,7 +1363,19 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
 
 /* Persistent workers, W threads each: 256 (one workgroup per block) or 64
  * (one wave per block, four independent workers per workgroup); this is
- * worker wk of nwk. */
+ * worker wk of nwk.
+ *
+ * A job's remaining intra-list entries [jstart, n_intra) are dealt out to
+ * its c workers by stride: worker r of them takes entries jstart + r,
+ * jstart + r + c, ... in that order. Nothing is fetched through an atomic
+ * counter: a counter on one address hands out only some tens of entries
+ * per microsecond, which held the intra stage's first levels back. The
+ * lowest unfinished entry of a job always runs: its worker has finished
+ * every earlier entry it owns, and its dependencies are lower entries,
+ * finished too. A job that no worker started on (its share of the batch's
+ * intra blocks is below one worker) is served through its ticket counter
+ * by the workers that have finished their own entries: the tickets a
+ * worker holds are its current one and a later one of the same job. */
 template <int W>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
                                              uint32_t nwk)
@@ -1358,25 +1384,26 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
     const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
 
-    /* Job, ticket and the count of jobs left to visit are made wave-uniform
-     * with readfirstlane, so every loop test is a scalar branch. With an
-     * LDS value as the exit test the compiler treats the exit as divergent
-     * and restructures the loop so that thread 0's ticket fetch runs outside
-     * the loop holding the barriers: the other lanes then repeat one ticket
-     * forever. */
+    /* Job, entry and stride are made wave-uniform with readfirstlane, so
+     * every loop test is a scalar branch. With an LDS value as the exit
+     * test the compiler treats the exit as divergent and restructures the
+     * loop so that thread 0's ticket fetch runs outside the loop holding the
+     * barriers: the other lanes then repeat one ticket forever. */
     uint32_t j = __builtin_amdgcn_readfirstlane(
         find_job(B.pre_intra, B.njobs, (uint32_t)((uint64_t)wk * total / nwk)));
-    if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
-    worker_sync<W>();
-    uint32_t tk = __builtin_amdgcn_readfirstlane(S.ticket);
+    const uint32_t f0 = job_first_worker(B, j, nwk, total);
+    uint32_t stride = __builtin_amdgcn_readfirstlane(job_first_worker(B, j + 1, nwk, total) - f0);
+    uint32_t tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + (wk - f0));
+    bool counted = false; /* entries from the job's ticket counter */
     for (;;) {
         const DJob &J = B.jobs[j];
         if (tk >= J.n_intra) {
-            /* this job's blocks are all taken: go on with a job that has
-             * some left (the first sync: every thread has read S.ticket) */
+            /* this worker's entries of the job are done: go on with a job
+             * that no worker started on and has tickets left (the first
+             * sync: every thread has read S.ticket) */
             worker_sync<W>();
             if (tid < 64) {
-                const uint32_t nj = find_open_job(B, j);
+                const uint32_t nj = find_open_job(B, j, nwk, total);
                 if (tid == 0) {
                     S.job = nj;
                     if (nj < B.njobs) S.ticket = atomicAdd(B.jticket + nj, 1u);
@@ -1385,22 +1412,22 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
             worker_sync<W>();
             j = __builtin_amdgcn_readfirstlane(S.job);
             if (j >= B.njobs) break;
-            tk = __builtin_amdgcn_readfirstlane(S.ticket);
+            tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + S.ticket);
+            counted = true;
             continue;
         }
-        /* the next ticket of this job, fetched while this block runs; the
-         * tickets a worker holds are its current one and a later one of the
-         * same job, so the lowest unfinished ticket of a job always runs.
-         * (atomicInc: the compiler's wave aggregation of atomicAdd would wait
-         * for the result right here; tickets never reach the wrap limit.) */
+        /* with a counted job, the next ticket is fetched while this block
+         * runs (atomicInc: the compiler's wave aggregation of atomicAdd
+         * would wait for the result right here; tickets never reach the
+         * wrap limit) */
         uint32_t next = 0;
-        if (tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
+        if (counted && tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
         intra_block<W, true>(B, J, tk, S, tid);
 
-        /* next ticket; the sync also orders LDS reuse */
+        /* next entry; the sync also orders LDS reuse */
         if (tid == 0) S.ticket = next;
         worker_sync<W>();
-        tk = __builtin_amdgcn_readfirstlane(S.ticket);
+        tk = counted ? __builtin_amdgcn_readfirstlane(B.jstart[j] + S.ticket) : tk + stride;
     }
 }
 

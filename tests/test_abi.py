@@ -260,9 +260,10 @@ def check_order(pkg, frame, b):
     pos = {blk: p for p, blk in enumerate(order)}
     for blk, d in deps.items():
         assert all(pos[o] < pos[blk] for o in d), f"block {blk} handed out before a dependency"
-    # level 0 (no intra dependency) comes first: small blocks, then large, each in list order
+    # level 0 (no intra dependency) comes first: small blocks, then large, each by
+    # the height of the chains they head, then in list order
     free = [blk for blk, d in deps.items() if not d]
-    assert order[:len(free)] == by_class(b, w, h, free)
+    assert order[:len(free)] == by_class(b, w, h, free, brute_heights(deps))
     return n
 
 
@@ -271,10 +272,26 @@ def small(k, w, h):
     return min(int(k["width"]), w - int(k["x"])) <= 16 and min(int(k["height"]), h - int(k["y"])) <= 16
 
 
-def by_class(b, w, h, blks):
-    """The planner's order inside one level: small blocks, then large, each in list order."""
-    return (sorted(x for x in blks if small(b[x], w, h)) +
-            sorted(x for x in blks if not small(b[x], w, h)))
+def by_class(b, w, h, blks, hgt):
+    """The planner's order inside one level: small blocks, then large, each
+    with the blocks heading the longest dependency chains first, then in list
+    order."""
+    key = lambda x: (-hgt[x], x)
+    return (sorted((x for x in blks if small(b[x], w, h)), key=key) +
+            sorted((x for x in blks if not small(b[x], w, h)), key=key))
+
+
+def brute_heights(deps):
+    """Height of each intra block: 1 without a later intra block depending on
+    it, else one more than the highest height among its dependents."""
+    users = {blk: [] for blk in deps}
+    for blk, d in deps.items():
+        for o in d:
+            users[o].append(blk)
+    hgt = {}
+    for blk in sorted(deps, reverse=True):
+        hgt[blk] = 1 + max((hgt[u] for u in users[blk]), default=0)
+    return hgt
 
 
 @pytest.mark.parametrize("preset,w,h", [(1, 320, 184), (2, 643, 362), (0, 136, 72)])
@@ -321,7 +338,9 @@ def test_intra_levels_match_dependencies(pkg, preset, w, h, shuffle):
             keep = rng.random(b.size) >= 0.1
             b = b[keep][rng.permutation(int(keep.sum()))]
             f = pkg.OwnedFrame(w, h, b, f.coeffs(), frame_type=f.pf.hdr.frame_type)
-        lev = brute_levels(intra_dependencies(b, w, h))
+        deps = intra_dependencies(b, w, h)
+        lev = brute_levels(deps)
+        hgt = brute_heights(deps)
         starts = pkg.intra_levels(f)
         order = np.zeros(max(len(lev), 1), np.uint32)
         n = pkg.lib().cvp9_hip_intra_order(f.ptr, order.ctypes.data_as(pkg.C.POINTER(pkg.C.c_uint32)),
@@ -331,5 +350,5 @@ def test_intra_levels_match_dependencies(pkg, preset, w, h, shuffle):
         assert len(starts) == (depth + 1 if depth else 0)
         for L in range(depth):
             got = order[starts[L]:starts[L + 1]].tolist()
-            want = by_class(b, w, h, [blk for blk, v in lev.items() if v == L])
+            want = by_class(b, w, h, [blk for blk, v in lev.items() if v == L], hgt)
             assert got == want, f"frame {idx} level {L}"
