@@ -1775,7 +1775,7 @@ Tune cvp9_default_tune(int cus)
     t.res_split[2] = 2;
     t.res_split[3] = 4;
     t.intra_lanes = 256;
-    t.intra_workers = 2u * (uint32_t)(cus > 0 ? cus : 256);
+    t.intra_workers = 4u * (uint32_t)(cus > 0 ? cus : 256);
     t.intra_level_min = 0;
     return t;
 }
