@@ -789,10 +789,12 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
     /* each candidate run measures every stage at once; the transform split
      * is uniform over sizes inside a run and chosen per size afterwards */
     struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; uint32_t level_min; };
+    /* (the intra launch holds the worker count to what stays resident) */
     const Cand cands[] = {
-        {1, 256, 2 * cu, 0},      {4, 256, 2 * cu, 0},      {2, 64, 2 * cu, 0},
-        {1, 64, 4 * cu, 0},       {2, 64, 4 * cu, 4 * cu},  {4, 256, 2 * cu, 8 * cu},
-        {1, 256, 2 * cu, 2 * cu}, {2, 256, 2 * cu, 4 * cu}, {1, 256, 4 * cu, 4 * cu},
+        {1, 256, 4 * cu, 0},      {2, 256, 4 * cu, 0},       {4, 256, 3 * cu, 0},
+        {1, 64, 8 * cu, 0},       {2, 64, 16 * cu, 0},       {4, 64, 12 * cu, 0},
+        {1, 256, 2 * cu, 4 * cu}, {2, 256, 4 * cu, 4 * cu},  {4, 256, 4 * cu, 16 * cu},
+        {1, 64, 16 * cu, 4 * cu},
     };
     const bool was_timing = e->timing;
     const Tune saved = e->tune;

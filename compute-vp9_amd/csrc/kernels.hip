@@ -1372,10 +1372,12 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, u
  * per microsecond, which held the intra stage's first levels back. The
  * lowest unfinished entry of a job always runs: its worker has finished
  * every earlier entry it owns, and its dependencies are lower entries,
- * finished too. A job that no worker started on (its share of the batch's
- * intra blocks is below one worker) is served through its ticket counter
- * by the workers that have finished their own entries: the tickets a
- * worker holds are its current one and a later one of the same job. */
+ * finished too (every worker of the launch becomes resident: the launch is
+ * held to what the device holds, see intra_resident_wgs). A job that no
+ * worker started on (its share of the batch's intra blocks is below one
+ * worker) is served through its ticket counter by the workers that have
+ * finished their own entries: the tickets a worker holds are its current
+ * one and a later one of the same job. */
 template <int W>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
                                              uint32_t nwk)
@@ -1725,6 +1727,31 @@ hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s)
     return hipGetLastError();
 }
 
+/* Half the workgroups of k_intra<W> that the current device holds at once
+ * when nothing else runs (cached per device). Persistent workers own their
+ * intra entries by stride, and a worker may wait on an entry of any other
+ * worker of its job, so every workgroup of a launch must become resident
+ * while the others wait. The kernels that run beside it never wait, so
+ * they end and give their slots up; the only other launch that may wait is
+ * the intra stage of a batch on the engine's other queue, which is held to
+ * the other half. */
+template <int W>
+static uint32_t intra_resident_wgs()
+{
+    static uint32_t cache[16] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    if (!cache[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra<W>, 256, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per_cu < 1 || cus < 1)
+            return 128;
+        cache[dev] = (uint32_t)(per_cu * cus) / 2;
+    }
+    return cache[dev];
+}
+
 uint32_t cvp9_fused_max_wgs(int device)
 {
     int per_cu = 0, cus = 0;
@@ -1874,10 +1901,13 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         }
         if (n.intra_tail) {
             const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;
-            if (n.tune.intra_lanes == 64)
-                hipLaunchKernelGGL(k_intra<64>, dim3(div_up(w, 4)), dim3(256), 0, s, b);
-            else
-                hipLaunchKernelGGL(k_intra<256>, dim3(w), dim3(256), 0, s, b);
+            if (n.tune.intra_lanes == 64) {
+                const uint32_t g = div_up(w, 4), cap = intra_resident_wgs<64>();
+                hipLaunchKernelGGL(k_intra<64>, dim3(g < cap ? g : cap), dim3(256), 0, s, b);
+            } else {
+                const uint32_t cap = intra_resident_wgs<256>();
+                hipLaunchKernelGGL(k_intra<256>, dim3(w < cap ? w : cap), dim3(256), 0, s, b);
+            }
         }
         break;
     case STAGE_LF:
