@@ -933,12 +933,13 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
 
 
 
+/* Every candidate is computed and one selected, with no branch: the
+ * neighbour samples come from LDS, and a branch per mode would move each
+ * read into its own branch and wait for it there. */
 __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc)
 {
-    if (mode == 0) return dc;
-    if (mode == 1) return a;
-    if (mode == 2) return l;
-    return clampi(a + l - tl, 0, 255);
+    const int tm = clampi(a + l - tl, 0, 255);
+    return mode == 0 ? dc : (mode == 1 ? a : (mode == 2 ? l : tm));
 }
 
 /* One neighbour cell of an intra block (thread c of the gather; kind 0
@@ -1047,8 +1048,8 @@ __device__ __forceinline__ void worker_sync()
 }
 
 struct IntraShared {
-    uint8_t nb[3][2][64];   /* above / left samples per plane */
-    uint8_t edge[3][2][64]; /* final bottom row / right column */
+    uint8_t nb[3][2][64];                 /* above / left samples per plane */
+    __align__(4) uint8_t edge[3][2][64];  /* final bottom row / right column */
     uint32_t sum[3];
     uint32_t ticket;
     uint32_t job;
@@ -1226,10 +1227,11 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
             else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
             if (4 * wi < len) {
-                uint64_t d = (uint64_t)epoch << 32;
-    #pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
+                /* four samples in one LDS read (the edge rows are 4-aligned),
+                 * those past the edge's length zeroed */
+                const int n = len - 4 * wi;
+                const uint32_t q = *(const uint32_t *)&S.edge[p][dir][4 * wi];
+                const uint64_t d = (uint64_t)epoch << 32 | (n >= 4 ? q : q & ((1u << (8 * n)) - 1));
                 __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -1316,10 +1318,11 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             if (t < 32) { p = 0; dir = t >= 16; wi = t & 15; len = dir ? bh : bw; }
             else { p = (t < 48) ? 1 : 2; dir = (t & 15) >= 8; wi = t & 7; len = dir ? cbh : cbw; }
             if (4 * wi < len) {
-                uint64_t d = (uint64_t)epoch << 32;
-    #pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (4 * wi + e < len) d |= (uint64_t)S.edge[p][dir][4 * wi + e] << (8 * e);
+                /* four samples in one LDS read (the edge rows are 4-aligned),
+                 * those past the edge's length zeroed */
+                const int n = len - 4 * wi;
+                const uint32_t q = *(const uint32_t *)&S.edge[p][dir][4 * wi];
+                const uint64_t d = (uint64_t)epoch << 32 | (n >= 4 ? q : q & ((1u << (8 * n)) - 1));
                 __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
