@@ -1132,6 +1132,11 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
      *    the neighbour stage; cells of earlier intra blocks from their
      *    edge records. ncx + ncy + 1 <= 33 cells: one per thread. */
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
+    /* edge samples of Y, U and V; a block with at most 64 of them (up to
+     * 16x16) computes and publishes them inside the gathering wave, from
+     * its registers, with no barrier in between */
+    const int ne = bw + bh + 2 * (cbw + cbh);
+    const bool fast = EARLY && ne <= 64;
     if (tid < 64) { /* the worker's first wave, whole */
         const int c = tid;
         const int kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
@@ -1164,6 +1169,59 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         }
         syu = wave_sum(syu);
         sv = wave_sum(sv);
+        if (EARLY && fast) {
+            /* edge sample i = tid, numbered as in the edge loop below: Y
+             * bottom row, Y right column, then U and V likewise. Its above,
+             * left and above-left samples come from the lanes holding those
+             * cells (an above or left cell is 4 luma or 2 chroma samples
+             * wide; bytes 0-3 luma, 4-5 U, 6-7 V; the above-left cell's
+             * samples in bytes 3, 5, 7). */
+            const int dy = (int)(uint8_t)(((syu & 0xffff) + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
+            const int du = (int)(uint8_t)(((syu >> 16) + (uint32_t)((cbw + cbh) >> 1)) / (uint32_t)(cbw + cbh));
+            const int dv = (int)(uint8_t)((sv + (uint32_t)((cbw + cbh) >> 1)) / (uint32_t)(cbw + cbh));
+            const int i = tid;
+            int p, dir, e;
+            if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
+            else {
+                const int q = i - bw - bh;
+                p = q < cbw + cbh ? 1 : 2;
+                const int u = q - (p - 1) * (cbw + cbh);
+                dir = u >= cbw;
+                e = dir ? u - cbw : u;
+            }
+            const int r = p == 0 ? (dir ? e : bh - 1) : (dir ? e : cbh - 1);
+            const int cc = p == 0 ? (dir ? bw - 1 : e) : (dir ? cbw - 1 : e);
+            const int qa = min(p == 0 ? cc >> 2 : cc >> 1, 63);
+            const int ql = min(ncx + (p == 0 ? r >> 2 : r >> 1), 63);
+            const int qt = ncx + ncy;
+            const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+            const uint32_t alo = __shfl(lo, qa), ahi = __shfl(hi, qa);
+            const uint32_t llo = __shfl(lo, ql), lhi = __shfl(hi, ql);
+            const uint32_t tlo = __shfl(lo, qt), thi = __shfl(hi, qt);
+            const uint32_t sa = p == 0 ? 8 * (cc & 3) : 8 * (cc & 1) + (p == 2 ? 16 : 0);
+            const uint32_t sl = p == 0 ? 8 * (r & 3) : 8 * (r & 1) + (p == 2 ? 16 : 0);
+            const int a = (int)(((p == 0 ? alo : ahi) >> sa) & 0xff);
+            const int l = (int)(((p == 0 ? llo : lhi) >> sl) & 0xff);
+            const int t = (int)(p == 0 ? tlo >> 24 : (p == 1 ? (thi >> 8) & 0xff : thi >> 24));
+            int es = intra_pred(p == 0 ? k.ymode : k.uvmode, a, l, t, p == 0 ? dy : (p == 1 ? du : dv));
+            if (p == 0) es = clampi(es + (eres_ok ? eres0 : 0), 0, 255);
+            /* edge word w = tid: its four samples from the lanes holding
+             * them, those past the edge's length zero */
+            const int w = tid;
+            int wp, wdir, wi, len;
+            if (w < 32) { wp = 0; wdir = w >= 16; wi = w & 15; len = wdir ? bh : bw; }
+            else { wp = (w < 48) ? 1 : 2; wdir = (w & 15) >= 8; wi = w & 7; len = wdir ? cbh : cbw; }
+            const int base = wp == 0 ? (wdir ? bw : 0) : bw + bh + (wp - 1) * (cbw + cbh) + (wdir ? cbw : 0);
+            uint32_t word = 0;
+#pragma unroll
+            for (int e2 = 0; e2 < 4; e2++) {
+                const uint32_t x2 = __shfl((uint32_t)es, min(base + 4 * wi + e2, 63));
+                if (4 * wi + e2 < len) word |= (x2 & 0xff) << (8 * e2);
+            }
+            if (4 * wi < len)
+                __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + w,
+                                   (uint64_t)epoch << 32 | word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (c == 0) {
             S.sum[0] = syu & 0xffff;
             S.sum[1] = syu >> 16;
@@ -1189,11 +1247,10 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     worker_sync<W>();
 
     const int dc0 = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
-    if constexpr (EARLY) {
+    if (EARLY && !fast) {
         /* 2. with EARLY, the edge samples first (bottom row and right
          *    column of Y, U and V), published at once, so blocks waiting on
          *    this one go on while its interior is written */
-        const int ne = bw + bh + 2 * (cbw + cbh);
         for (int i = tid; i < ne; i += W) {
             int p, dir, e;
             if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
@@ -1437,7 +1494,7 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_intra(DBatch B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_intra(DBatch B)
 {
     __shared__ IntraShared shm[256 / W];
     intra_worker<W>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W, gridDim.x * (256 / W));
