@@ -536,8 +536,8 @@ def main():
                     help="only run the calibration copy kernel (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(streams shard across GPUs with no collectives; see the design notes)",
+        print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(streams would shard across GPUs, one engine per GPU, with no collectives)",
               file=sys.stderr)
         sys.exit(2)
     P = load_package()

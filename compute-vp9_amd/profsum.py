@@ -59,8 +59,8 @@ def main():
         ff = (a.copy_bytes / 1024) / cf
         wf = (a.copy_bytes / 1024) / cw
 
-    lines = [f"# {a.name}", "", f"Workload: {a.workload}", "", "Commands (from the repository root "
-             "on the GPU box, `cd /tmp && export TMPDIR=/tmp` first; `$R` is the repository):", ""]
+    lines = [f"# {a.name}", "", f"Workload: {a.workload}", "", "Commands (on a machine with the GPU, "
+             "after `cd /tmp && export TMPDIR=/tmp`; `$R` is the repository root):", ""]
     for c in a.command:
         lines += ["```", c, "```"]
     lines += ["", "## Kernel durations (rocprofv3 --kernel-trace --stats)", "",
