@@ -933,6 +933,17 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
 
 
 
+/* x / n for x < 2^16 and 1 <= n <= 128, in a few instructions: the
+ * truncated product with a float reciprocal (v_rcp_f32, within 1 ulp) is
+ * within one of the quotient, and one step each way corrects it (exact over
+ * the whole range for reciprocal errors up to 2 ulp, checked exhaustively) */
+__device__ __forceinline__ uint32_t udiv_small(uint32_t x, uint32_t n)
+{
+    const uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)n));
+    const int r = (int)x - (int)(q * n);
+    return q + (r >= (int)n ? 1u : 0u) - (r < 0 ? 1u : 0u);
+}
+
 /* Every candidate is computed and one selected, with no branch: the
  * neighbour samples come from LDS, and a branch per mode would move each
  * read into its own branch and wait for it there. */
@@ -1098,7 +1109,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
     const int nq = (bw + 3) >> 2;
     auto resid_quad = [&](int i) -> const GMEM int16_t * {
-        const int r = i / nq, c0 = (i - r * nq) * 4;
+        const int r = (int)udiv_small((uint32_t)i, (uint32_t)nq), c0 = (i - r * nq) * 4;
         if (r >= rrows || c0 >= rcols) return nullptr;
         return jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
                ((r & (T - 1)) << lt) + (c0 & (T - 1));
@@ -1176,9 +1187,9 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
              * cells (an above or left cell is 4 luma or 2 chroma samples
              * wide; bytes 0-3 luma, 4-5 U, 6-7 V; the above-left cell's
              * samples in bytes 3, 5, 7). */
-            const int dy = (int)(uint8_t)(((syu & 0xffff) + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
-            const int du = (int)(uint8_t)(((syu >> 16) + (uint32_t)((cbw + cbh) >> 1)) / (uint32_t)(cbw + cbh));
-            const int dv = (int)(uint8_t)((sv + (uint32_t)((cbw + cbh) >> 1)) / (uint32_t)(cbw + cbh));
+            const int dy = (int)(uint8_t)udiv_small((syu & 0xffff) + (uint32_t)((bw + bh) >> 1), (uint32_t)(bw + bh));
+            const int du = (int)(uint8_t)udiv_small((syu >> 16) + (uint32_t)((cbw + cbh) >> 1), (uint32_t)(cbw + cbh));
+            const int dv = (int)(uint8_t)udiv_small(sv + (uint32_t)((cbw + cbh) >> 1), (uint32_t)(cbw + cbh));
             const int i = tid;
             int p, dir, e;
             if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
@@ -1246,7 +1257,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     }
     worker_sync<W>();
 
-    const int dc0 = (int)(uint8_t)((S.sum[0] + (uint32_t)((bw + bh) >> 1)) / (uint32_t)(bw + bh));
+    const int dc0 = (int)(uint8_t)udiv_small(S.sum[0] + (uint32_t)((bw + bh) >> 1), (uint32_t)(bw + bh));
     if (EARLY && !fast) {
         /* 2. with EARLY, the edge samples first (bottom row and right
          *    column of Y, U and V), published at once, so blocks waiting on
@@ -1269,8 +1280,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                 v = clampi(v + rv, 0, 255);
             } else {
                 const int r = dir ? e : cbh - 1, c = dir ? cbw - 1 : e;
-                const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
-                                               (uint32_t)(cbw + cbh));
+                const int dcc = (int)(uint8_t)udiv_small(S.sum[p] + (uint32_t)((cbw + cbh) >> 1),
+                                                         (uint32_t)(cbw + cbh));
                 v = intra_pred(k.uvmode, S.nb[p][0][c], S.nb[p][1][r], S.tl[p], dcc);
             }
             S.edge[p][dir][e] = (uint8_t)v;
@@ -1321,7 +1332,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             for (int u = 0; u < 4; u++) {
                 const int i = i0 + u * W;
                 if (i >= nitems) break;
-                const int r = i / nq, c0 = (i - r * nq) * 4;
+                const int r = (int)udiv_small((uint32_t)i, (uint32_t)nq), c0 = (i - r * nq) * 4;
                 const int l = S.nb[0][1][r];
                 uint32_t word = 0;
 #pragma unroll
@@ -1344,9 +1355,9 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         for (int i = tid; i < 2 * nqc * cbh; i += W) {
             const int p = i < nqc * cbh ? 1 : 2;
             const int q = i - (p - 1) * nqc * cbh;
-            const int r = q / nqc, c0 = (q - r * nqc) * 4;
-            const int dcc = (int)(uint8_t)((S.sum[p] + (uint32_t)((cbw + cbh) >> 1)) /
-                                           (uint32_t)(cbw + cbh));
+            const int r = (int)udiv_small((uint32_t)q, (uint32_t)nqc), c0 = (q - r * nqc) * 4;
+            const int dcc = (int)(uint8_t)udiv_small(S.sum[p] + (uint32_t)((cbw + cbh) >> 1),
+                                                     (uint32_t)(cbw + cbh));
             const int l = S.nb[p][1][r];
             uint32_t word = 0;
 #pragma unroll
