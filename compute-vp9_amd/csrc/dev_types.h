@@ -13,8 +13,10 @@
  *   Prepared frame ("prep"): the parsed frame staged once into HBM:
  *     DBlock[nblk]   24-byte block records (clamped, normalised modes)
  *     mc_tab[]       motion-compensation units: block << 8 | unit in block
- *     intra[]        local indices of intra blocks, by dependency level
- *                    then list order
+ *     intra[]        local indices of intra blocks, by dependency level;
+ *                    inside a level small blocks (at most 16x16) first,
+ *                    each group by the length of its chain of later
+ *                    dependents, longest first
  *     tx_tab[c][]    transforms of size class c (inter and intra blocks, list
  *                    order), 8 bytes each: coefficient offset; x | y << 14,
  *                    bit 31 set for an intra block
