@@ -77,6 +77,28 @@ def test_tiny_frames(pkg, orc, gpu_ready, preset, w, h, n):
     compare(pkg, orc, frames, f"tiny preset{preset} {w}x{h} n={n}")
 
 
+def _random_shapes(count, seed=20261203):
+    """Frame sizes drawn from a fixed generator: 2..300 x 2..200, none 1 mod 8
+    (the host validator sends those to the CPU path), with a preset, a seed
+    and a KEY-frame interval each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        w, h = int(rng.integers(2, 301)), int(rng.integers(2, 201))
+        if w % 8 == 1 or h % 8 == 1:
+            continue
+        out.append((int(rng.integers(0, 3)), w, h, int(rng.integers(1, 1 << 20)),
+                    int(rng.choice([2, 3, 1000]))))
+    return out
+
+
+@pytest.mark.parametrize("preset,w,h,seed,key", _random_shapes(12))
+def test_random_sizes(pkg, orc, gpu_ready, preset, w, h, seed, key):
+    p = pkg.synth_params(preset, w, h, seed, key_interval=key)
+    frames = [pkg.SynthFrame(p, i) for i in range(4)]
+    compare(pkg, orc, frames, f"random preset{preset} {w}x{h} seed{seed} key/{key}")
+
+
 def test_skip_blocks_and_sharpness(pkg, orc, gpu_ready):
     for sharp in (0, 3, 6):
         p = pkg.synth_params(1, 200, 136, 55 + sharp, skip_permille=300, sharpness=sharp)
