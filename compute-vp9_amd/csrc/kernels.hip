@@ -1698,13 +1698,9 @@ __global__ __launch_bounds__(256) void k_nv12(DBatch B)
 
 __device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
 {
-    /* every thread releases its own stores at agent scope before the
-     * workgroup arrives: a wave's release waits only for that wave's stores
-     * (its own vmcnt), so one thread's fence would let the other waves'
-     * stores still be in flight when the other workgroups go on */
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_fetch_add(B.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint32_t spins = 0;
         while (__hip_atomic_load(B.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -1714,9 +1710,9 @@ __device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
                 break;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 /* transform split used inside the fused kernel, per size 4, 8, 16, 32 */
