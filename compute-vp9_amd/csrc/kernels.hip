@@ -1698,6 +1698,11 @@ __global__ __launch_bounds__(256) void k_nv12(DBatch B)
 
 __device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
 {
+    /* Every wave waits until its own stores have been acknowledged by L2
+     * before the workgroup barrier; thread 0's agent-scope release then
+     * writes back an L2 that holds all of the workgroup's stores. (The
+     * release orders only the issuing wave's own accesses.) */
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");

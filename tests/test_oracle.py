@@ -37,8 +37,6 @@ def gen(pkg, case):
 @pytest.mark.parametrize("name", list(load_hashes()["hash_cases"]))
 def test_oracle_matches_golden_hashes(pkg, orc, name):
     case = load_hashes()["hash_cases"][name]
-    if case["width"] * case["height"] > 2_100_000:
-        pytest.skip("2160p golden case runs in the slow GPU parity test")
     frames = gen(pkg, case)
     for f, h in zip(frames, case["input"]):
         got = sha(np.concatenate([f.blocks().view(np.uint8), f.coeffs().view(np.uint8)]))
@@ -216,8 +214,14 @@ def test_lf_vertical_pass_reads_horizontal_output(orc):
     img[8:, :8] = 104
     img[8:, 8:] = 108
     out = orc.loop_filter(img, 32, 0).astype(int)
-    # the corner sample (7,7) is moved by both passes
-    assert out[7, 7] == 104 - 2 + 0 or out[7, 7] != img[7, 7]
+    # Horizontal pass (edge y=8): columns < 8 step 100 -> 104, columns >= 8
+    # step 104 -> 108; both give delta (3*4+4)>>3 = 2, so rows 6..9 become
+    # 101,102,102,103 (left) and 105,106,106,107 (right). The vertical pass
+    # (edge x=8) then reads those rows: row 7 steps 102 -> 106 (delta 2), so
+    # (7,6..9) = 103,104,104,105; row 6 steps 101 -> 105, so (6,7) = 103.
+    # A vertical pass that read the unfiltered image would give (7,7) = 102.
+    assert out[7, 6] == 103 and out[7, 7] == 104 and out[7, 8] == 104 and out[7, 9] == 105
+    assert out[6, 7] == 103 and out[8, 7] == 104
     assert out[0, 7] == 102 and out[7, 0] == 102
 
 
