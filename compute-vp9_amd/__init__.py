@@ -33,6 +33,7 @@ VP9_FRAME_KEY = 0
 VP9_FRAME_NON_KEY = 1
 CVP9_HIP_INFLIGHT = 3
 CVP9_HIP_DEBUG_NO_DEBLOCK = 1
+CVP9_HIP_DEBUG_SPIN_ONE = 2
 
 
 class Cvp9Error(RuntimeError):
@@ -163,6 +164,7 @@ SIGNATURES = {
     "hip_export_buffer_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(ExportBuffer)]),
     "hip_export_buffer_free": (None, [C.c_void_p, C.POINTER(ExportBuffer)]),
     "hip_set_render_target": (C.c_int, [C.c_void_p, C.POINTER(ExportBuffer)]),
+    "hip_backend_read_ref": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(FrameInfo)]),
     # synth.h
     "cvp9_synth_preset": (C.c_int, [C.POINTER(SynthParams), C.c_int, C.c_uint32,
                                     C.c_uint32, C.c_uint64]),
@@ -175,6 +177,7 @@ SIGNATURES = {
     "cvp9_engine_stream_open": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "cvp9_engine_stream_close": (None, [C.c_void_p, C.c_int]),
     "cvp9_engine_stream_bytes": (C.c_uint64, [C.c_void_p, C.c_int]),
+    "cvp9_engine_stream_error": (C.c_uint32, [C.c_void_p, C.c_int]),
     "cvp9_engine_prepare": (C.c_int, [C.c_void_p, PF_P, C.POINTER(C.c_void_p)]),
     "cvp9_engine_prep_free": (None, [C.c_void_p, C.c_void_p]),
     "cvp9_engine_prep_bytes": (C.c_uint64, [C.c_void_p]),
@@ -220,6 +223,7 @@ SIGNATURES = {
     "cvp9_hip_device_count": (C.c_int, []),
     "cvp9_hip_idct_table": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "cvp9_hip_plan_check": (C.c_int, [PF_P]),
+    "cvp9_hip_route_check": (C.c_int, [C.POINTER(PF_P), C.c_int, C.POINTER(C.c_int)]),
     "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_intra_levels": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_export_read": (C.c_int, [C.c_void_p, C.POINTER(ExportBuffer), C.POINTER(C.c_uint8),
@@ -366,6 +370,18 @@ def i420_size(w, h):
     return w * h + 2 * ((w + 1) // 2) * ((h + 1) // 2)
 
 
+def route_check(frames):
+    """Codes hip_decode_frame would return for a sequence before any device
+    work (validator + the decline-until-KEY rule); needs no device."""
+    n = len(frames)
+    arr = (PF_P * n)(*[f.ptr for f in frames])
+    codes = (C.c_int * n)()
+    r = lib().cvp9_hip_route_check(arr, n, codes)
+    if r != n:
+        raise Cvp9Error(r, "cvp9_hip_route_check")
+    return list(codes)
+
+
 # ---- HIP backend (include/cvp9_hip/hip_backend.h) ----------------------------
 class HipBackend:
     """Thin mirror of the eight hip_* entry points."""
@@ -389,6 +405,24 @@ class HipBackend:
 
     def set_debug(self, flags):
         check(lib().hip_backend_set_debug(self.ctx, flags), "hip_backend_set_debug")
+
+    def read_ref(self, slot):
+        """Reference slot `slot` of the device pool as packed I420, or None
+        when the slot is empty (hip_backend_read_ref)."""
+        info = FrameInfo()
+        check(lib().hip_backend_read_ref(self.ctx, slot, C.byref(info)), "hip_backend_read_ref")
+        if info.width == 0:
+            return None
+        w, h = info.width, info.height
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        out = np.zeros(w * h + 2 * cw * ch, np.uint8)
+        base = out.ctypes.data
+        info.stride_y, info.stride_uv = w, cw
+        info.plane_y = C.cast(base, C.POINTER(C.c_uint8))
+        info.plane_u = C.cast(base + w * h, C.POINTER(C.c_uint8))
+        info.plane_v = C.cast(base + w * h + cw * ch, C.POINTER(C.c_uint8))
+        check(lib().hip_backend_read_ref(self.ctx, slot, C.byref(info)), "hip_backend_read_ref")
+        return out
 
     def export_buffer_alloc(self, size):
         """(code, ExportBuffer): hip_export_buffer_alloc."""
@@ -459,6 +493,9 @@ class Engine:
 
     def stream_bytes(self, sid):
         return int(lib().cvp9_engine_stream_bytes(self.e, sid))
+
+    def stream_error(self, sid):
+        return int(lib().cvp9_engine_stream_error(self.e, sid))
 
     def prepare(self, frame):
         p = C.c_void_p()

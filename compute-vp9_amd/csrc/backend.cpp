@@ -55,6 +55,10 @@ struct Backend {
     struct ExportPriv *pending_priv = nullptr;
     uint32_t debug = 0;
     bool dmabuf = false;
+    /* A frame was declined (CVP9_ERR_UNSUPPORTED): the caller reconstructed
+     * it on the CPU, so the device reference pool lacks it. Every frame is
+     * declined until the next KEY frame, which writes all 8 slots. */
+    bool declined = false;
 };
 
 /* An export buffer: dedicated VRAM with its DMA-BUF, and the CPU view the
@@ -134,6 +138,21 @@ void free_slot(Slot &s)
     if (s.out_dev) hipFree(s.out_dev);
     if (s.err_host) hipHostFree(s.err_host);
     s = Slot();
+}
+
+/* The backend's decision for one frame before any device work: CVP9_OK
+ * (take it), CVP9_ERR_UNSUPPORTED (the caller's CPU path reconstructs it)
+ * or CVP9_ERR_INVALID_DATA. A frame the validator declines sets `declined`;
+ * from then on every frame is declined until a KEY frame (which writes all
+ * 8 reference slots, so the device pool is whole again). */
+cvp9_err_t route(bool &declined, const vp9_parsed_frame_t *pf, PrepLayout &L)
+{
+    cvp9_err_t err = plan_measure(pf, L);
+    if (err == CVP9_ERR_UNSUPPORTED) declined = true;
+    if (err != CVP9_OK) return err;
+    if (declined && pf->hdr.frame_type != VP9_FRAME_KEY) return CVP9_ERR_UNSUPPORTED;
+    declined = false;
+    return CVP9_OK;
 }
 
 /* oldest in-flight frame, released from the ring (vulkan_backend.c:1391-1426) */
@@ -218,12 +237,19 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     Backend *b = (Backend *)ctx;
     if (!b || !pf) return CVP9_ERR_INVALID_DATA;
     if (b->count == CVP9_HIP_INFLIGHT) return CVP9_ERR_AGAIN;
+    const bool key = pf->hdr.frame_type == VP9_FRAME_KEY;
     PrepLayout L;
-    cvp9_err_t err = plan_measure(pf, L);
+    cvp9_err_t err = route(b->declined, pf, L);
     if (err != CVP9_OK) return err;
     /* an armed NV12 target must hold the whole frame */
     if (b->pending_target && b->pending_target_size < i420_bytes(L.W, L.H)) return CVP9_ERR_INVALID_DATA;
     hipSetDevice(b->device);
+    /* After a device error (reported for that frame and every later one)
+     * a KEY frame starts the stream afresh: frames still in flight finish
+     * first, then the pool is emptied and the error cleared. */
+    if (key && cvp9_engine_stream_error(b->eng, b->sid)) {
+        if ((err = cvp9_engine_stream_reset(b->eng, b->sid)) != CVP9_OK) return err;
+    }
 
     Slot &s = b->slots[(b->head + b->count) % CVP9_HIP_INFLIGHT];
     const size_t need = L.bytes;
@@ -260,7 +286,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
         (b->pending_priv && b->pending_priv->shadow &&
          (e = hipMemcpyAsync(b->pending_priv->map, b->pending_priv->dev, ob, hipMemcpyDeviceToHost,
                              q))) ||
-        (e = hipMemcpyAsync(s.err_host, r.err, 4, hipMemcpyDeviceToHost, q)) ||
+        (e = hipMemcpyAsync(s.err_host, r.err, 4, hipMemcpyDefault, q)) ||
         (e = hipEventRecord(s.done, q)))
         return to_err(e);
 
@@ -406,6 +432,34 @@ cvp9_err_t hip_set_render_target(void *ctx, const cvp9_export_buffer_t *target)
     return CVP9_OK;
 }
 
+cvp9_err_t hip_backend_read_ref(void *ctx, int slot, cvp9_frame_info_t *frame)
+{
+    Backend *b = (Backend *)ctx;
+    if (!b || !frame || slot < 0 || slot > 7) return CVP9_ERR_INVALID_DATA;
+    hipSetDevice(b->device);
+    cvp9_err_t err = cvp9_engine_sync(b->eng);
+    if (err != CVP9_OK) return err;
+    const DFrame f = engine_stream_ref(b->eng, b->sid, slot);
+    if (!f.y) {
+        frame->width = frame->height = 0;
+        return CVP9_OK;
+    }
+    if (!frame->plane_y) {
+        frame->width = f.w;
+        frame->height = f.h;
+        return CVP9_OK;
+    }
+    const uint32_t cw = (f.w + 1) / 2, ch = (f.h + 1) / 2;
+    if (frame->width != f.w || frame->height != f.h || !frame->plane_u || !frame->plane_v ||
+        frame->stride_y < f.w || frame->stride_uv < cw)
+        return CVP9_ERR_INVALID_DATA;
+    if (hipMemcpy2D(frame->plane_y, frame->stride_y, f.y, f.sy, f.w, f.h, hipMemcpyDeviceToHost) ||
+        hipMemcpy2D(frame->plane_u, frame->stride_uv, f.u, f.suv, cw, ch, hipMemcpyDeviceToHost) ||
+        hipMemcpy2D(frame->plane_v, frame->stride_uv, f.v, f.suv, cw, ch, hipMemcpyDeviceToHost))
+        return CVP9_ERR_GPU;
+    return CVP9_OK;
+}
+
 /* ---- testing hooks (cvp9_hip/testing.h) --------------------------------- */
 
 const char *cvp9_hip_version(void) { return "cvp9_hip 0.1.0 (gfx950)"; }
@@ -542,7 +596,19 @@ cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags)
     Backend *b = (Backend *)ctx;
     if (!b) return CVP9_ERR_INVALID_DATA;
     b->debug = flags;
-    return CVP9_OK;
+    /* dependency waits give up after one poll (0: the default bound) */
+    return cvp9_engine_set_spin_limit(b->eng, (flags & CVP9_HIP_DEBUG_SPIN_ONE) ? 1u : 0u);
+}
+
+int cvp9_hip_route_check(const vp9_parsed_frame_t *const *frames, int n, int *codes)
+{
+    if (n < 0 || (n && (!frames || !codes))) return CVP9_ERR_INVALID_DATA;
+    bool declined = false;
+    for (int i = 0; i < n; i++) {
+        PrepLayout L;
+        codes[i] = frames[i] ? route(declined, frames[i], L) : CVP9_ERR_INVALID_DATA;
+    }
+    return n;
 }
 
 } // extern "C"

@@ -137,13 +137,16 @@ struct DJob {
     uint64_t       *nbr;           /* per intra-list entry: CVP9_NB_WORDS words */
     int16_t        *resid;         /* intra residuals, indexed like coeffs */
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
-    uint32_t       *err;           /* error word (spin timeout) */
+    uint32_t       *err;           /* error word (host-visible): byte 0 a dependency wait
+                                      gave up, byte 1 a neighbour record was inconsistent,
+                                      byte 2 a grid barrier timed out */
+    uint32_t       *ectr;          /* the stream's epoch counter (device) */
     uint32_t n_blocks, n_inter, n_intra, n_mc;
     uint32_t n_mc_a;               /* motion-compensation units [0, n_mc_a) run in the
                                       early launch, the rest in the late one */
     uint32_t cls_ntx[4];
     uint32_t ow, oh;
-    uint32_t epoch;
+    uint32_t epoch;                /* written by the batch's first kernel: *ectr + 1 */
     int32_t  lf_thr;               /* 0: no deblocking */
     uint32_t lf_wx, lf_wy;         /* deblocking windows per row / column */
     uint32_t fill;                 /* 1: frame not fully covered by blocks */

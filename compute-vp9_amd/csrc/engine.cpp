@@ -412,13 +412,14 @@ StreamState::~StreamState()
         hipFree(resid_[p]);
     }
     hipFree(edge_);
-    hipFree(err_);
+    hipFree(ectr_);
+    if (err_) hipHostFree(err_);
 }
 
 bool StreamState::scratch_fits(const PrepLayout &L) const
 {
     const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
-    return err_ && cells <= owner_cap_[par_] && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
+    return err_ && ectr_ && cells <= owner_cap_[par_] && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
            (L.n_intra ? L.n_intra : 1) <= nbr_cap_[par_] &&
            (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_[par_];
 }
@@ -479,14 +480,15 @@ void StreamState::update_refs(bool key, FrameBuf *out)
     }
 }
 
-hipError_t StreamState::reset(hipStream_t s)
+hipError_t StreamState::reset()
 {
     for (int i = 0; i < 8; i++) {
         release(refs_[i]);
         refs_[i] = nullptr;
     }
     last_written_ = 1; /* a frame still in flight may read any slot's old frame */
-    return err_ ? hipMemsetAsync(err_, 0, 256, s) : hipSuccess;
+    if (err_) __atomic_store_n(err_, 0u, __ATOMIC_RELEASE);
+    return hipSuccess;
 }
 
 hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
@@ -494,8 +496,16 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
     hipError_t e;
     const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
     if (!err_) {
-        if ((e = hipMalloc(&err_, 256)) != hipSuccess) return e;
-        if ((e = hipMemsetAsync(err_, 0, 256, s)) != hipSuccess) return e;
+        if ((e = hipHostMalloc((void **)&err_, 64, hipHostMallocCoherent)) != hipSuccess) {
+            err_ = nullptr;
+            return e;
+        }
+        memset(err_, 0, 64);
+        if ((e = hipHostGetDevicePointer((void **)&err_dev_, err_, 0)) != hipSuccess) return e;
+    }
+    if (!ectr_) {
+        if ((e = hipMalloc(&ectr_, 256)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(ectr_, 0, 256, s)) != hipSuccess) return e;
     }
     const int p = par_;
     if (cells > owner_cap_[p]) {
@@ -575,7 +585,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     reader_[par_] = serial;
     if (two_sets_) par_ ^= 1;
     j.nv12 = nv12;
-    j.err = err_;
+    j.err = err_dev_;
+    j.ectr = ectr_;
     j.n_blocks = L.n_blocks;
     j.n_inter = L.n_inter;
     j.n_mc = L.n_mc;
@@ -583,8 +594,7 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.n_intra = L.n_intra;
     j.ow = (L.W + 3) / 4;
     j.oh = (L.H + 3) / 4;
-    if (++epoch_ == 0) epoch_ = 1;
-    j.epoch = epoch_;
+    j.epoch = 0; /* set on the device by the batch's first kernel */
     j.lf_thr = L.lf_thr;
     j.lf_wx = (L.W + 2 + 63) / 64;
     j.lf_wy = (L.H + 2 + 63) / 64;
@@ -596,7 +606,7 @@ size_t StreamState::resident_bytes() const
 {
     size_t b = (owner_cap_[0] + owner_cap_[1]) * 4 + edge_cap_ * CVP9_EDGE_WORDS * 8 +
                (nbr_cap_[0] + nbr_cap_[1]) * CVP9_NB_WORDS * 8 +
-               (resid_cap_[0] + resid_cap_[1]) * 2 + (err_ ? 256 : 0);
+               (resid_cap_[0] + resid_cap_[1]) * 2 + (ectr_ ? 256 : 0);
     for (const FrameBuf *f : all_) b += f->cap;
     return b;
 }

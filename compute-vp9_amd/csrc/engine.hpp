@@ -97,8 +97,8 @@ public:
      * (src/decoder/vp9_decoder.c:224-231) */
     void update_refs(bool key, FrameBuf *out);
     /* back to the state of a new stream (empty slots, no error), keeping
-     * device memory; work already queued on s is unaffected */
-    hipError_t reset(hipStream_t s);
+     * device memory; the caller has synchronised with the stream's work */
+    hipError_t reset();
     /* owner map, edge and neighbour records and residual scratch large
      * enough for L;
      * may synchronize s */
@@ -110,7 +110,12 @@ public:
                   uint64_t serial = 0);
     /* bytes of HBM this stream holds */
     size_t resident_bytes() const;
-    uint32_t *err_word() const { return err_; }
+    /* the stream's error word as the kernels see it (device pointer of a
+     * host-visible word), and its current value read on the host: nonzero
+     * once a dependency wait of the stream gave up (bit 0), a neighbour
+     * record was inconsistent (bit 8) or a grid barrier timed out (bit 16) */
+    uint32_t *err_word() const { return err_dev_; }
+    uint32_t error() const { return err_ ? __atomic_load_n(err_, __ATOMIC_ACQUIRE) : 0u; }
     FrameBuf *ref(int i) const { return refs_[i]; }
 
 private:
@@ -125,8 +130,13 @@ private:
     size_t nbr_cap_[2] = {0, 0}; /* intra blocks */
     int16_t *resid_[2] = {nullptr, nullptr};
     size_t resid_cap_[2] = {0, 0};
-    uint32_t *err_ = nullptr;
-    uint32_t epoch_ = 0;
+    /* error word: pinned, coherent host memory the kernels write with
+     * system-scope byte stores, so the host reads it without a copy */
+    uint32_t *err_ = nullptr, *err_dev_ = nullptr;
+    /* device word: the stream's frame epoch, advanced on the device by the
+     * first kernel of every batch (so a replayed graph gets new epochs);
+     * never reset, so edge words of earlier frames never match a new one */
+    uint32_t *ectr_ = nullptr;
     bool two_sets_ = false;
     int par_ = 0;
     uint64_t reader_[2] = {0, 0};
@@ -174,6 +184,9 @@ struct StagedResult {
     hipStream_t queue;
     uint32_t *err;
 };
+/* Reference slot `slot` of engine stream sid as the host pool tracks it
+ * (y == nullptr: empty). */
+DFrame engine_stream_ref(cvp9_engine_t *e, int sid, int slot);
 cvp9_err_t engine_run_staged(cvp9_engine_t *e, int sid, const PrepLayout &L, const uint8_t *host,
                              uint8_t *dev, uint8_t *nv12, StagedResult *res);
 

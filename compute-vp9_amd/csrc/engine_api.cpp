@@ -151,7 +151,6 @@ struct cvp9_engine {
     std::vector<hipEvent_t> free_events;
     double stage_ms[STAGE_COUNT] = {0};
     uint32_t stage_n[STAGE_COUNT] = {0};
-    std::vector<uint32_t *> err_words;
     bool fused = false;
     uint32_t fused_wgs = 0;
     cvp9_graph *capture = nullptr; /* recording batches into a graph */
@@ -331,7 +330,7 @@ cvp9_err_t cvp9_engine_stream_reset(cvp9_engine_t *e, int id)
     hipError_t he = e->sync_all();
     if (he != hipSuccess) return to_err(he);
     e->sq[id] = 0;
-    return to_err(e->streams[id]->reset(e->stream));
+    return to_err(e->streams[id]->reset());
 }
 
 void *cvp9_engine_hip_stream(cvp9_engine_t *e)
@@ -358,6 +357,12 @@ cvp9_err_t cvp9_engine_set_pipeline(cvp9_engine_t *e, int on)
     if (!e) return CVP9_ERR_INVALID_DATA;
     e->pipeline = on != 0;
     return CVP9_OK;
+}
+
+uint32_t cvp9_engine_stream_error(cvp9_engine_t *e, int id)
+{
+    if (!e || id < 0 || id >= (int)e->streams.size() || !e->streams[id]) return 0;
+    return e->streams[id]->error();
 }
 
 uint64_t cvp9_engine_stream_bytes(cvp9_engine_t *e, int id)
@@ -430,6 +435,15 @@ cvp9_err_t cvp9_engine_run(cvp9_engine_t *e, int n, const int *streams, cvp9_pre
 }
 
 namespace cvp9 {
+DFrame engine_stream_ref(cvp9_engine_t *e, int sid, int slot)
+{
+    DFrame f = {};
+    if (!e || sid < 0 || sid >= (int)e->streams.size() || !e->streams[sid] || slot < 0 || slot > 7)
+        return f;
+    const FrameBuf *b = e->streams[sid]->ref(slot);
+    return b ? b->dframe() : f;
+}
+
 cvp9_err_t engine_run_staged(cvp9_engine_t *e, int sid, const PrepLayout &L, const uint8_t *host,
                              uint8_t *dev, uint8_t *nv12, StagedResult *res)
 {
@@ -465,6 +479,13 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
         for (int k = 0; k < i; k++)
             if (streams[k] == id) return CVP9_ERR_INVALID_DATA;
     }
+    /* A stream whose earlier frame reported a device error (a dependency
+     * wait that gave up) takes no more frames until it is reset: its
+     * reference pool is not trustworthy, and a wait that keeps timing out
+     * would otherwise repeat batch after batch. The word is host memory
+     * the kernels write, so this costs no copy. */
+    for (int i = 0; i < n; i++)
+        if (e->streams[streams[i]]->error()) return CVP9_ERR_GPU;
     hipError_t he;
     /* the queue (see the engine's description) */
     const bool single = e->capture || e->fused || e->timing || e->nqueues < 2;
@@ -716,12 +737,8 @@ cvp9_err_t cvp9_engine_sync(cvp9_engine_t *e)
     hipSetDevice(e->device);
     if (e->sync_all() != hipSuccess) return CVP9_ERR_GPU;
     e->resolve_timing();
-    for (StreamState *s : e->streams) {
-        if (!s || !s->err_word()) continue;
-        uint32_t v = 0;
-        if (hipMemcpy(&v, s->err_word(), 4, hipMemcpyDeviceToHost) != hipSuccess) return CVP9_ERR_GPU;
-        if (v) return CVP9_ERR_GPU;
-    }
+    for (StreamState *s : e->streams)
+        if (s && s->error()) return CVP9_ERR_GPU;
     return CVP9_OK;
 }
 
@@ -744,11 +761,7 @@ cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int id, uint8_t *dst, uint64_t cap
         hipMemcpy2D(dst + (size_t)w * h + (size_t)cw * ch, cw, d.v, d.suv, cw, ch,
                     hipMemcpyDeviceToHost) != hipSuccess)
         return CVP9_ERR_GPU;
-    uint32_t v = 0;
-    if (e->streams[id]->err_word() &&
-        hipMemcpy(&v, e->streams[id]->err_word(), 4, hipMemcpyDeviceToHost) != hipSuccess)
-        return CVP9_ERR_GPU;
-    return v ? CVP9_ERR_GPU : CVP9_OK;
+    return e->streams[id]->error() ? CVP9_ERR_GPU : CVP9_OK;
 }
 
 cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t)

@@ -124,6 +124,38 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
                       __builtin_amdgcn_readlane(v, 47) + __builtin_amdgcn_readlane(v, 63));
 }
 
+/* Error reporting: the stream's error word lives in host memory (the host
+ * reads it without a copy); each cause sets its own byte with a plain
+ * system-scope store, so no read-modify-write crosses the bus. */
+__device__ __forceinline__ void flag_error(uint32_t *err, int byte)
+{
+    __hip_atomic_store((uint8_t *)err + byte, (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t read_error(const uint32_t *err)
+{
+    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+/* The batch's first kernel gives every job its frame epoch: the stream's
+ * counter advanced by one (0 is skipped: zeroed edge words never look
+ * valid). A stream appears at most once per batch and its batches run in
+ * order, so one thread per job updates the counter. Later kernels of the
+ * batch read the epoch from the job descriptor. */
+__device__ __forceinline__ uint32_t next_epoch(uint32_t *ctr)
+{
+    uint32_t e = *gm(ctr) + 1;
+    if (e == 0) e = 1;
+    *gm(ctr) = e;
+    return e;
+}
+__device__ __forceinline__ void assign_epochs(const DBatch &B)
+{
+    for (uint32_t j = threadIdx.x; j < B.njobs; j += blockDim.x) {
+        DJob &J = ((DJob *)B.jobs)[j];
+        gm(&J.epoch)[0] = next_epoch(J.ectr);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* 128 fill of the output frame and NONE fill of the owner map, for jobs    */
 /* whose blocks do not cover the frame (vp9_frame.c:27-30 initial value).   */
@@ -190,6 +222,7 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
 /* workgroup wg of the owner-map stage: 16 blocks, 16 lanes each */
 __device__ __forceinline__ void owner_wg(const DBatch &B, uint32_t wg)
 {
+    if (wg == 0) assign_epochs(B);
     const uint32_t total = B.pre_blocks[B.njobs];
     /* the wave's first block, wave-uniform */
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(wg * 16 + (threadIdx.x >> 6) * 4);
@@ -853,7 +886,7 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
                     } else if (kind == 2 && right_col) {
                         side = 1; lw = (y - 4 - oy) >> 2; ci = cby - 2 - (oy >> 1);
                     } else {
-                        atomicOr(J.err, 2u); /* not on an edge: impossible for disjoint blocks */
+                        flag_error(J.err, 1); /* not on an edge: impossible for disjoint blocks */
                         side = 0; lw = 0; ci = 0;
                     }
                     val = (uint64_t)o | (uint64_t)side << 32 | (uint64_t)(lw & 15) << 33 |
@@ -917,12 +950,10 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
         if (spins < 8) __builtin_amdgcn_s_sleep(1); /* 64 clocks */
         else __builtin_amdgcn_s_sleep(4);           /* 256 clocks */
         if (++spins > spin_limit) {
-            atomicOr(err, 1u);
+            flag_error(err, 0);
             break;
         }
-        if ((spins & 255) == 0 &&
-            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            break;
+        if ((spins & 255) == 0 && read_error(err)) break;
         if (sy) wy = __hip_atomic_load(py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (su) wu = __hip_atomic_load(pu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (sv) wv = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1082,7 +1113,9 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const GMEM int16_t *const jres = gm(J.resid);
     uint64_t *const jedge = J.edge;
     uint32_t *const jerr = J.err;
-    const uint32_t epoch = J.epoch;
+    /* (a vector load: in the fused kernel the epoch is written by another
+     * workgroup of the same launch, before a grid barrier) */
+    const uint32_t epoch = __hip_atomic_load(&((DJob &)J).epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     /* block index, block record, dependency mask and this thread's cell
      * all come from the neighbour record, in one round of loads */
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
@@ -1398,23 +1431,11 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     }
 }
 
-/* Persistent workers are spread over the batch's jobs in proportion to
- * their intra blocks: worker w starts on the job holding intra block
- * floor(w * total / nwk) of the batch, so job j's workers are
- * [first(j), first(j + 1)) with first(j) = ceil(pre_intra[j] * nwk / total). */
-__device__ __forceinline__ uint32_t job_first_worker(const DBatch &B, uint32_t j, uint32_t nwk,
-                                                     uint32_t total)
-{
-    return (uint32_t)(((uint64_t)B.pre_intra[j] * nwk + total - 1) / total);
-}
-
-/* A job of the batch that no worker started on and that has intra tickets
- * left, looked for by one whole wave from job j + 1 on, round robin;
- * B.njobs when there is none. Each lane reads one job's ticket counter: a
- * plain load, so workers that run out of work do not queue up on the
- * counters' atomics. */
-__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, uint32_t nwk,
-                                                  uint32_t total)
+/* A job of the batch with intra tickets left, looked for by one whole wave
+ * from job j + 1 on, round robin (job j itself last); B.njobs when there is
+ * none. Each lane reads one job's ticket counter with a plain load, so
+ * workers that run out of work do not queue up on the counters' atomics. */
+__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
 {
     const uint32_t lane = __lane_id();
     for (uint32_t k = 0; k < B.njobs; k += 64) {
@@ -1422,10 +1443,9 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, u
         while (idx >= B.njobs) idx -= B.njobs;
         bool open = false;
         if (k + lane < B.njobs)
-            open = job_first_worker(B, idx, nwk, total) == job_first_worker(B, idx + 1, nwk, total) &&
-                   B.jstart[idx] +
-                           __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                       B.jobs[idx].n_intra;
+            open = B.jstart[idx] +
+                       __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                   B.jobs[idx].n_intra;
         const uint64_t m = __ballot(open);
         if (m) return __shfl(idx, __builtin_ctzll(m));
     }
@@ -1436,19 +1456,20 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, u
  * (one wave per block, four independent workers per workgroup); this is
  * worker wk of nwk.
  *
- * A job's remaining intra-list entries [jstart, n_intra) are dealt out to
- * its c workers by stride: worker r of them takes entries jstart + r,
- * jstart + r + c, ... in that order. Nothing is fetched through an atomic
- * counter: a counter on one address hands out only some tens of entries
- * per microsecond, which held the intra stage's first levels back. The
- * lowest unfinished entry of a job always runs: its worker has finished
- * every earlier entry it owns, and its dependencies are lower entries,
- * finished too (every worker of the launch becomes resident: the launch is
- * held to what the device holds, see intra_resident_wgs). A job that no
- * worker started on (its share of the batch's intra blocks is below one
- * worker) is served through its ticket counter by the workers that have
- * finished their own entries: the tickets a worker holds are its current
- * one and a later one of the same job. */
+ * Entries are claimed, one at a time, from the job's ticket counter, in
+ * the order of the job's intra list (dependency level, then the blocks
+ * heading the longest chains first), starting at the job's jstart. A worker
+ * holds its current ticket and the next one, claimed while the current
+ * block runs. Workers start on the job holding intra block
+ * floor(wk * total / nwk) of the batch and, when its tickets run out, go on
+ * with any job that has tickets left.
+ *
+ * Every wait ends without any assumption about which workers are resident:
+ * the lowest unfinished entry of a job is claimed (tickets go out in list
+ * order, and every entry below it is finished, so a worker is free to claim
+ * it) by a running worker, whose current ticket it is (a worker's prefetched
+ * ticket is higher than its current one), and its dependencies are lower
+ * entries, finished. So the launch may be any size, beside any other work. */
 template <int W>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
                                              uint32_t nwk)
@@ -1456,27 +1477,27 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
     IntraShared &S = shm[threadIdx.x / W];
     const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
+    if (total == 0) return;
 
-    /* Job, entry and stride are made wave-uniform with readfirstlane, so
-     * every loop test is a scalar branch. With an LDS value as the exit
-     * test the compiler treats the exit as divergent and restructures the
-     * loop so that thread 0's ticket fetch runs outside the loop holding the
+    /* Job and ticket are made wave-uniform with readfirstlane, so every
+     * loop test is a scalar branch. With an LDS value as the exit test the
+     * compiler treats the exit as divergent and restructures the loop so
+     * that thread 0's ticket fetch runs outside the loop holding the
      * barriers: the other lanes then repeat one ticket forever. */
     uint32_t j = __builtin_amdgcn_readfirstlane(
         find_job(B.pre_intra, B.njobs, (uint32_t)((uint64_t)wk * total / nwk)));
-    const uint32_t f0 = job_first_worker(B, j, nwk, total);
-    uint32_t stride = __builtin_amdgcn_readfirstlane(job_first_worker(B, j + 1, nwk, total) - f0);
-    uint32_t tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + (wk - f0));
-    bool counted = false; /* entries from the job's ticket counter */
+    if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
+    worker_sync<W>();
+    uint32_t t = __builtin_amdgcn_readfirstlane(S.ticket);
     for (;;) {
         const DJob &J = B.jobs[j];
+        const uint32_t tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + t);
         if (tk >= J.n_intra) {
-            /* this worker's entries of the job are done: go on with a job
-             * that no worker started on and has tickets left (the first
-             * sync: every thread has read S.ticket) */
+            /* the job's tickets are gone: go on with a job that has some
+             * left (the first sync: every thread has read S.ticket) */
             worker_sync<W>();
             if (tid < 64) {
-                const uint32_t nj = find_open_job(B, j, nwk, total);
+                const uint32_t nj = find_open_job(B, j);
                 if (tid == 0) {
                     S.job = nj;
                     if (nj < B.njobs) S.ticket = atomicAdd(B.jticket + nj, 1u);
@@ -1485,22 +1506,20 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
             worker_sync<W>();
             j = __builtin_amdgcn_readfirstlane(S.job);
             if (j >= B.njobs) break;
-            tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + S.ticket);
-            counted = true;
+            t = __builtin_amdgcn_readfirstlane(S.ticket);
             continue;
         }
-        /* with a counted job, the next ticket is fetched while this block
-         * runs (atomicInc: the compiler's wave aggregation of atomicAdd
-         * would wait for the result right here; tickets never reach the
-         * wrap limit) */
+        /* the next ticket is claimed while this block runs (atomicInc: the
+         * compiler's wave aggregation of atomicAdd would wait for the result
+         * right here; tickets never reach the wrap limit) */
         uint32_t next = 0;
-        if (counted && tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
+        if (tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
         intra_block<W, true>(B, J, tk, S, tid);
 
         /* next entry; the sync also orders LDS reuse */
         if (tid == 0) S.ticket = next;
         worker_sync<W>();
-        tk = counted ? __builtin_amdgcn_readfirstlane(B.jstart[j] + S.ticket) : tk + stride;
+        t = __builtin_amdgcn_readfirstlane(S.ticket);
     }
 }
 
@@ -1711,7 +1730,7 @@ __device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
         while (__hip_atomic_load(B.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(2);
             if (++spins > B.spin_limit) {
-                atomicOr(B.jobs[0].err, 4u);
+                flag_error(B.jobs[0].err, 2);
                 break;
             }
         }
@@ -1745,6 +1764,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     const int wave = threadIdx.x >> 6;
     uint32_t phase = 0;
 
+    if (blockIdx.x == 0) assign_epochs(B); /* read by the intra phase, after a grid barrier */
     const uint32_t nfill = B.pre_fill[B.njobs];
     if (nfill) {
         for (uint32_t w = blockIdx.x; w < nfill; w += nwg) fill_item(B, w);
@@ -1801,31 +1821,6 @@ hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fused, dim3(wgs), dim3(256), 0, s, b);
     return hipGetLastError();
-}
-
-/* Half the workgroups of k_intra<W> that the current device holds at once
- * when nothing else runs (cached per device). Persistent workers own their
- * intra entries by stride, and a worker may wait on an entry of any other
- * worker of its job, so every workgroup of a launch must become resident
- * while the others wait. The kernels that run beside it never wait, so
- * they end and give their slots up; the only other launch that may wait is
- * the intra stage of a batch on the engine's other queue, which is held to
- * the other half. */
-template <int W>
-static uint32_t intra_resident_wgs()
-{
-    static uint32_t cache[16] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-    if (!cache[dev]) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra<W>, 256, 0) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            per_cu < 1 || cus < 1)
-            return 128;
-        cache[dev] = (uint32_t)(per_cu * cus) / 2;
-    }
-    return cache[dev];
 }
 
 uint32_t cvp9_fused_max_wgs(int device)
@@ -1977,13 +1972,11 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         }
         if (n.intra_tail) {
             const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;
-            if (n.tune.intra_lanes == 64) {
-                const uint32_t g = div_up(w, 4), cap = intra_resident_wgs<64>();
-                hipLaunchKernelGGL(k_intra<64>, dim3(g < cap ? g : cap), dim3(256), 0, s, b);
-            } else {
-                const uint32_t cap = intra_resident_wgs<256>();
-                hipLaunchKernelGGL(k_intra<256>, dim3(w < cap ? w : cap), dim3(256), 0, s, b);
-            }
+            /* any number of workers is deadlock-free (see intra_worker) */
+            if (n.tune.intra_lanes == 64)
+                hipLaunchKernelGGL(k_intra<64>, dim3(div_up(w, 4)), dim3(256), 0, s, b);
+            else
+                hipLaunchKernelGGL(k_intra<256>, dim3(w), dim3(256), 0, s, b);
         }
         break;
     case STAGE_LF:

@@ -53,6 +53,13 @@ cvp9_err_t cvp9_engine_set_queues(cvp9_engine_t *e, int n);
  * Graph capture, fused mode and per-stage timing never overlap batches.
  * Output never depends on it. */
 cvp9_err_t cvp9_engine_set_pipeline(cvp9_engine_t *e, int on);
+/* A stream's device error flags, read on the host with no copy: 0 while
+ * healthy, nonzero once a dependency wait of one of its frames gave up
+ * (bits 0-7), a neighbour record was inconsistent (bits 8-15) or a grid
+ * barrier of the fused kernel timed out (bits 16-23). A stream in error
+ * takes no more frames (cvp9_engine_run returns CVP9_ERR_GPU) until
+ * cvp9_engine_stream_reset. */
+uint32_t   cvp9_engine_stream_error(cvp9_engine_t *e, int stream_id);
 /* HBM held by a stream (frame pool, scratch). */
 uint64_t   cvp9_engine_stream_bytes(cvp9_engine_t *e, int stream_id);
 

@@ -45,8 +45,12 @@ void hip_backend_destroy(void *ctx);
  *   CVP9_ERR_AGAIN        CVP9_HIP_INFLIGHT frames in flight: get one first
  *   CVP9_ERR_UNSUPPORTED  layout this backend does not take (blocks off the
  *                         4x4 grid or overlapping, tx_size > 3, W or H = 1
- *                         mod 8, ...): nothing queued, the caller runs the
- *                         CPU path (vp9_decoder.c:209)
+ *                         mod 8, ...), or any non-KEY frame after such a
+ *                         frame: nothing queued, the caller runs the CPU
+ *                         path (vp9_decoder.c:209) until the next KEY frame
+ *                         (see hip_backend_read_ref)
+ * After a device error (CVP9_ERR_GPU from hip_get_frame) every later frame
+ * reports CVP9_ERR_GPU until a KEY frame, which starts the pool afresh.
  *   CVP9_ERR_INVALID_DATA NULL arguments or coefficient offsets past num_coeffs
  *   CVP9_ERR_GPU / CVP9_ERR_NOMEM on HIP failures. */
 cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts);
@@ -77,6 +81,19 @@ void hip_export_buffer_free(void *ctx, cvp9_export_buffer_t *buf);
  * bytes) for the next hip_decode_frame, which packs the frame into it on
  * the GPU. NULL disarms. */
 cvp9_err_t hip_set_render_target(void *ctx, const cvp9_export_buffer_t *target);
+
+/* No counterpart in the reference's Vulkan backend, which never declines a
+ * frame. For the reference's dispatch (src/decoder/vp9_decoder.c:209-247)
+ * when this backend declines a frame mid-stream: the CPU path that takes
+ * over reconstructs from ctx->ref_frames, which frames decoded here never
+ * updated. Reads reference slot `slot` (0..7) of the device pool as of the
+ * last frame accepted: with frame->plane_y == NULL it only sets
+ * frame->width/height (0 x 0 for an empty slot); otherwise frame must be
+ * that size and its I420 planes (stride_y, stride_uv) receive the slot's
+ * samples. Waits for queued work. After declining a frame the backend
+ * declines every frame until the next KEY frame (which writes all 8 slots),
+ * so the CPU path carries the stream until then. */
+cvp9_err_t hip_backend_read_ref(void *ctx, int slot, cvp9_frame_info_t *frame);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,12 @@ int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t
  * blocks (depth + 1 entries, at most cap), or a negative cvp9_err_t. */
 int cvp9_hip_intra_levels(const vp9_parsed_frame_t *pf, uint32_t *starts, uint32_t cap);
 
+/* The code hip_decode_frame would return for each frame of a sequence
+ * before any device work (CVP9_OK, CVP9_ERR_UNSUPPORTED or
+ * CVP9_ERR_INVALID_DATA), applying the validator and the rule that after a
+ * declined frame every frame is declined until the next KEY frame. Needs no
+ * device. Returns n or a negative cvp9_err_t. */
+int cvp9_hip_route_check(const vp9_parsed_frame_t *const *frames, int n, int *codes);
 /* Copies the first n bytes of an export buffer (hip_export_buffer_alloc)
  * to host memory through HIP, after the backend's queued work; for checking
  * what a render-target decode wrote independently of the CPU mapping. */
@@ -62,8 +68,11 @@ cvp9_err_t cvp9_hip_dmabuf_read(int fd, uint64_t size, uint64_t offset, uint8_t 
  * calibrate HBM traffic counters and to report a measured copy peak. */
 double cvp9_hip_copy_probe(uint64_t bytes, int iters);
 
-/* Debug flags for one backend context (bit 0: skip the deblocking stage). */
+/* Debug flags for one backend context (bit 0: skip the deblocking stage;
+ * bit 1: every dependency wait of later frames gives up after one poll, so
+ * a test can force the device-error path while every kernel still ends). */
 #define CVP9_HIP_DEBUG_NO_DEBLOCK 1u
+#define CVP9_HIP_DEBUG_SPIN_ONE   2u
 cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags);
 
 /* Bound on every dependency wait of the engine's later batches, in polls

@@ -118,15 +118,14 @@ def test_config4_serving_mix(pkg, orc, gpu_ready):
         with pkg.Server(eng, budget, max_batch=4096, max_inflight=2) as srv:
             handle, waiting = {}, []
             nxt, done = [0] * 16, [0] * 16
-            checked, waited, turn = 0, False, 0
+            checked, waited, turn, arrived = 0, False, 0, 0
             while checked < sum(lengths):
                 turn += 1
                 assert turn < 20000, "serving loop made no progress"
-                if turn % 2 == 0 and len(waiting) + len(handle) + sum(
-                        1 for s in range(16) if done[s] == lengths[s]) < 16:
-                    arrived = len(waiting) + len(handle) + sum(
-                        1 for s in range(16) if done[s] == lengths[s] and s not in handle)
-                    waiting.append(arrived)
+                for _ in range(2):  # two arrivals per loop turn, in stream order
+                    if arrived < 16:
+                        waiting.append(arrived)
+                        arrived += 1
                 while waiting:
                     h = srv.open(*sizes[waiting[0]])
                     if h is None:
