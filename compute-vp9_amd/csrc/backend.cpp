@@ -41,7 +41,8 @@ struct Slot {
     uint8_t *out_dev = nullptr;     size_t out_dev_cap = 0;   /* packed I420, VRAM */
     int out_fd = -1;                /* its DMA-BUF, or -1 */
     uint64_t out_off = 0;           /* out_dev's offset inside that DMA-BUF */
-    uint32_t *err_host = nullptr;
+    uint32_t *err_host = nullptr;   /* the stream's error word after this frame */
+    uint32_t *err_dev = nullptr;    /* its device view */
 };
 
 struct Backend {
@@ -198,7 +199,8 @@ cvp9_err_t hip_backend_init(void **ctx)
     }
     for (Slot &s : b->slots) {
         if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
-            hipHostMalloc((void **)&s.err_host, 64, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&s.err_host, 64, hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&s.err_dev, s.err_host, 0) != hipSuccess) {
             hip_backend_destroy(b);
             return CVP9_ERR_GPU;
         }
@@ -286,7 +288,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
         (b->pending_priv && b->pending_priv->shadow &&
          (e = hipMemcpyAsync(b->pending_priv->map, b->pending_priv->dev, ob, hipMemcpyDeviceToHost,
                              q))) ||
-        (e = hipMemcpyAsync(s.err_host, r.err, 4, hipMemcpyDefault, q)) ||
+        (e = cvp9_launch_copy_word(r.err, s.err_dev, q)) ||
         (e = hipEventRecord(s.done, q)))
         return to_err(e);
 
@@ -309,7 +311,7 @@ cvp9_err_t hip_get_frame(void *ctx, cvp9_frame_info_t *info, int wait)
     Slot *s = nullptr;
     cvp9_err_t err = acquire_oldest(b, wait, &s);
     if (err != CVP9_OK) return err;
-    if (*s->err_host) return CVP9_ERR_GPU;
+    if (__atomic_load_n(s->err_host, __ATOMIC_ACQUIRE)) return CVP9_ERR_GPU;
     const uint32_t w = s->w, h = s->h, cw = (w + 1) / 2, ch = (h + 1) / 2;
     info->width = w;
     info->height = h;
@@ -331,7 +333,7 @@ cvp9_err_t hip_get_frame_dmabuf(void *ctx, cvp9_dmabuf_frame_t *out)
     Slot *s = nullptr;
     cvp9_err_t err = acquire_oldest(b, 1, &s);
     if (err != CVP9_OK) return err;
-    if (*s->err_host) return CVP9_ERR_GPU;
+    if (__atomic_load_n(s->err_host, __ATOMIC_ACQUIRE)) return CVP9_ERR_GPU;
     if (s->out_fd < 0) return CVP9_ERR_UNSUPPORTED;
     const uint32_t w = s->w, h = s->h, cw = (w + 1) / 2, ch = (h + 1) / 2;
     out->fd = dup(s->out_fd);

@@ -78,6 +78,20 @@ cvp9_err_t to_err(hipError_t e)
     return CVP9_ERR_GPU;
 }
 
+/* Order-independent signature of a set of stream ids (nonzero). */
+uint64_t stream_set_signature(const int *streams, int n)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t)n;
+    for (int i = 0; i < n; i++) {
+        uint64_t x = (uint64_t)(uint32_t)streams[i] + 0x632be59bd9b4e019ull;
+        x ^= x >> 31;
+        x *= 0xbf58476d1ce4e5b9ull;
+        x ^= x >> 29;
+        h += x; /* a sum: the order of the ids does not matter */
+    }
+    return h ? h : 1;
+}
+
 uint64_t prep_algo_bytes(const vp9_parsed_frame_t *pf)
 {
     /* coefficients of transforms taken + 32-byte block records + reference
@@ -139,6 +153,10 @@ struct cvp9_engine {
     bool graph_pending[2] = {false, false}; /* phase queue q has not waited for it */
     int nqueues = 2;
     int last_q = 1;
+    /* per queue: the stream set of its latest batch (a signature) and that
+     * batch's serial, for the overlap rule in run_batch */
+    uint64_t qsig[2] = {0, 0};
+    uint64_t qserial[2] = {0, 0};
     std::vector<int8_t> sq;        /* per stream id: queue of its last batch, -1 none */
     hipEvent_t xev[2] = {nullptr, nullptr};
     std::vector<StreamState *> streams;
@@ -177,6 +195,13 @@ struct cvp9_engine {
             if (r != hipSuccess) return r;
         }
         return hipSuccess;
+    }
+
+    /* the latest batch on queue q may still be running */
+    bool queue_busy(int q) const
+    {
+        hipEvent_t ev = batch_event(qserial[q]);
+        return ev && hipEventQuery(ev) == hipErrorNotReady;
     }
 
     /* done event of the batch with this serial while it may still run */
@@ -495,6 +520,15 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
     else if (q < 0) q = 1 - e->last_q;
     bool other = false;
     for (int i = 0; i < n; i++) other |= e->sq[streams[i]] >= 0 && e->sq[streams[i]] != q;
+    /* Batches overlap on the two queues only in the pattern the tests cover:
+     * each queue's consecutive batches hold the same streams (a caller
+     * alternating two halves of its streams). A batch whose stream set
+     * differs from its queue's previous batch while the other queue is
+     * still busy (three or more stream groups alternating over the two
+     * queues) runs after the other queue's work instead: the overlapped
+     * form of that schedule stalled once on MI355X with no cause found. */
+    const uint64_t sig = stream_set_signature(streams, n);
+    if (!single && !other && e->qsig[q] && e->qsig[q] != sig && e->queue_busy(1 - q)) other = true;
     if (other && (he = e->order(1 - q, q)) != hipSuccess) return to_err(he);
     const hipStream_t qs = e->queue(q);
     const bool resid_all = e->pipeline && !e->fused;
@@ -649,6 +683,8 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
     }
     for (int i = 0; i < n; i++) e->sq[streams[i]] = (int8_t)q;
     e->last_q = q;
+    e->qsig[q] = sig;
+    e->qserial[q] = serial;
     return CVP9_OK;
 }
 } // namespace

@@ -69,4 +69,7 @@ hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_
 hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s);
 uint32_t cvp9_fused_max_wgs(int device);
 
+/* *dst = *src on queue s (a stream's error word into a per-frame host word). */
+hipError_t cvp9_launch_copy_word(const uint32_t *src, uint32_t *dst, hipStream_t s);
+
 #endif

@@ -1431,11 +1431,23 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     }
 }
 
-/* A job of the batch with intra tickets left, looked for by one whole wave
- * from job j + 1 on, round robin (job j itself last); B.njobs when there is
- * none. Each lane reads one job's ticket counter with a plain load, so
- * workers that run out of work do not queue up on the counters' atomics. */
-__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
+/* Persistent workers are spread over the batch's jobs in proportion to
+ * their intra blocks: worker w starts on the job holding intra block
+ * floor(w * total / nwk) of the batch, so job j's workers are
+ * [first(j), first(j + 1)) with first(j) = ceil(pre_intra[j] * nwk / total). */
+__device__ __forceinline__ uint32_t job_first_worker(const DBatch &B, uint32_t j, uint32_t nwk,
+                                                     uint32_t total)
+{
+    return (uint32_t)(((uint64_t)B.pre_intra[j] * nwk + total - 1) / total);
+}
+
+/* A job of the batch that no worker started on and that has intra tickets
+ * left, looked for by one whole wave from job j + 1 on, round robin;
+ * B.njobs when there is none. Each lane reads one job's ticket counter: a
+ * plain load, so workers that run out of work do not queue up on the
+ * counters' atomics. */
+__device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, uint32_t nwk,
+                                                  uint32_t total)
 {
     const uint32_t lane = __lane_id();
     for (uint32_t k = 0; k < B.njobs; k += 64) {
@@ -1443,9 +1455,10 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
         while (idx >= B.njobs) idx -= B.njobs;
         bool open = false;
         if (k + lane < B.njobs)
-            open = B.jstart[idx] +
-                       __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                   B.jobs[idx].n_intra;
+            open = job_first_worker(B, idx, nwk, total) == job_first_worker(B, idx + 1, nwk, total) &&
+                   B.jstart[idx] +
+                           __hip_atomic_load(B.jticket + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       B.jobs[idx].n_intra;
         const uint64_t m = __ballot(open);
         if (m) return __shfl(idx, __builtin_ctzll(m));
     }
@@ -1456,20 +1469,19 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j)
  * (one wave per block, four independent workers per workgroup); this is
  * worker wk of nwk.
  *
- * Entries are claimed, one at a time, from the job's ticket counter, in
- * the order of the job's intra list (dependency level, then the blocks
- * heading the longest chains first), starting at the job's jstart. A worker
- * holds its current ticket and the next one, claimed while the current
- * block runs. Workers start on the job holding intra block
- * floor(wk * total / nwk) of the batch and, when its tickets run out, go on
- * with any job that has tickets left.
- *
- * Every wait ends without any assumption about which workers are resident:
- * the lowest unfinished entry of a job is claimed (tickets go out in list
- * order, and every entry below it is finished, so a worker is free to claim
- * it) by a running worker, whose current ticket it is (a worker's prefetched
- * ticket is higher than its current one), and its dependencies are lower
- * entries, finished. So the launch may be any size, beside any other work. */
+ * A job's remaining intra-list entries [jstart, n_intra) are dealt out to
+ * its c workers by stride: worker r of them takes entries jstart + r,
+ * jstart + r + c, ... in that order. Nothing is fetched through an atomic
+ * counter: a counter on one address hands out only some tens of entries
+ * per microsecond, which held the intra stage's first levels back. The
+ * lowest unfinished entry of a job always runs: its worker has finished
+ * every earlier entry it owns, and its dependencies are lower entries,
+ * finished too (every worker of the launch becomes resident: the launch is
+ * held to what the device holds, see intra_resident_wgs). A job that no
+ * worker started on (its share of the batch's intra blocks is below one
+ * worker) is served through its ticket counter by the workers that have
+ * finished their own entries: the tickets a worker holds are its current
+ * one and a later one of the same job. */
 template <int W>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
                                              uint32_t nwk)
@@ -1477,27 +1489,27 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
     IntraShared &S = shm[threadIdx.x / W];
     const int tid = threadIdx.x % W;
     const uint32_t total = B.pre_intra[B.njobs];
-    if (total == 0) return;
 
-    /* Job and ticket are made wave-uniform with readfirstlane, so every
-     * loop test is a scalar branch. With an LDS value as the exit test the
-     * compiler treats the exit as divergent and restructures the loop so
-     * that thread 0's ticket fetch runs outside the loop holding the
+    /* Job, entry and stride are made wave-uniform with readfirstlane, so
+     * every loop test is a scalar branch. With an LDS value as the exit
+     * test the compiler treats the exit as divergent and restructures the
+     * loop so that thread 0's ticket fetch runs outside the loop holding the
      * barriers: the other lanes then repeat one ticket forever. */
     uint32_t j = __builtin_amdgcn_readfirstlane(
         find_job(B.pre_intra, B.njobs, (uint32_t)((uint64_t)wk * total / nwk)));
-    if (tid == 0) S.ticket = atomicAdd(B.jticket + j, 1u);
-    worker_sync<W>();
-    uint32_t t = __builtin_amdgcn_readfirstlane(S.ticket);
+    const uint32_t f0 = job_first_worker(B, j, nwk, total);
+    uint32_t stride = __builtin_amdgcn_readfirstlane(job_first_worker(B, j + 1, nwk, total) - f0);
+    uint32_t tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + (wk - f0));
+    bool counted = false; /* entries from the job's ticket counter */
     for (;;) {
         const DJob &J = B.jobs[j];
-        const uint32_t tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + t);
         if (tk >= J.n_intra) {
-            /* the job's tickets are gone: go on with a job that has some
-             * left (the first sync: every thread has read S.ticket) */
+            /* this worker's entries of the job are done: go on with a job
+             * that no worker started on and has tickets left (the first
+             * sync: every thread has read S.ticket) */
             worker_sync<W>();
             if (tid < 64) {
-                const uint32_t nj = find_open_job(B, j);
+                const uint32_t nj = find_open_job(B, j, nwk, total);
                 if (tid == 0) {
                     S.job = nj;
                     if (nj < B.njobs) S.ticket = atomicAdd(B.jticket + nj, 1u);
@@ -1506,20 +1518,22 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
             worker_sync<W>();
             j = __builtin_amdgcn_readfirstlane(S.job);
             if (j >= B.njobs) break;
-            t = __builtin_amdgcn_readfirstlane(S.ticket);
+            tk = __builtin_amdgcn_readfirstlane(B.jstart[j] + S.ticket);
+            counted = true;
             continue;
         }
-        /* the next ticket is claimed while this block runs (atomicInc: the
-         * compiler's wave aggregation of atomicAdd would wait for the result
-         * right here; tickets never reach the wrap limit) */
+        /* with a counted job, the next ticket is fetched while this block
+         * runs (atomicInc: the compiler's wave aggregation of atomicAdd
+         * would wait for the result right here; tickets never reach the
+         * wrap limit) */
         uint32_t next = 0;
-        if (tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
+        if (counted && tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
         intra_block<W, true>(B, J, tk, S, tid);
 
         /* next entry; the sync also orders LDS reuse */
         if (tid == 0) S.ticket = next;
         worker_sync<W>();
-        t = __builtin_amdgcn_readfirstlane(S.ticket);
+        tk = counted ? __builtin_amdgcn_readfirstlane(B.jstart[j] + S.ticket) : tk + stride;
     }
 }
 
@@ -1823,6 +1837,31 @@ hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s)
     return hipGetLastError();
 }
 
+/* Half the workgroups of k_intra<W> that the current device holds at once
+ * when nothing else runs (cached per device). Persistent workers own their
+ * intra entries by stride, and a worker may wait on an entry of any other
+ * worker of its job, so every workgroup of a launch must become resident
+ * while the others wait. The kernels that run beside it never wait, so
+ * they end and give their slots up; the only other launch that may wait is
+ * the intra stage of a batch on the engine's other queue, which is held to
+ * the other half. */
+template <int W>
+static uint32_t intra_resident_wgs()
+{
+    static uint32_t cache[16] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    if (!cache[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra<W>, 256, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per_cu < 1 || cus < 1)
+            return 128;
+        cache[dev] = (uint32_t)(per_cu * cus) / 2;
+    }
+    return cache[dev];
+}
+
 uint32_t cvp9_fused_max_wgs(int device)
 {
     int per_cu = 0, cus = 0;
@@ -1847,6 +1886,21 @@ hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipS
 {
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
+/* Snapshot of a stream's error word into a per-frame host word, in queue
+ * order (both live in host memory, which the device reaches with
+ * system-scope accesses; a host-to-host memory copy would block the
+ * caller until the queue drains). */
+__global__ void k_copy_word(const uint32_t *src, uint32_t *dst)
+{
+    __hip_atomic_store(dst, read_error(src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t cvp9_launch_copy_word(const uint32_t *src, uint32_t *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(1), 0, s, src, dst);
     return hipGetLastError();
 }
 
@@ -1972,11 +2026,13 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         }
         if (n.intra_tail) {
             const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;
-            /* any number of workers is deadlock-free (see intra_worker) */
-            if (n.tune.intra_lanes == 64)
-                hipLaunchKernelGGL(k_intra<64>, dim3(div_up(w, 4)), dim3(256), 0, s, b);
-            else
-                hipLaunchKernelGGL(k_intra<256>, dim3(w), dim3(256), 0, s, b);
+            if (n.tune.intra_lanes == 64) {
+                const uint32_t g = div_up(w, 4), cap = intra_resident_wgs<64>();
+                hipLaunchKernelGGL(k_intra<64>, dim3(g < cap ? g : cap), dim3(256), 0, s, b);
+            } else {
+                const uint32_t cap = intra_resident_wgs<256>();
+                hipLaunchKernelGGL(k_intra<256>, dim3(w < cap ? w : cap), dim3(256), 0, s, b);
+            }
         }
         break;
     case STAGE_LF:

@@ -194,3 +194,26 @@ def test_pipelined_resolution_changes(pkg, orc, gpu_ready):
         w, h, _ = segs[-1]
         eng.sync()
         assert np.array_equal(eng.read(sid, w, h), want[-1])
+
+
+@pytest.mark.skip(reason="the overlapped form of four stream groups alternating over the engine's "
+                         "two queues stalled once on MI355X (no output for 180 s) and no cause has "
+                         "been shown; the engine now runs a batch whose streams differ from its "
+                         "queue's previous batch after the other queue's work, and the schedule stays "
+                         "off the GPU until a cause is found")
+def test_four_groups_alternating(pkg, orc, gpu_ready):
+    sizes = [(136, 72), (200, 120), (264, 136), (320, 184)] * 2
+    n = 3
+    seqs = [[pkg.SynthFrame(pkg.synth_params(1, w, h, 400 + s), i) for i in range(n)]
+            for s, (w, h) in enumerate(sizes)]
+    want = [orc.run_oracle(q) for q in seqs]
+    with pkg.Engine(0) as eng:
+        sids = [eng.open_stream() for _ in sizes]
+        preps = [[eng.prepare(f) for f in q] for q in seqs]
+        for i in range(n):
+            for g in range(4):
+                mine = [s for s in range(len(sizes)) if s % 4 == g]
+                eng.run([sids[s] for s in mine], [preps[s][i] for s in mine])
+        eng.sync()
+        for s, (w, h) in enumerate(sizes):
+            assert np.array_equal(eng.read(sids[s], w, h), want[s][n - 1]), f"stream {s}"
