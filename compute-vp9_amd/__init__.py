@@ -128,6 +128,11 @@ class ExportBuffer(C.Structure):
     ]
 
 
+class FrameBuf(C.Structure):
+    """cvp9_frame_buf_t (include/cvp9_hip/engine.h): a device frame buffer."""
+    _fields_ = [("base", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
 class Tune(C.Structure):
     """cvp9_tune_t (include/cvp9_hip/engine.h)."""
     _fields_ = [("res_split", C.c_uint8 * 4), ("intra_lanes", C.c_uint16),
@@ -232,6 +237,12 @@ SIGNATURES = {
                                        C.c_uint64]),
     "hip_backend_set_debug": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cvp9_hip_copy_probe": (C.c_double, [C.c_uint64, C.c_int]),
+    "cvp9_hip_dev_alloc": (C.c_void_p, [C.c_uint64]),
+    "cvp9_hip_dev_free": (None, [C.c_void_p]),
+    "cvp9_hip_dev_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8), C.c_uint64]),
+    "cvp9_frame_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+    "cvp9_engine_run_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -368,6 +379,46 @@ def frame_info_to_i420(info):
 
 def i420_size(w, h):
     return w * h + 2 * ((w + 1) // 2) * ((h + 1) // 2)
+
+
+class DeviceFrame:
+    """A zero-filled device frame buffer in the engine's layout (test use)."""
+
+    def __init__(self, w, h):
+        self.w, self.h = w, h
+        self.nbytes = int(lib().cvp9_frame_bytes(w, h))
+        self.ptr = lib().cvp9_hip_dev_alloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError("cvp9_hip_dev_alloc failed")
+        self.buf = FrameBuf(self.ptr, w, h)
+
+    def read_i420(self):
+        """Packed I420 (Y W*H, U, V ceil(W/2)*ceil(H/2)) of the buffer."""
+        raw = np.zeros(self.nbytes, np.uint8)
+        check(lib().cvp9_hip_dev_read(self.ptr, raw.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                      self.nbytes), "cvp9_hip_dev_read")
+        w, h = self.w, self.h
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        sy, suv = (w + 63) // 64 * 64, (cw + 63) // 64 * 64
+        y = raw[:sy * h].reshape(h, sy)[:, :w]
+        u = raw[sy * h:sy * h + suv * ch].reshape(ch, suv)[:, :cw]
+        v = raw[sy * h + suv * ch:sy * h + 2 * suv * ch].reshape(ch, suv)[:, :cw]
+        return np.concatenate([y.ravel(), u.ravel(), v.ravel()])
+
+    def free(self):
+        if self.ptr:
+            lib().cvp9_hip_dev_free(self.ptr)
+            self.ptr = None
+
+
+def run_static(prep, refs, out, nv12=None, status=None, stream=None):
+    """cvp9_engine_run_static: refs is a list of 8 DeviceFrame or None."""
+    arr = (FrameBuf * 8)()
+    for i, r in enumerate(refs or []):
+        if r is not None:
+            arr[i] = r.buf
+    return lib().cvp9_engine_run_static(prep, C.cast(arr, C.c_void_p), C.cast(C.pointer(out.buf), C.c_void_p),
+                                        nv12, status, stream)
 
 
 def route_check(frames):

@@ -602,6 +602,29 @@ cvp9_err_t hip_backend_set_debug(void *ctx, uint32_t flags)
     return cvp9_engine_set_spin_limit(b->eng, (flags & CVP9_HIP_DEBUG_SPIN_ONE) ? 1u : 0u);
 }
 
+void *cvp9_hip_dev_alloc(uint64_t bytes)
+{
+    void *p = nullptr;
+    if (!bytes || hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, bytes) != hipSuccess) {
+        hipFree(p);
+        return nullptr;
+    }
+    return p;
+}
+
+void cvp9_hip_dev_free(void *p)
+{
+    if (p) hipFree(p);
+}
+
+cvp9_err_t cvp9_hip_dev_read(const void *src, uint8_t *dst, uint64_t n)
+{
+    if (!src || !dst) return CVP9_ERR_INVALID_DATA;
+    if (hipDeviceSynchronize() != hipSuccess) return CVP9_ERR_GPU;
+    return hipMemcpy(dst, src, n, hipMemcpyDeviceToHost) == hipSuccess ? CVP9_OK : CVP9_ERR_GPU;
+}
+
 int cvp9_hip_route_check(const vp9_parsed_frame_t *const *frames, int n, int *codes)
 {
     if (n < 0 || (n && (!frames || !codes))) return CVP9_ERR_INVALID_DATA;

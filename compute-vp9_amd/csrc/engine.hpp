@@ -198,4 +198,11 @@ void idct_matrix(int n, double *m);
 
 } // namespace cvp9
 
+/* A parsed frame staged in HBM (cvp9_engine_prepare). */
+struct cvp9_prep {
+    cvp9::PrepLayout L;
+    uint8_t *dev = nullptr;
+    uint64_t algo = 0;
+};
+
 #endif

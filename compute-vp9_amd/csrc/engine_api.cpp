@@ -16,11 +16,6 @@ static_assert(CVP9_ENGINE_STAGES == STAGE_COUNT, "stage list of the ABI and the 
 
 using namespace cvp9;
 
-struct cvp9_prep {
-    PrepLayout L;
-    uint8_t *dev = nullptr;
-    uint64_t algo = 0;
-};
 
 namespace {
 

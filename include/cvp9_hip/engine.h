@@ -142,6 +142,48 @@ cvp9_err_t cvp9_engine_set_timing(cvp9_engine_t *e, int on);
 cvp9_err_t cvp9_engine_stage_stats(cvp9_engine_t *e, double *ms, uint32_t *launches,
                                    int reset);
 
+/* A frame buffer in device memory in the engine's layout: Y rows of
+ * roundup(W, 64) bytes (H rows), then U and V rows of roundup(ceil(W/2), 64)
+ * bytes (ceil(H/2) rows each), then 64 bytes of slack; base 256-byte
+ * aligned (hipMalloc's alignment). */
+typedef struct {
+    uint8_t *base;
+    uint32_t width, height;
+} cvp9_frame_buf_t;
+/* Bytes of such a buffer for a width x height frame. */
+uint64_t cvp9_frame_bytes(uint32_t width, uint32_t height);
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cvp9_engine_run for one frame, with the reference
+ * frames and the output frame held by the caller, and every per-frame
+ * scratch buffer (batch descriptor, owner map, intra edge and neighbour
+ * records, residuals, error word) taken from a fixed device array of
+ * CVP9_STATIC_SCRATCH_BYTES that the library holds on each GPU until the
+ * process ends. It calls no hipMalloc, hipFree or other allocating routine.
+ *
+ *   prep     a prepared frame (cvp9_engine_prepare), W x H;
+ *   refs     the 8 reference slots (refs[i].base == NULL or refs == NULL:
+ *            empty slot, predicted as flat 127), each any size;
+ *   out      the output frame, W x H; must not be one of the refs;
+ *   nv12     optional device pointer receiving the frame as NV12 (pitch W);
+ *   status   optional device-visible word receiving the frame's device error
+ *            flags (0: none; see cvp9_engine_stream_error), in stream order;
+ *   stream   the hipStream_t to enqueue on (NULL: the null stream).
+ * Asynchronous. The reference pool rule (KEY: the frame goes to all 8 slots,
+ * else slot 0) is the caller's. Output is bit-exact to cvp9_engine_run on
+ * the same frame and references. Limits: at most 2^20 4x4 luma cells,
+ * 49152 blocks, 26880 intra blocks and 12 Mi coefficients per frame (a
+ * 3840x2160 frame of the benchmark workloads fits); a larger frame returns
+ * CVP9_ERR_UNSUPPORTED without enqueuing anything. Every call on a GPU shares
+ * that GPU's scratch array: calls that overlap on one GPU (other streams,
+ * other threads) race, and nothing beside the call may keep waiting kernels
+ * resident (the intra stage needs its workers co-resident). */
+#define CVP9_STATIC_SCRATCH_BYTES (60ull * 1024 * 1024 + 8192)
+cvp9_err_t cvp9_engine_run_static(const cvp9_prep_t *prep, const cvp9_frame_buf_t *refs,
+                                  const cvp9_frame_buf_t *out, uint8_t *nv12, uint32_t *status,
+                                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,12 @@ cvp9_err_t cvp9_hip_dmabuf_read(int fd, uint64_t size, uint64_t offset, uint8_t 
  * calibrate HBM traffic counters and to report a measured copy peak. */
 double cvp9_hip_copy_probe(uint64_t bytes, int iters);
 
+/* Plain device memory for tests of entry points that take device buffers:
+ * zero-filled memory on the current device (NULL on failure), its release,
+ * and a blocking copy of n bytes from device memory to host memory. */
+void *cvp9_hip_dev_alloc(uint64_t bytes);
+void cvp9_hip_dev_free(void *p);
+cvp9_err_t cvp9_hip_dev_read(const void *src, uint8_t *dst, uint64_t n);
 /* Debug flags for one backend context (bit 0: skip the deblocking stage;
  * bit 1: every dependency wait of later frames gives up after one poll, so
  * a test can force the device-error path while every kernel still ends). */
