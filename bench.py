@@ -40,7 +40,7 @@ METRIC = "recon frames/s @1080p (1 vs 64 streams), p50/p99 frame latency, HBM GB
 MIX = [(640, 360), (854, 480), (1280, 720), (1920, 1080)]
 
 
-KERNEL_OF = {"intra": "k_intra", "neighbours": "k_nbr", "mc": "k_mc", "deblock": "k_lf",
+KERNEL_OF = {"intra": "k_intra", "neighbours": "k_nbr", "mc": "k_mc", "deblock": "k_lf<256>",
              "owner": "k_owner",
              "fill": "k_fill", "nv12": "k_nv12", "residual4": "k_residual<4>",
              "residual8": "k_residual<8>", "residual16": "k_residual<16>",
@@ -349,21 +349,28 @@ def serving_saturated(P, eng, preps, layout, F, args):
     assert all(h is not None for h in hs)
     nxt, done, sub_ns, lat = [0] * n, [0] * n, {}, []
 
-    def submit(s):
-        k = nxt[s]
-        idx = k if k < F else 1 + (k - 1) % (F - 1)
-        tag = s * 100000 + k
-        sub_ns[tag] = time.monotonic_ns()
-        srv.submit(hs[s], preps[s * F + idx], tag)
-        nxt[s] += 1
+    def submit(streams):
+        """the next frame of each stream, in one call"""
+        ids, ps, tags = [], [], []
+        now = time.monotonic_ns()
+        for s in streams:
+            k = nxt[s]
+            idx = k if k < F else 1 + (k - 1) % (F - 1)
+            tag = s * 100000 + k
+            sub_ns[tag] = now
+            ids.append(hs[s])
+            ps.append(preps[s * F + idx])
+            tags.append(tag)
+            nxt[s] += 1
+        srv.submit_many(ids, ps, tags)
 
     t0 = time.monotonic_ns()
-    for s in range(n):
-        for _ in range(2):
-            submit(s)
+    submit(range(n))
+    submit(range(n))
     finished = 0
     while finished < n:
         srv.pump()
+        more = []
         for tag, dns in srv.poll():
             tag = int(tag)
             s = tag // 100000
@@ -372,7 +379,9 @@ def serving_saturated(P, eng, preps, layout, F, args):
             if done[s] == total:
                 finished += 1
             elif nxt[s] < total:
-                submit(s)
+                more.append(s)
+        if more:
+            submit(more)
     wall = (time.monotonic_ns() - t0) / 1e9
     srv.drain()
     for h in hs:
@@ -632,7 +641,7 @@ def main():
     ap.add_argument("--mwarmup", type=int, default=5)
     ap.add_argument("--mreps", type=int, default=5,
                     help="timed runs of each multi-stream schedule (median reported, with min/max)")
-    ap.add_argument("--sat-frames", type=int, default=30,
+    ap.add_argument("--sat-frames", type=int, default=120,
                     help="frames per stream of the saturated 64-stream serving run")
     ap.add_argument("--real-budget-gib", type=float, default=200.0,
                     help="HBM budget of the serving runs that stand for the device's memory")

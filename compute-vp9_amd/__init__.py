@@ -216,6 +216,8 @@ SIGNATURES = {
     "cvp9_server_open": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
     "cvp9_server_close": (None, [C.c_void_p, C.c_int]),
     "cvp9_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
+    "cvp9_server_submit_n": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
     "cvp9_server_pump": (C.c_int, [C.c_void_p]),
     "cvp9_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    C.c_int]),
@@ -678,6 +680,16 @@ class Server:
 
     def submit(self, sid, prep, tag):
         check(lib().cvp9_server_submit(self.s, sid, prep, tag), "server_submit")
+
+    def submit_many(self, sids, preps, tags):
+        """cvp9_server_submit_n: frames of several streams in one call."""
+        n = len(sids)
+        if n == 0:
+            return
+        check(lib().cvp9_server_submit_n(self.s, n, (C.c_int * n)(*sids),
+                                         (C.c_void_p * n)(*[p.value if hasattr(p, "value") else p
+                                                             for p in preps]),
+                                         (C.c_uint64 * n)(*tags)), "server_submit_n")
 
     def pump(self):
         r = lib().cvp9_server_pump(self.s)

@@ -137,9 +137,11 @@ struct DJob {
     uint64_t       *nbr;           /* per intra-list entry: CVP9_NB_WORDS words */
     int16_t        *resid;         /* intra residuals, indexed like coeffs */
     uint8_t        *nv12;          /* optional NV12 target, pitch W */
-    uint32_t       *err;           /* error word (host-visible): byte 0 a dependency wait
-                                      gave up, byte 1 a neighbour record was inconsistent,
-                                      byte 2 a grid barrier timed out */
+    uint32_t       *err;           /* error word (device memory, polled by waits): byte 0
+                                      a dependency wait gave up, byte 1 a neighbour record
+                                      was inconsistent, byte 2 a grid barrier timed out */
+    uint32_t       *err_host;      /* the same flags in host-visible memory, set with
+                                      them, so the host reads them without a copy */
     uint32_t       *ectr;          /* the stream's epoch counter (device) */
     uint32_t n_blocks, n_inter, n_intra, n_mc;
     uint32_t n_mc_a;               /* motion-compensation units [0, n_mc_a) run in the

@@ -413,13 +413,14 @@ StreamState::~StreamState()
     }
     hipFree(edge_);
     hipFree(ectr_);
+    hipFree(errd_);
     if (err_) hipHostFree(err_);
 }
 
 bool StreamState::scratch_fits(const PrepLayout &L) const
 {
     const size_t cells = (((size_t)((L.W + 3) / 4) * ((L.H + 3) / 4)) + 3) & ~(size_t)3;
-    return err_ && ectr_ && cells <= owner_cap_[par_] && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
+    return err_ && errd_ && ectr_ && cells <= owner_cap_[par_] && (L.n_blocks ? L.n_blocks : 1) <= edge_cap_ &&
            (L.n_intra ? L.n_intra : 1) <= nbr_cap_[par_] &&
            (L.num_coeffs ? L.num_coeffs : 1) <= resid_cap_[par_];
 }
@@ -488,7 +489,7 @@ hipError_t StreamState::reset()
     }
     last_written_ = 1; /* a frame still in flight may read any slot's old frame */
     if (err_) __atomic_store_n(err_, 0u, __ATOMIC_RELEASE);
-    return hipSuccess;
+    return errd_ ? hipMemset(errd_, 0, 256) : hipSuccess;
 }
 
 hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
@@ -502,6 +503,10 @@ hipError_t StreamState::ensure_scratch(const PrepLayout &L, hipStream_t s)
         }
         memset(err_, 0, 64);
         if ((e = hipHostGetDevicePointer((void **)&err_dev_, err_, 0)) != hipSuccess) return e;
+    }
+    if (!errd_) {
+        if ((e = hipMalloc(&errd_, 256)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(errd_, 0, 256, s)) != hipSuccess) return e;
     }
     if (!ectr_) {
         if ((e = hipMalloc(&ectr_, 256)) != hipSuccess) return e;
@@ -585,7 +590,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     reader_[par_] = serial;
     if (two_sets_) par_ ^= 1;
     j.nv12 = nv12;
-    j.err = err_dev_;
+    j.err = errd_;
+    j.err_host = err_dev_;
     j.ectr = ectr_;
     j.n_blocks = L.n_blocks;
     j.n_inter = L.n_inter;

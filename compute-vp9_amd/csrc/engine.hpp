@@ -110,11 +110,11 @@ public:
                   uint64_t serial = 0);
     /* bytes of HBM this stream holds */
     size_t resident_bytes() const;
-    /* the stream's error word as the kernels see it (device pointer of a
-     * host-visible word), and its current value read on the host: nonzero
-     * once a dependency wait of the stream gave up (bit 0), a neighbour
-     * record was inconsistent (bit 8) or a grid barrier timed out (bit 16) */
-    uint32_t *err_word() const { return err_dev_; }
+    /* the stream's error word in device memory (the kernels poll it), and
+     * its host-visible copy's value read on the host: nonzero once a
+     * dependency wait of the stream gave up (bit 0), a neighbour record was
+     * inconsistent (bit 8) or a grid barrier timed out (bit 16) */
+    uint32_t *err_word() const { return errd_; }
     uint32_t error() const { return err_ ? __atomic_load_n(err_, __ATOMIC_ACQUIRE) : 0u; }
     FrameBuf *ref(int i) const { return refs_[i]; }
 
@@ -130,8 +130,11 @@ private:
     size_t nbr_cap_[2] = {0, 0}; /* intra blocks */
     int16_t *resid_[2] = {nullptr, nullptr};
     size_t resid_cap_[2] = {0, 0};
-    /* error word: pinned, coherent host memory the kernels write with
-     * system-scope byte stores, so the host reads it without a copy */
+    /* error word: errd_ in device memory (polled by waits), err_ its copy
+     * in pinned, coherent host memory that the kernels set with
+     * system-scope byte stores (err_dev_ its device view), so the host reads
+     * it without a copy */
+    uint32_t *errd_ = nullptr;
     uint32_t *err_ = nullptr, *err_dev_ = nullptr;
     /* device word: the stream's frame epoch, advanced on the device by the
      * first kernel of every batch (so a replayed graph gets new epochs);

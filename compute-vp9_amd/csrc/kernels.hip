@@ -124,16 +124,21 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
                       __builtin_amdgcn_readlane(v, 47) + __builtin_amdgcn_readlane(v, 63));
 }
 
-/* Error reporting: the stream's error word lives in host memory (the host
- * reads it without a copy); each cause sets its own byte with a plain
- * system-scope store, so no read-modify-write crosses the bus. */
-__device__ __forceinline__ void flag_error(uint32_t *err, int byte)
+/* Error reporting. Each cause sets its own byte of the stream's error
+ * word: in device memory, which waits poll to give up early once another
+ * wait of the stream has (a host-memory word there would put a bus read in
+ * every long wait), and in a host-visible copy with a plain system-scope
+ * byte store (no read-modify-write crosses the bus), which the host reads
+ * without a copy. */
+__device__ __forceinline__ void flag_error(uint32_t *err, uint32_t *err_host, int byte)
 {
-    __hip_atomic_store((uint8_t *)err + byte, (uint8_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_fetch_or(err, 1u << (8 * byte), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((uint8_t *)err_host + byte, (uint8_t)1, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ uint32_t read_error(const uint32_t *err)
 {
-    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* The batch's first kernel gives every job its frame epoch: the stream's
@@ -886,7 +891,7 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
                     } else if (kind == 2 && right_col) {
                         side = 1; lw = (y - 4 - oy) >> 2; ci = cby - 2 - (oy >> 1);
                     } else {
-                        flag_error(J.err, 1); /* not on an edge: impossible for disjoint blocks */
+                        flag_error(J.err, J.err_host, 1); /* not on an edge: impossible for disjoint blocks */
                         side = 0; lw = 0; ci = 0;
                     }
                     val = (uint64_t)o | (uint64_t)side << 32 | (uint64_t)(lw & 15) << 33 |
@@ -930,7 +935,7 @@ __global__ __launch_bounds__(256) void k_nbr(DBatch B)
  * reads), so thousands of waiting lanes do not crowd the memory system
  * that the blocks they wait for are using. */
 __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32_t epoch, uint32_t *err,
-                                                    uint64_t d, uint32_t spin_limit)
+                                                    uint32_t *err_host, uint64_t d, uint32_t spin_limit)
 {
     const uint32_t o = (uint32_t)d;
     const uint32_t side = (uint32_t)(d >> 32) & 1, lw = (uint32_t)(d >> 33) & 15;
@@ -950,7 +955,7 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
         if (spins < 8) __builtin_amdgcn_s_sleep(1); /* 64 clocks */
         else __builtin_amdgcn_s_sleep(4);           /* 256 clocks */
         if (++spins > spin_limit) {
-            flag_error(err, 0);
+            flag_error(err, err_host, 0);
             break;
         }
         if ((spins & 255) == 0 && read_error(err)) break;
@@ -1102,9 +1107,23 @@ struct IntraShared {
  * S. EARLY publishes the edge record before writing the block (for workers
  * that others wait on); otherwise the edges are collected while the block
  * is written and published at the end. */
+/* A job's frame epoch. The per-stage kernels read it from the descriptor
+ * (written by the batch's first kernel, an earlier launch); inside the
+ * fused kernel another workgroup of the same launch writes it before a grid
+ * barrier, so that kernel reads it with a vector load that bypasses the
+ * scalar cache. */
+template <bool FUSED>
+__device__ __forceinline__ uint32_t job_epoch(const DJob &J)
+{
+    if constexpr (FUSED)
+        return __hip_atomic_load(&((DJob &)J).epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return J.epoch;
+}
+
 template <int W, bool EARLY>
 __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint32_t tk,
-                                            IntraShared &S, int tid)
+                                            IntraShared &S, int tid, uint32_t epoch)
 {
     /* the job's fields, loaded once, together: after any store the
      * compiler would otherwise reload them (a store could alias the
@@ -1112,10 +1131,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const DFrame out = J.out;
     const GMEM int16_t *const jres = gm(J.resid);
     uint64_t *const jedge = J.edge;
-    uint32_t *const jerr = J.err;
-    /* (a vector load: in the fused kernel the epoch is written by another
-     * workgroup of the same launch, before a grid barrier) */
-    const uint32_t epoch = __hip_atomic_load(&((DJob &)J).epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t *const jerr = J.err, *const jerr_host = J.err_host;
     /* block index, block record, dependency mask and this thread's cell
      * all come from the neighbour record, in one round of loads */
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
@@ -1193,7 +1209,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                                  inter, present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
         /* a neighbour still running: poll the cell's words until it has
          * published them */
-        if (stale) v = cell_from_edges(jedge, epoch, jerr, cell, B.spin_limit);
+        if (stale) v = cell_from_edges(jedge, epoch, jerr, jerr_host, cell, B.spin_limit);
         /* DC sums over the w above and h left samples of each plane
          * (vp9_reconstruct.c:67-90), reduced across the wave: Y | U << 16
          * and V (at most 128 * 255 and 64 * 255) */
@@ -1482,7 +1498,7 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, u
  * worker) is served through its ticket counter by the workers that have
  * finished their own entries: the tickets a worker holds are its current
  * one and a later one of the same job. */
-template <int W>
+template <int W, bool FUSED>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
                                              uint32_t nwk)
 {
@@ -1528,7 +1544,7 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
          * wrap limit) */
         uint32_t next = 0;
         if (counted && tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
-        intra_block<W, true>(B, J, tk, S, tid);
+        intra_block<W, true>(B, J, tk, S, tid, job_epoch<FUSED>(J));
 
         /* next entry; the sync also orders LDS reuse */
         if (tid == 0) S.ticket = next;
@@ -1541,7 +1557,7 @@ template <int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_intra(DBatch B)
 {
     __shared__ IntraShared shm[256 / W];
-    intra_worker<W>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W, gridDim.x * (256 / W));
+    intra_worker<W, false>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W, gridDim.x * (256 / W));
 }
 
 /* Leading dependency levels: one launch per level, one worker per intra
@@ -1562,12 +1578,12 @@ __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level, u
         if (g >= ps[B.njobs]) return; /* uniform per worker */
         const uint32_t j = find_job_wave(ps, B.njobs, g);
         intra_block<64, false>(B, B.jobs[j], start[j] + (g - ps[j]), shm[threadIdx.x / 64],
-                               threadIdx.x % 64);
+                               threadIdx.x % 64, job_epoch<false>(B.jobs[j]));
     } else {
         const uint32_t g = blockIdx.x - small_wgs; /* < pl[njobs] by the grid */
         const uint32_t j = find_job_wave(pl, B.njobs, g);
         intra_block<256, false>(B, B.jobs[j], start[j] + (ps[j + 1] - ps[j]) + (g - pl[j]), shm[0],
-                                threadIdx.x);
+                                threadIdx.x, job_epoch<false>(B.jobs[j]));
     }
 }
 
@@ -1589,6 +1605,25 @@ __device__ __forceinline__ void lf_edge(uint8_t &p1, uint8_t &p0, uint8_t &q0, u
     }
 }
 
+/* Synchronises the T threads filtering one window: a whole workgroup, or a
+ * single wave (whose LDS accesses execute in order; the fences keep the
+ * compiler from reordering them). */
+template <int T>
+__device__ __forceinline__ void lf_sync()
+{
+    if constexpr (T == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+/* Window w of the batch on T threads (64: one wave per window, as k_lf
+ * runs it, so a CU keeps four times the windows in flight that 256 threads
+ * per window allow; 256 in the fused kernel). */
+template <int T>
 __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*tile)[65])
 {
     const uint32_t j = find_job_wave(B.pre_lf, B.njobs, w);
@@ -1602,30 +1637,30 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
     const int thr = J.lf_thr;
     GMEM uint8_t *const Y = gm(J.out.y);
     const uint32_t sy = J.out.sy;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x % T;
 
     /* rows move as aligned dwords: the window starts 2 bytes into one (rows
      * are 64-byte aligned and every dword that meets [c0, c1) lies inside
-     * the row's pitch) */
+     * the row's pitch); nd <= 17 and indices below 64 * 17, inside the
+     * range where udiv_small is exact */
     const int a0 = c0 & ~3, nd = (c1 - a0 + 3) >> 2;
-    /* at most 64 rows of 17 dwords: five per thread, all loads issued
-     * before the first is used (past the window's end a thread loads the
-     * window's last dword again and discards it: unconditional loads need no
-     * wait before the join) */
-    constexpr int LF_ITEMS = 5;
-    static_assert(LF_ITEMS * 256 >= 64 * 17, "window dwords per thread");
+    /* at most 64 rows of 17 dwords, every load issued before the first is
+     * used (past the window's end a thread loads the window's last dword
+     * again and discards it: unconditional loads need no wait before the
+     * join) */
+    constexpr int LF_ITEMS = (64 * 17 + T - 1) / T;
     uint32_t lv[LF_ITEMS];
 #pragma unroll
     for (int u = 0; u < LF_ITEMS; u++) {
-        const int i = min(tid + 256 * u, nr * nd - 1);
-        const int r = i / nd, p = a0 + 4 * (i - r * nd);
+        const int i = min(tid + T * u, nr * nd - 1);
+        const int r = (int)udiv_small((uint32_t)i, (uint32_t)nd), p = a0 + 4 * (i - r * nd);
         lv[u] = *(const GMEM uint32_t *)(Y + (size_t)(r0 + r) * sy + p);
     }
 #pragma unroll
     for (int u = 0; u < LF_ITEMS; u++) {
-        const int i = tid + 256 * u;
+        const int i = tid + T * u;
         if (i < nr * nd) {
-            const int r = i / nd, p = a0 + 4 * (i - r * nd);
+            const int r = (int)udiv_small((uint32_t)i, (uint32_t)nd), p = a0 + 4 * (i - r * nd);
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const int c = p + e - c0;
@@ -1633,8 +1668,8 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
             }
         }
     }
-    __syncthreads();
-    for (int i = tid; i < 8 * 64; i += 256) { /* horizontal edges */
+    lf_sync<T>();
+    for (int i = tid; i < 8 * 64; i += T) { /* horizontal edges */
         const int e = i >> 6, c = i & 63;
         const int ey = 64 * wy + 8 * e;
         if (ey >= 8 && ey < H && c < nc) {
@@ -1642,8 +1677,8 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
             lf_edge(tile[lr - 2][c], tile[lr - 1][c], tile[lr][c], tile[lr + 1][c], thr);
         }
     }
-    __syncthreads();
-    for (int i = tid; i < 8 * 64; i += 256) { /* vertical edges */
+    lf_sync<T>();
+    for (int i = tid; i < 8 * 64; i += T) { /* vertical edges */
         const int e = i >> 6, r = i & 63;
         const int ex = 64 * wx + 8 * e;
         if (ex >= 8 && ex < W && r < nr) {
@@ -1651,12 +1686,12 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
             lf_edge(tile[r][lc - 2], tile[r][lc - 1], tile[r][lc], tile[r][lc + 1], thr);
         }
     }
-    __syncthreads();
+    lf_sync<T>();
     /* whole dwords where the window covers them; the partial dwords at the
      * window's ends (their other bytes belong to the neighbouring window)
      * byte by byte */
-    for (int i = tid; i < nr * nd; i += 256) {
-        const int r = i / nd, p = a0 + 4 * (i - r * nd);
+    for (int i = tid; i < nr * nd; i += T) {
+        const int r = (int)udiv_small((uint32_t)i, (uint32_t)nd), p = a0 + 4 * (i - r * nd);
         uint32_t v = 0, mask = 0;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -1680,12 +1715,19 @@ __device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*t
 /* Windows next to each other share the cache lines of their overlapping
  * columns (a window's rows start 4 bytes left of its first column, and its
  * first rows are the last ones of the window above), so each XCD takes one
- * contiguous run of the list and neighbours meet in its L2. */
-__global__ __launch_bounds__(256) void k_lf(DBatch B)
+ * contiguous run of the list and neighbours meet in its L2. T threads per
+ * window: 64 when the batch has windows enough to fill the device several
+ * times over (a CU then keeps 32 windows in flight instead of 8), else 256
+ * (fewer windows, each finished sooner). */
+template <int T>
+__global__ __launch_bounds__(T) void k_lf(DBatch B)
 {
     __shared__ uint8_t tile[64][65];
-    lf_item(B, xcd_block(blockIdx.x, gridDim.x), tile);
+    lf_item<T>(B, xcd_block(blockIdx.x, gridDim.x), tile);
 }
+
+/* windows from which one wave per window is the faster form */
+constexpr uint32_t LF_WAVE_WINDOWS = 4096;
 
 /* ------------------------------------------------------------------------ */
 /* I420 -> NV12 into the job's render target, pitch W (vp9_nv12_pack and    */
@@ -1744,7 +1786,7 @@ __device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
         while (__hip_atomic_load(B.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(2);
             if (++spins > B.spin_limit) {
-                flag_error(B.jobs[0].err, 2);
+                flag_error(B.jobs[0].err, B.jobs[0].err_host, 2);
                 break;
             }
         }
@@ -1810,17 +1852,17 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
         if (B.fused_lanes == 64) {
             const uint32_t wgs = (nwk + 3) / 4 < nwg ? (nwk + 3) / 4 : nwg;
             if (blockIdx.x < wgs)
-                intra_worker<64>(B, (IntraShared *)smem, blockIdx.x * 4 + wave, wgs * 4);
+                intra_worker<64, true>(B, (IntraShared *)smem, blockIdx.x * 4 + wave, wgs * 4);
         } else {
             const uint32_t wgs = nwk < nwg ? nwk : nwg;
-            if (blockIdx.x < wgs) intra_worker<256>(B, (IntraShared *)smem, blockIdx.x, wgs);
+            if (blockIdx.x < wgs) intra_worker<256, true>(B, (IntraShared *)smem, blockIdx.x, wgs);
         }
         grid_barrier(B, ++phase * nwg);
     }
     /* deblocking windows */
     const uint32_t nlf = B.pre_lf[B.njobs];
     for (uint32_t w = blockIdx.x; w < nlf; w += nwg) {
-        lf_item(B, w, (uint8_t(*)[65])smem);
+        lf_item<256>(B, w, (uint8_t(*)[65])smem);
         __syncthreads();
     }
     /* NV12 */
@@ -1908,10 +1950,9 @@ hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipS
     return hipGetLastError();
 }
 
-/* Snapshot of a stream's error word into a per-frame host word, in queue
- * order (both live in host memory, which the device reaches with
- * system-scope accesses; a host-to-host memory copy would block the
- * caller until the queue drains). */
+/* Snapshot of a stream's (device) error word into a per-frame host word,
+ * in queue order (a copy into host memory through a kernel: the host's
+ * word is read without any further synchronisation). */
 __global__ void k_copy_word(const uint32_t *src, uint32_t *dst)
 {
     __hip_atomic_store(dst, read_error(src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -2055,7 +2096,8 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         }
         break;
     case STAGE_LF:
-        if (n.lf) hipLaunchKernelGGL(k_lf, dim3(n.lf), dim3(256), 0, s, b);
+        if (n.lf >= LF_WAVE_WINDOWS) hipLaunchKernelGGL(k_lf<64>, dim3(n.lf), dim3(64), 0, s, b);
+        else if (n.lf) hipLaunchKernelGGL(k_lf<256>, dim3(n.lf), dim3(256), 0, s, b);
         break;
     case STAGE_NV12:
         if (n.nv12) hipLaunchKernelGGL(k_nv12, dim3(n.nv12), dim3(256), 0, s, b);

@@ -6,7 +6,7 @@
  * device array, `scratch`, partitioned into regions of fixed offsets:
  *
  *   [0, 4 KiB)         batch descriptor of one job (DJob + prefix rows)
- *   [4 KiB, 8 KiB)     error word (byte flags), epoch counter
+ *   [4 KiB, 8 KiB)     error words (byte flags), epoch counter
  *   owner map          4 MiB   (2^20 4x4 luma cells)
  *   edge records       24 MiB  (49152 blocks x 64 words)
  *   neighbour records  8 MiB   (26880 intra blocks x 39 words)
@@ -17,7 +17,7 @@
  * all predate the epoch counter's next value (the counter lives in the
  * array too and only grows), so a word left by an earlier call never looks
  * valid. The descriptor is written by a one-thread-block kernel from its
- * kernel arguments, which also clears the error word: no memcpy, no memset.
+ * kernel arguments, which also clears the error words: no memcpy, no memset.
  *
  * In host code `scratch` is only a stand-in for the symbol: its device
  * address comes from hipGetSymbolAddress, once per device, and only that
@@ -36,7 +36,7 @@ using namespace cvp9;
 namespace {
 
 constexpr size_t kOffDesc = 0;
-constexpr size_t kOffCtrl = 4096;            /* error word at +0, epoch counter at +64 */
+constexpr size_t kOffCtrl = 4096;            /* error words at +0 and +4, epoch counter at +64 */
 constexpr size_t kOffOwner = 8192;
 constexpr size_t kOwnerBytes = 4ull << 20;
 constexpr size_t kOffEdge = kOffOwner + kOwnerBytes;
@@ -60,12 +60,15 @@ struct DescImage {
 
 static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 
-/* Writes the batch descriptor and clears the error word (one block). */
+/* Writes the batch descriptor and clears the error words (one block). */
 __global__ __launch_bounds__(256) void k_static_setup(uint8_t *base, DescImage img, uint32_t nwords)
 {
     uint32_t *d = (uint32_t *)(base + kOffDesc);
     for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) d[i] = img.w[i];
-    if (threadIdx.x == 0) *(uint32_t *)(base + kOffCtrl) = 0u;
+    if (threadIdx.x == 0) {
+        *(uint32_t *)(base + kOffCtrl) = 0u;
+        *(uint32_t *)(base + kOffCtrl + 4) = 0u;
+    }
 }
 
 /* Zeroes [lo, hi) of the array (16-byte multiples): the control words and
@@ -188,6 +191,7 @@ cvp9_err_t cvp9_engine_run_static(const cvp9_prep_t *prep, const cvp9_frame_buf_
     j.resid = (int16_t *)(base + kOffResid);
     j.nv12 = nv12;
     j.err = (uint32_t *)(base + kOffCtrl);
+    j.err_host = (uint32_t *)(base + kOffCtrl + 4); /* unread here: the status copy reads j.err */
     j.ectr = (uint32_t *)(base + kOffCtrl + 64);
     j.n_blocks = L.n_blocks;
     j.n_inter = L.n_inter;

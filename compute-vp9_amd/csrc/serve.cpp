@@ -196,6 +196,17 @@ cvp9_err_t cvp9_server_submit(cvp9_server_t *s, int stream, cvp9_prep_t *prep, u
     return CVP9_OK;
 }
 
+cvp9_err_t cvp9_server_submit_n(cvp9_server_t *s, int n, const int *streams,
+                                cvp9_prep_t *const *preps, const uint64_t *tags)
+{
+    if (!s || n < 0 || (n && (!streams || !preps || !tags))) return CVP9_ERR_INVALID_DATA;
+    for (int i = 0; i < n; i++) {
+        const cvp9_err_t err = cvp9_server_submit(s, streams[i], preps[i], tags[i]);
+        if (err != CVP9_OK) return err;
+    }
+    return CVP9_OK;
+}
+
 int cvp9_server_pump(cvp9_server_t *s)
 {
     if (!s) return CVP9_ERR_INVALID_DATA;

@@ -49,6 +49,12 @@ void cvp9_server_close(cvp9_server_t *s, int stream);
  * until it completes). Frames of one stream complete in submission order. */
 cvp9_err_t cvp9_server_submit(cvp9_server_t *s, int stream, cvp9_prep_t *prep, uint64_t tag);
 
+/* cvp9_server_submit for n frames at once (a driver that receives frames of
+ * many streams together pays one call): frame i is preps[i] on streams[i]
+ * with tags[i], in order. Stops at the first invalid entry and returns its
+ * error; returns CVP9_OK when all n are queued. */
+cvp9_err_t cvp9_server_submit_n(cvp9_server_t *s, int n, const int *streams,
+                                cvp9_prep_t *const *preps, const uint64_t *tags);
 /* Launches one batch (the oldest queued frame of each stream) unless
  * max_inflight batches are already on the GPU. Returns frames launched, or
  * a negative cvp9_err_t. */
