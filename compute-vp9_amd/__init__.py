@@ -230,6 +230,7 @@ SIGNATURES = {
     "cvp9_hip_device_count": (C.c_int, []),
     "cvp9_hip_idct_table": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "cvp9_hip_plan_check": (C.c_int, [PF_P]),
+    "cvp9_hip_plan_image": (C.c_int64, [PF_P, C.c_int, C.POINTER(C.c_uint8), C.c_uint64]),
     "cvp9_hip_route_check": (C.c_int, [C.POINTER(PF_P), C.c_int, C.POINTER(C.c_int)]),
     "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_intra_levels": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
@@ -421,6 +422,17 @@ def run_static(prep, refs, out, nv12=None, status=None, stream=None):
             arr[i] = r.buf
     return lib().cvp9_engine_run_static(prep, C.cast(arr, C.c_void_p), C.cast(C.pointer(out.buf), C.c_void_p),
                                         nv12, status, stream)
+
+
+def plan_image(frame, threads=0):
+    """The planner's device image of a frame (cvp9_hip_plan_image)."""
+    n = lib().cvp9_hip_plan_image(frame.ptr, threads, None, 0)
+    if n < 0:
+        raise Cvp9Error(int(n), "cvp9_hip_plan_image")
+    out = np.zeros(n, np.uint8)
+    check(int(lib().cvp9_hip_plan_image(frame.ptr, threads, out.ctypes.data_as(C.POINTER(C.c_uint8)), n)
+              >= 0) - 1, "cvp9_hip_plan_image")
+    return out
 
 
 def route_check(frames):

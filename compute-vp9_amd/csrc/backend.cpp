@@ -486,6 +486,18 @@ cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf)
     return plan_measure(pf, L);
 }
 
+int64_t cvp9_hip_plan_image(const vp9_parsed_frame_t *pf, int threads, uint8_t *dst, uint64_t cap)
+{
+    PrepLayout L;
+    cvp9_err_t err = plan_measure(pf, L, threads);
+    if (err != CVP9_OK) return err;
+    if (dst && cap >= L.bytes) {
+        memset(dst, 0, L.bytes); /* the alignment gaps between the tables */
+        plan_write(pf, L, dst);
+    }
+    return (int64_t)L.bytes;
+}
+
 int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap)
 {
     PrepLayout L;

@@ -7,6 +7,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -50,6 +53,91 @@ hipError_t init_device(int device)
     return e;
 }
 
+/* ---- host thread pool ---------------------------------------------------- */
+
+struct HostPool::Job {
+    const std::function<void(int)> *fn = nullptr;
+    int n = 0;
+    std::atomic<int> next{0};
+};
+
+/* how long an idle worker watches for the next job before it sleeps: frames
+ * arrive every few hundred microseconds, and a wake-up through the
+ * condition variable costs tens of them */
+constexpr auto kPoolSpin = std::chrono::microseconds(300);
+
+HostPool &HostPool::get()
+{
+    static HostPool pool;
+    return pool;
+}
+
+HostPool::HostPool()
+{
+    const unsigned hc = std::thread::hardware_concurrency();
+    const unsigned n = hc > 1 ? std::min(7u, hc - 1) : 0u;
+    for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { worker(); });
+}
+
+HostPool::~HostPool()
+{
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread &t : th_) t.join();
+}
+
+void HostPool::worker()
+{
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        if (!stop_ && gen_.load(std::memory_order_relaxed) == seen) {
+            lk.unlock();
+            const auto t0 = std::chrono::steady_clock::now();
+            while (gen_.load(std::memory_order_acquire) == seen &&
+                   std::chrono::steady_clock::now() - t0 < kPoolSpin)
+                __builtin_ia32_pause();
+            lk.lock();
+        }
+        cv_.wait(lk, [&] { return stop_ || (job_ && gen_ != seen); });
+        if (stop_) return;
+        seen = gen_;
+        Job *j = job_;
+        users_++;
+        lk.unlock();
+        for (int i; (i = j->next.fetch_add(1)) < j->n;) (*j->fn)(i);
+        lk.lock();
+        if (--users_ == 0) idle_cv_.notify_all();
+    }
+}
+
+void HostPool::run(int n, const std::function<void(int)> &fn)
+{
+    std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
+    if (!busy.owns_lock() || th_.empty() || n <= 1) {
+        for (int i = 0; i < n; i++) fn(i);
+        return;
+    }
+    Job j;
+    j.fn = &fn;
+    j.n = n;
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        job_ = &j;
+        gen_++;
+    }
+    cv_.notify_all();
+    for (int i; (i = j.next.fetch_add(1)) < n;) fn(i);
+    /* every task has been taken; wait for the workers still running one
+     * (workers that had not joined the job by now never will) */
+    std::unique_lock<std::mutex> lk(mu_);
+    job_ = nullptr;
+    idle_cv_.wait(lk, [&] { return users_ == 0; });
+}
+
 /* ---- planning ---------------------------------------------------------- */
 
 namespace {
@@ -63,6 +151,20 @@ bool claim_bits(std::vector<uint64_t> &bits, uint64_t start, uint64_t len)
         const uint64_t mask = (n == 64 ? ~0ull : ((1ull << n) - 1)) << o;
         if (bits[w] & mask) return false;
         bits[w] |= mask;
+        start += n;
+        len -= n;
+    }
+    return true;
+}
+
+/* the same on a bitmap that several threads claim from at once */
+bool claim_bits(std::atomic<uint64_t> *bits, uint64_t start, uint64_t len)
+{
+    while (len) {
+        const uint64_t w = start >> 6, o = start & 63;
+        const uint64_t n = len < 64 - o ? len : 64 - o;
+        const uint64_t mask = (n == 64 ? ~0ull : ((1ull << n) - 1)) << o;
+        if (bits[w].fetch_or(mask, std::memory_order_relaxed) & mask) return false;
         start += n;
         len -= n;
     }
@@ -88,9 +190,76 @@ inline uint8_t mode_class(uint8_t m)
     return m <= 2 ? m : 3;
 }
 
+/* Validation of blocks [c.b0, c.b1) and their counts. The first error of
+ * the range (block index err_at); overlap with a block claimed earlier in
+ * the bitmap is `conflict` (in a parallel pass the other block may come
+ * later in the list, so the caller then repeats the validation serially). */
+struct RangeResult {
+    cvp9_err_t err = CVP9_OK;
+    uint32_t err_at = UINT32_MAX;
+    bool conflict = false;
+    uint64_t covered = 0;
+};
+
+template <typename Bits>
+void measure_range(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H, uint32_t ow, Bits &&bits,
+                   PlanChunk &out, RangeResult &res)
+{
+    /* counted in locals, stored once: the ranges' records share cache lines */
+    PlanChunk c = out;
+    RangeResult r;
+    struct Store {
+        PlanChunk &c0, &c;
+        RangeResult &r0, &r;
+        ~Store() { c0 = c; r0 = r; }
+    } store{out, c, res, r};
+    for (uint32_t i = c.b0; i < c.b1; i++) {
+        const vp9_macroblock_info_t &b = pf->blocks[i];
+        const Clamped cl = clamp_block(b, (int)W, (int)H);
+        if (cl.bw <= 0 || cl.bh <= 0) continue;
+        auto fail = [&](cvp9_err_t e) {
+            r.err = e;
+            r.err_at = i;
+        };
+        /* blocks on the 4x4 grid: neighbour ownership is exact per cell */
+        if ((b.x & 3) || (b.y & 3) || (b.width & 3) || (b.height & 3)) return fail(CVP9_ERR_UNSUPPORTED);
+        const bool intra = b.is_intra != 0;
+        /* VP9 blocks are at most 64x64; intra blocks publish their edges in
+         * fixed 64-sample records, motion compensation spreads a block row
+         * over at most 16 lanes */
+        if (cl.bw > CVP9_MAX_INTRA_DIM || cl.bh > CVP9_MAX_INTRA_DIM) return fail(CVP9_ERR_UNSUPPORTED);
+        if (!b.skip) {
+            if (b.tx_size > 3) return fail(CVP9_ERR_UNSUPPORTED);
+            const uint32_t T = 4u << b.tx_size;
+            const uint64_t ntx = (uint64_t)(cl.bw / T) * (cl.bh / T);
+            if (ntx) {
+                if (!pf->coeffs) return fail(CVP9_ERR_INVALID_DATA);
+                if ((uint64_t)b.coeff_offset + ntx * T * T > pf->num_coeffs) return fail(CVP9_ERR_INVALID_DATA);
+                c.ntx[b.tx_size] += (uint32_t)ntx;
+            }
+        }
+        const uint32_t cx0 = b.x >> 2, cx1 = (b.x + cl.bw - 1) >> 2;
+        const uint32_t cy0 = b.y >> 2, cy1 = (b.y + cl.bh - 1) >> 2;
+        for (uint32_t cy = cy0; cy <= cy1; cy++)
+            if (!claim_bits(bits, (uint64_t)cy * ow + cx0, cx1 - cx0 + 1)) {
+                r.conflict = true;
+                return fail(CVP9_ERR_UNSUPPORTED);
+            }
+        r.covered += (uint64_t)(cx1 - cx0 + 1) * (cy1 - cy0 + 1);
+        c.n_blocks++;
+        if (intra) c.n_intra++;
+        else {
+            const uint32_t rows = cvp9_mc_unit_rows((uint32_t)cl.bw);
+            const uint32_t units = ((uint32_t)cl.bh + rows - 1) / rows;
+            if (b.ref_frame[0] != 1) c.n_mc_not0 += units; /* slot ref_frame - 1 */
+            else c.n_mc_0 += units;
+        }
+    }
+}
+
 } // namespace
 
-cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
+cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads)
 {
     if (!pf) return CVP9_ERR_INVALID_DATA;
     L = PrepLayout();
@@ -116,49 +285,56 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
     }
 
     const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
-    thread_local std::vector<uint64_t> bits;
-    bits.assign(((uint64_t)ow * oh + 63) / 64 + 1, 0);
-    uint64_t covered = 0;
-
-    for (uint32_t i = 0; i < pf->num_blocks; i++) {
-        const vp9_macroblock_info_t &b = pf->blocks[i];
-        const Clamped c = clamp_block(b, (int)W, (int)H);
-        if (c.bw <= 0 || c.bh <= 0) continue;
-        /* blocks on the 4x4 grid: neighbour ownership is exact per cell */
-        if ((b.x & 3) || (b.y & 3) || (b.width & 3) || (b.height & 3)) return CVP9_ERR_UNSUPPORTED;
-        const bool intra = b.is_intra != 0;
-        /* VP9 blocks are at most 64x64; intra blocks publish their edges in
-         * fixed 64-sample records, motion compensation spreads a block row
-         * over at most 16 lanes */
-        if (c.bw > CVP9_MAX_INTRA_DIM || c.bh > CVP9_MAX_INTRA_DIM) return CVP9_ERR_UNSUPPORTED;
-        bool res = !b.skip;
-        if (res) {
-            if (b.tx_size > 3) return CVP9_ERR_UNSUPPORTED;
-            const uint32_t T = 4u << b.tx_size;
-            const uint64_t ntx = (uint64_t)(c.bw / T) * (c.bh / T);
-            if (ntx == 0) res = false;
-            else {
-                if (!pf->coeffs) return CVP9_ERR_INVALID_DATA;
-                if ((uint64_t)b.coeff_offset + ntx * T * T > pf->num_coeffs) return CVP9_ERR_INVALID_DATA;
-                L.cls_nblk[b.tx_size]++;
-                L.cls_ntx[b.tx_size] += (uint32_t)ntx;
-            }
-        }
-        const uint32_t cx0 = b.x >> 2, cx1 = (b.x + c.bw - 1) >> 2;
-        const uint32_t cy0 = b.y >> 2, cy1 = (b.y + c.bh - 1) >> 2;
-        for (uint32_t cy = cy0; cy <= cy1; cy++)
-            if (!claim_bits(bits, (uint64_t)cy * ow + cx0, cx1 - cx0 + 1)) return CVP9_ERR_UNSUPPORTED;
-        covered += (uint64_t)(cx1 - cx0 + 1) * (cy1 - cy0 + 1);
-        L.n_blocks++;
-        if (intra) L.n_intra++;
-        else {
-            L.n_inter++;
-            const uint32_t rows = cvp9_mc_unit_rows((uint32_t)c.bw);
-            const uint32_t units = ((uint32_t)c.bh + rows - 1) / rows;
-            L.n_mc += units;
-            if (b.ref_frame[0] != 1) L.n_mc_not0 += units; /* slot ref_frame - 1 */
-        }
+    const uint64_t words = ((uint64_t)ow * oh + 63) / 64 + 1;
+    const uint32_t nb = pf->num_blocks;
+    int P = nb >= kParallelBlocks ? HostPool::get().threads() : 1;
+    if (threads > 0 && threads < P) P = threads;
+    std::vector<RangeResult> res((size_t)P);
+    L.chunks.assign((size_t)P, PlanChunk());
+    for (int c = 0; c < P; c++) {
+        L.chunks[c].b0 = (uint32_t)((uint64_t)nb * c / P);
+        L.chunks[c].b1 = (uint32_t)((uint64_t)nb * (c + 1) / P);
     }
+    auto serial = [&]() {
+        thread_local std::vector<uint64_t> bits;
+        bits.assign(words, 0);
+        L.chunks.assign(1, PlanChunk());
+        L.chunks[0].b1 = nb;
+        res.assign(1, RangeResult());
+        measure_range(pf, W, H, ow, bits, L.chunks[0], res[0]);
+    };
+    if (P == 1) {
+        serial();
+    } else {
+        thread_local std::unique_ptr<std::atomic<uint64_t>[]> abits;
+        thread_local uint64_t acap = 0;
+        if (acap < words) {
+            abits.reset(new std::atomic<uint64_t>[words]);
+            acap = words;
+        }
+        for (uint64_t w = 0; w < words; w++) abits[w].store(0, std::memory_order_relaxed);
+        std::atomic<uint64_t> *const bits = abits.get();
+        HostPool::get().run(P, [&](int c) { measure_range(pf, W, H, ow, bits, L.chunks[c], res[c]); });
+        bool conflict = false;
+        for (const RangeResult &r : res) conflict |= r.conflict;
+        if (conflict) serial(); /* an overlap: the serial pass names the later block */
+    }
+    const RangeResult *first = nullptr;
+    uint64_t covered = 0;
+    for (const RangeResult &r : res) {
+        covered += r.covered;
+        if (r.err != CVP9_OK && (!first || r.err_at < first->err_at)) first = &r;
+    }
+    if (first) return first->err;
+
+    for (const PlanChunk &c : L.chunks) {
+        L.n_blocks += c.n_blocks;
+        L.n_intra += c.n_intra;
+        L.n_mc_not0 += c.n_mc_not0;
+        L.n_mc += c.n_mc_not0 + c.n_mc_0;
+        for (int k = 0; k < 4; k++) L.cls_ntx[k] += c.ntx[k];
+    }
+    L.n_inter = L.n_blocks - L.n_intra;
     L.fill = covered != (uint64_t)ow * oh;
     L.num_coeffs = pf->coeffs ? pf->num_coeffs : 0;
 
@@ -174,64 +350,90 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L)
     return CVP9_OK;
 }
 
-/* Blocks that one 64-thread intra worker handles in a single pass. */
-static inline bool intra_small(const DBlock &k)
+/* An intra block as the ordering pass sees it: its local block index and
+ * its clamped rectangle in 4x4 cells, inclusive. */
+struct IntraGeom {
+    uint32_t b;
+    uint16_t cx0, cy0, cx1, cy1;
+};
+
+/* Blocks that one 64-thread intra worker handles in a single pass
+ * (at most 16x16, i.e. at most 4x4 cells). */
+static inline bool intra_small(const IntraGeom &g)
 {
-    return k.bw <= 16 && k.bh <= 16;
+    return g.cx1 - g.cx0 < 4 && g.cy1 - g.cy0 < 4;
 }
 
+/* A per-cell scratch map whose entries carry the pass that wrote them
+ * (upper 16 bits) beside a 16-bit value, so a pass needs no clearing of what
+ * the previous one wrote: an entry of another pass reads as 0. Levels and
+ * chain heights stay below 2^16 (a dependency chain climbs one level per
+ * 4x4 cell step right or down, at most 8192 steps across a 16384x16384
+ * frame). */
+struct StampMap {
+    std::vector<uint32_t> v;
+    uint32_t stamp = 0;
+    void next_pass(size_t cells)
+    {
+        if (v.size() < cells) v.assign(cells, 0);
+        if (++stamp == 0x10000) { /* the stamps wrapped: start afresh */
+            std::fill(v.begin(), v.end(), 0u);
+            stamp = 1;
+        }
+    }
+    uint32_t get(size_t i) const { return (v[i] >> 16) == stamp ? (v[i] & 0xffff) : 0u; }
+    void set(size_t i, uint32_t x) { v[i] = stamp << 16 | x; }
+};
+
 /* Orders the intra list by dependency level, then size class (small
- * first), then list index. An intra block depends on the earlier intra
- * blocks owning its above row, left column and above-left sample (for
- * chroma too: a chroma neighbour lies in the same 4x4 luma cells). Level 0
- * has no such block; otherwise the level is one more than the highest
- * level among them. Intra workers take blocks in this order, so every block
- * a worker waits for was handed out earlier, and the blocks of one level
- * can all run at once. Those neighbour samples
- * lie on the owner's bottom row or right column, so only those cells record
- * the owner's level (plus one) in the scratch map, in list order: a cell
- * still at zero when block b reads it is not owned by an earlier intra
- * block. */
-static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint32_t W, uint32_t H,
+ * first), then the length of the chain of later blocks depending on the
+ * block (longest first), then list index; writes it to `intra` and the
+ * position where each level starts (and its small blocks) to levels /
+ * level_small. An intra block depends on the earlier intra blocks owning
+ * its above row, left column and above-left sample (for chroma too: a
+ * chroma neighbour lies in the same 4x4 luma cells). Level 0 has no such
+ * block; otherwise the level is one more than the highest level among them.
+ * Intra workers take blocks in this order, so every block a worker waits
+ * for was handed out earlier, and the blocks of one level can all run at
+ * once. Those neighbour samples lie on the owner's bottom row or right
+ * column, so only those cells record the owner's level (plus one), in list
+ * order: a cell without a mark of this pass when block b reads it is not
+ * owned by an earlier intra block. g holds the intra blocks in list
+ * order. */
+static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32_t W, uint32_t H,
                         std::vector<uint32_t> &levels, std::vector<uint32_t> &level_small)
 {
     levels.clear();
     level_small.clear();
+    const uint32_t ni = (uint32_t)g.size();
     if (ni == 0) return;
     if (ni == 1) {
+        intra[0] = g[0].b;
         levels = {0, 1};
-        level_small = {intra_small(blocks[intra[0]]) ? 1u : 0u};
+        level_small = {intra_small(g[0]) ? 1u : 0u};
         return;
     }
-    const uint32_t ow = (W + 3) / 4, oh = (H + 3) / 4;
-    thread_local std::vector<uint32_t> map; /* level + 1 per 4x4 cell, zero between frames */
-    thread_local std::vector<uint32_t> lev, cnt, tmp, hgt, pos;
-    if (map.size() < (size_t)ow * oh) map.assign((size_t)ow * oh, 0);
+    const size_t ow = (W + 3) / 4, cells = ow * ((H + 3) / 4);
+    thread_local StampMap map;
+    thread_local std::vector<uint32_t> lev, hgt;
+    thread_local std::vector<uint64_t> key;
     lev.resize(ni);
+    hgt.resize(ni);
     uint32_t depth = 0;
+    map.next_pass(cells);
     for (uint32_t i = 0; i < ni; i++) {
-        const DBlock &k = blocks[intra[i]];
-        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
-        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
+        const IntraGeom &k = g[i];
         uint32_t m = 0;
-        if (cy0 > 0) {
-            const uint32_t *row = &map[(size_t)(cy0 - 1) * ow];
-            for (uint32_t cx = cx0 ? cx0 - 1 : 0; cx <= cx1; cx++) m = std::max(m, row[cx]);
+        if (k.cy0 > 0) {
+            const size_t row = (size_t)(k.cy0 - 1) * ow;
+            for (uint32_t cx = k.cx0 ? k.cx0 - 1 : 0; cx <= k.cx1; cx++) m = std::max(m, map.get(row + cx));
         }
-        if (cx0 > 0)
-            for (uint32_t cy = cy0; cy <= cy1; cy++) m = std::max(m, map[(size_t)cy * ow + cx0 - 1]);
+        if (k.cx0 > 0)
+            for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) m = std::max(m, map.get((size_t)cy * ow + k.cx0 - 1));
         lev[i] = m; /* level of this block */
         depth = std::max(depth, m + 1);
-        for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = m + 1;
-        for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = m + 1;
-    }
-    /* clear what this frame wrote */
-    for (uint32_t i = 0; i < ni; i++) {
-        const DBlock &k = blocks[intra[i]];
-        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
-        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
-        for (uint32_t cx = cx0; cx <= cx1; cx++) map[(size_t)cy1 * ow + cx] = 0;
-        for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx1] = 0;
+        for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) map.set((size_t)k.cy1 * ow + cx, m + 1);
+        for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) map.set((size_t)cy * ow + k.cx1, m + 1);
     }
     /* height: the longest chain of later intra blocks that depend on a
      * block (1 without any), from a pass in reverse list order. A later
@@ -240,84 +442,91 @@ static void order_intra(const DBlock *blocks, uint32_t *intra, uint32_t ni, uint
      * dependents are exactly the later blocks that read its bottom row or
      * right column, and blocks earlier in the list write their marks only
      * after it has been scanned. */
-    hgt.resize(ni);
+    map.next_pass(cells);
     for (uint32_t i = ni; i-- > 0;) {
-        const DBlock &k = blocks[intra[i]];
-        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
-        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
+        const IntraGeom &k = g[i];
         uint32_t h = 0;
-        for (uint32_t cx = cx0; cx <= cx1; cx++) h = std::max(h, map[(size_t)cy1 * ow + cx]);
-        for (uint32_t cy = cy0; cy <= cy1; cy++) h = std::max(h, map[(size_t)cy * ow + cx1]);
+        for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) h = std::max(h, map.get((size_t)k.cy1 * ow + cx));
+        for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) h = std::max(h, map.get((size_t)cy * ow + k.cx1));
         hgt[i] = h + 1;
-        if (cy0 > 0) {
-            uint32_t *row = &map[(size_t)(cy0 - 1) * ow];
-            for (uint32_t cx = cx0 ? cx0 - 1 : 0; cx <= cx1; cx++) row[cx] = std::max(row[cx], h + 1);
+        if (k.cy0 > 0) {
+            const size_t row = (size_t)(k.cy0 - 1) * ow;
+            for (uint32_t cx = k.cx0 ? k.cx0 - 1 : 0; cx <= k.cx1; cx++)
+                map.set(row + cx, std::max(map.get(row + cx), h + 1));
         }
-        if (cx0 > 0)
-            for (uint32_t cy = cy0; cy <= cy1; cy++)
-                map[(size_t)cy * ow + cx0 - 1] = std::max(map[(size_t)cy * ow + cx0 - 1], h + 1);
+        if (k.cx0 > 0)
+            for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) {
+                const size_t c = (size_t)cy * ow + k.cx0 - 1;
+                map.set(c, std::max(map.get(c), h + 1));
+            }
     }
-    for (uint32_t i = 0; i < ni; i++) { /* clear the marks */
-        const DBlock &k = blocks[intra[i]];
-        const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
-        const uint32_t cx1 = (k.x + k.bw - 1) >> 2, cy1 = (k.y + k.bh - 1) >> 2;
-        if (cy0 > 0)
-            for (uint32_t cx = cx0 ? cx0 - 1 : 0; cx <= cx1; cx++) map[(size_t)(cy0 - 1) * ow + cx] = 0;
-        if (cx0 > 0)
-            for (uint32_t cy = cy0; cy <= cy1; cy++) map[(size_t)cy * ow + cx0 - 1] = 0;
-    }
-    /* stable counting sort by (level, size class): small blocks of a level
-     * first, so a level launch gives them 64-thread workers four to a
-     * workgroup and the large ones a workgroup each */
-    cnt.assign(2 * depth + 1, 0);
-    for (uint32_t i = 0; i < ni; i++)
-        cnt[2 * lev[i] + (intra_small(blocks[intra[i]]) ? 0 : 1) + 1]++;
-    level_small.resize(depth);
-    for (uint32_t l = 0; l < depth; l++) level_small[l] = cnt[2 * l + 1];
-    for (uint32_t k = 0; k < 2 * depth; k++) cnt[k + 1] += cnt[k];
-    levels.resize(depth + 1);
-    for (uint32_t l = 0; l <= depth; l++) levels[l] = cnt[2 * l];
-    tmp.resize(ni);
-    pos.resize(ni);
+    /* one sort on (level, size class, longest chain first, list index) */
+    key.resize(ni);
+    level_small.assign(depth, 0);
     for (uint32_t i = 0; i < ni; i++) {
-        const uint32_t at = cnt[2 * lev[i] + (intra_small(blocks[intra[i]]) ? 0 : 1)]++;
-        tmp[at] = intra[i];
-        pos[at] = i;
+        const uint32_t small = intra_small(g[i]) ? 1u : 0u;
+        level_small[lev[i]] += small;
+        key[i] = (uint64_t)lev[i] << 48 | (uint64_t)(1 - small) << 47 |
+                 (uint64_t)(0xffff - std::min(hgt[i], 0xffffu)) << 31 | i;
     }
-    /* within a (level, size class) bucket, the blocks heading the longest
-     * chains first (then list order): tickets go out in list order, so the
-     * dependency chain that bounds the stage starts at once instead of
-     * behind the level's other blocks */
-    for (uint32_t k = 0; k < 2 * depth; k++) {
-        const uint32_t a = k ? cnt[k - 1] : 0, e = cnt[k];
-        std::stable_sort(pos.begin() + a, pos.begin() + e,
-                         [&](uint32_t u, uint32_t v) { return hgt[u] > hgt[v]; });
+    std::sort(key.begin(), key.end());
+    levels.assign(depth + 1, 0);
+    for (uint32_t i = 0; i < ni; i++) {
+        const uint32_t at = (uint32_t)(key[i] & 0x7fffffffu);
+        intra[i] = g[at].b;
+        levels[lev[at] + 1]++;
     }
-    for (uint32_t i = 0; i < ni; i++) tmp[i] = intra[pos[i]];
-    memcpy(intra, tmp.data(), (size_t)ni * 4);
+    for (uint32_t l = 0; l < depth; l++) levels[l + 1] += levels[l];
 }
 
-void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
+/* The intra blocks of pf in list order with their local block indices
+ * (blocks clamped away are not counted), for order_intra. */
+static void intra_geometry(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H,
+                           std::vector<IntraGeom> &g)
+{
+    g.clear();
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < pf->num_blocks; i++) {
+        const vp9_macroblock_info_t &b = pf->blocks[i];
+        const Clamped cl = clamp_block(b, (int)W, (int)H);
+        if (cl.bw <= 0 || cl.bh <= 0) continue;
+        if (b.is_intra)
+            g.push_back(IntraGeom{n, (uint16_t)(b.x >> 2), (uint16_t)(b.y >> 2),
+                                  (uint16_t)((b.x + cl.bw - 1) >> 2), (uint16_t)((b.y + cl.bh - 1) >> 2)});
+        n++;
+    }
+}
+
+namespace {
+
+/* Where a range of the block list writes: its first local block index,
+ * intra-list entry, motion-compensation unit of each kind (units of blocks
+ * predicted from slot 0 come after all others) and transform of each size
+ * class. */
+struct RangeBase {
+    uint32_t n, nm_not0, nm_0, tx[4];
+};
+
+void write_range(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst,
+                 const PlanChunk &c, RangeBase at)
 {
     const int W = (int)L.W, H = (int)L.H;
     DBlock *blocks = (DBlock *)(dst + L.off_blocks);
     uint32_t *mc_tab = (uint32_t *)(dst + L.off_mc_tab); /* 4 words per unit */
-    /* units of blocks predicted from slot 0 go after all others */
-    uint32_t nm_not0 = 0, nm_0 = L.n_mc_not0;
-    uint32_t *intra = (uint32_t *)(dst + L.off_intra);
-    uint32_t *txt[4], cn[4] = {0, 0, 0, 0};
+    uint32_t *txt[4];
     for (int k = 0; k < 4; k++) txt[k] = (uint32_t *)(dst + L.off_tx_tab[k]);
-    uint32_t n = 0, ni = 0;
-    for (uint32_t i = 0; i < pf->num_blocks; i++) {
+    uint32_t n = at.n, nm_not0 = at.nm_not0, nm_0 = at.nm_0;
+    uint32_t cn[4] = {at.tx[0], at.tx[1], at.tx[2], at.tx[3]};
+    for (uint32_t i = c.b0; i < c.b1; i++) {
         const vp9_macroblock_info_t &b = pf->blocks[i];
-        const Clamped c = clamp_block(b, W, H);
-        if (c.bw <= 0 || c.bh <= 0) continue;
+        const Clamped cl = clamp_block(b, W, H);
+        if (cl.bw <= 0 || cl.bh <= 0) continue;
         DBlock d;
         memset(&d, 0, sizeof(d));
         d.x = b.x;
         d.y = b.y;
-        d.bw = (uint8_t)c.bw;
-        d.bh = (uint8_t)c.bh;
+        d.bw = (uint8_t)cl.bw;
+        d.bh = (uint8_t)cl.bh;
         const bool is_intra = b.is_intra != 0;
         d.flags = is_intra ? DB_INTRA : 0;
         d.ymode = mode_class(b.y_mode);
@@ -332,8 +541,8 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         if (res) {
             const uint32_t T = 4u << b.tx_size;
             d.tx = b.tx_size;
-            d.nx = (uint32_t)c.bw / T;
-            ntx = d.nx * ((uint32_t)c.bh / T);
+            d.nx = (uint32_t)cl.bw / T;
+            ntx = d.nx * ((uint32_t)cl.bh / T);
             if (ntx == 0) res = false;
         }
         if (!res) {
@@ -342,8 +551,7 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
             d.nx = 0;
         }
         blocks[n] = d;
-        if (is_intra) intra[ni++] = n;
-        else {
+        if (!is_intra) {
             const uint32_t rows = cvp9_mc_unit_rows(d.bw);
             const uint32_t units = (d.bh + rows - 1) / rows;
             uint32_t &nm = d.ref == 0 ? nm_0 : nm_not0;
@@ -360,17 +568,57 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         if (res) { /* transforms of inter and intra blocks alike, list order */
             const int k = d.tx;
             const uint32_t T = 4u << k;
-            for (uint32_t l = 0; l < ntx; l++) {
-                const uint32_t ty = l / d.nx, tx = l - ty * d.nx;
-                uint32_t *e = txt[k] + 2 * (size_t)cn[k]++;
-                e[0] = d.coeff + l * T * T;
-                e[1] = (d.x + tx * T) | (d.y + ty * T) << 14 | (is_intra ? 1u << 31 : 0u);
-            }
+            uint32_t *e = txt[k] + 2 * (size_t)cn[k];
+            for (uint32_t ty = 0, l = 0; ty < ntx / d.nx; ty++)
+                for (uint32_t tx = 0; tx < d.nx; tx++, l++, e += 2) {
+                    e[0] = d.coeff + l * T * T;
+                    e[1] = (d.x + tx * T) | (d.y + ty * T) << 14 | (is_intra ? 1u << 31 : 0u);
+                }
+            cn[k] += ntx;
         }
         n++;
     }
-    order_intra(blocks, intra, ni, L.W, L.H, L.levels, L.level_small);
-    if (L.num_coeffs) memcpy(dst + L.off_coeffs, pf->coeffs, (size_t)L.num_coeffs * 2);
+}
+
+} // namespace
+
+void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
+{
+    const int P = (int)L.chunks.size();
+    std::vector<RangeBase> at((size_t)P);
+    RangeBase acc = {0, 0, L.n_mc_not0, {0, 0, 0, 0}};
+    for (int c = 0; c < P; c++) {
+        at[c] = acc;
+        const PlanChunk &k = L.chunks[c];
+        acc.n += k.n_blocks;
+        acc.nm_not0 += k.n_mc_not0;
+        acc.nm_0 += k.n_mc_0;
+        for (int t = 0; t < 4; t++) acc.tx[t] += k.ntx[t];
+    }
+    /* the parser's coefficients, verbatim, in P slices */
+    const size_t cbytes = (size_t)L.num_coeffs * 2;
+    auto copy_slice = [&](int c) {
+        const size_t a = cbytes * c / P, b = cbytes * (c + 1) / P;
+        if (b > a) memcpy(dst + L.off_coeffs + a, (const uint8_t *)pf->coeffs + a, b - a);
+    };
+    /* the intra order (the longest task, a serial pass over the intra
+     * blocks) beside the tables and the coefficient slices */
+    auto order = [&]() {
+        thread_local std::vector<IntraGeom> g;
+        intra_geometry(pf, L.W, L.H, g);
+        order_intra(g, (uint32_t *)(dst + L.off_intra), L.W, L.H, L.levels, L.level_small);
+    };
+    if (P == 1) {
+        write_range(pf, L, dst, L.chunks[0], at[0]);
+        copy_slice(0);
+        order();
+    } else {
+        HostPool::get().run(2 * P + 1, [&](int t) {
+            if (t == 0) order();
+            else if (t <= P) write_range(pf, L, dst, L.chunks[t - 1], at[t - 1]);
+            else copy_slice(t - 1 - P);
+        });
+    }
 }
 
 /* ---- frames and streams ------------------------------------------------ */

@@ -10,6 +10,12 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "cvp9_hip/abi_types.h"
@@ -19,6 +25,38 @@
 
 namespace cvp9 {
 
+/* A persistent pool of host threads for the planner: run(n, fn) calls
+ * fn(0) .. fn(n - 1) on the pool's threads and the calling thread and
+ * returns when all have returned. One job at a time: a caller that finds
+ * the pool busy (another thread's job) runs its tasks itself. */
+class HostPool {
+public:
+    static HostPool &get();
+    int threads() const { return (int)th_.size() + 1; }
+    void run(int n, const std::function<void(int)> &fn);
+    ~HostPool();
+
+private:
+    struct Job;
+    HostPool();
+    void worker();
+    std::vector<std::thread> th_;
+    std::mutex mu_, run_mu_;
+    std::condition_variable cv_, idle_cv_;
+    Job *job_ = nullptr;
+    std::atomic<uint64_t> gen_{0};
+    int users_ = 0;
+    bool stop_ = false;
+};
+
+/* Counts of a contiguous range [b0, b1) of the parsed block list, from
+ * which the planner's parallel pass finds where each range writes. */
+struct PlanChunk {
+    uint32_t b0 = 0, b1 = 0;
+    uint32_t n_blocks = 0, n_intra = 0, n_mc_not0 = 0, n_mc_0 = 0;
+    uint32_t ntx[4] = {0, 0, 0, 0};
+};
+
 /* Byte layout of a prepared frame (host staging and device copy alike). */
 struct PrepLayout {
     uint32_t W = 0, H = 0;
@@ -26,7 +64,7 @@ struct PrepLayout {
     int lf_thr = 0;
     uint32_t fill = 0;
     uint32_t n_blocks = 0, n_inter = 0, n_intra = 0;
-    uint32_t cls_nblk[4] = {0, 0, 0, 0}, cls_ntx[4] = {0, 0, 0, 0};
+    uint32_t cls_ntx[4] = {0, 0, 0, 0};
     uint32_t num_coeffs = 0;
     uint32_t n_mc = 0;              /* motion-compensation units (16 lanes each) */
     uint32_t n_mc_not0 = 0;         /* units of blocks not predicted from slot 0: first in
@@ -40,13 +78,22 @@ struct PrepLayout {
      * the number of small blocks (at most 16x16, first in their level) */
     std::vector<uint32_t> levels;
     std::vector<uint32_t> level_small;
+    /* the block list in ranges, one per planner thread (one range when the
+     * frame is planned on the calling thread) */
+    std::vector<PlanChunk> chunks;
 };
 
 /* Validates pf and fills L. Returns CVP9_OK, CVP9_ERR_UNSUPPORTED (layout
  * this backend does not take: the caller falls back to the CPU path) or
- * CVP9_ERR_INVALID_DATA (coefficient references outside the pool). */
-cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L);
-/* Writes the device image of pf into dst (L.bytes bytes) and L.levels. */
+ * CVP9_ERR_INVALID_DATA (coefficient references outside the pool): the code
+ * of the first offending block in list order, as a serial pass finds it.
+ * Frames of at least kParallelBlocks blocks are validated by up to
+ * `threads` host threads (0: all of the pool's). */
+constexpr uint32_t kParallelBlocks = 2048;
+cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads = 0);
+/* Writes the device image of pf into dst (L.bytes bytes) and L.levels, on
+ * as many threads as plan_measure used. The image does not depend on the
+ * number of threads. */
 void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst);
 
 /* Device frame buffer: planes Y, U, V back to back, 64-byte row pitch. */

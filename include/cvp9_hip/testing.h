@@ -30,6 +30,13 @@ int cvp9_hip_idct_table(int n, double *out);
  * HIP path takes the frame, else the code hip_decode_frame would return. */
 cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf);
 
+/* The device image of pf as the planner lays it out (blocks, work tables,
+ * intra order, coefficients), planned on at most `threads` host threads
+ * (0: as many as the backend uses; frames below 2048 blocks are planned on
+ * the calling thread). Writes it to dst when cap is large enough and
+ * returns its size in bytes, or a negative cvp9_err_t. The image never
+ * depends on the number of threads. Needs no device. */
+int64_t cvp9_hip_plan_image(const vp9_parsed_frame_t *pf, int threads, uint8_t *dst, uint64_t cap);
 /* The order in which intra workers take pf's intra blocks: indices into
  * pf->blocks, by dependency level (no earlier intra block among the
  * above/left/above-left neighbours: level 0), then list index. Returns the
