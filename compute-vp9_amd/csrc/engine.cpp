@@ -206,12 +206,16 @@ void measure_range(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H, uint32_
                    PlanChunk &out, RangeResult &res)
 {
     /* counted in locals, stored once: the ranges' records share cache lines */
-    PlanChunk c = out;
+    PlanChunk c;
+    c.b0 = out.b0;
+    c.b1 = out.b1;
+    c.intra.swap(out.intra);
+    c.intra.clear();
     RangeResult r;
     struct Store {
         PlanChunk &c0, &c;
         RangeResult &r0, &r;
-        ~Store() { c0 = c; r0 = r; }
+        ~Store() { c0 = std::move(c); r0 = r; }
     } store{out, c, res, r};
     for (uint32_t i = c.b0; i < c.b1; i++) {
         const vp9_macroblock_info_t &b = pf->blocks[i];
@@ -246,9 +250,13 @@ void measure_range(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H, uint32_
                 return fail(CVP9_ERR_UNSUPPORTED);
             }
         r.covered += (uint64_t)(cx1 - cx0 + 1) * (cy1 - cy0 + 1);
+        if (intra) {
+            c.intra.push_back(IntraGeom{c.n_blocks, (uint16_t)cx0, (uint16_t)cy0, (uint16_t)cx1,
+                                        (uint16_t)cy1});
+            c.n_intra++;
+        }
         c.n_blocks++;
-        if (intra) c.n_intra++;
-        else {
+        if (!intra) {
             const uint32_t rows = cvp9_mc_unit_rows((uint32_t)cl.bw);
             const uint32_t units = ((uint32_t)cl.bh + rows - 1) / rows;
             if (b.ref_frame[0] != 1) c.n_mc_not0 += units; /* slot ref_frame - 1 */
@@ -350,13 +358,6 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads
     return CVP9_OK;
 }
 
-/* An intra block as the ordering pass sees it: its local block index and
- * its clamped rectangle in 4x4 cells, inclusive. */
-struct IntraGeom {
-    uint32_t b;
-    uint16_t cx0, cy0, cx1, cy1;
-};
-
 /* Blocks that one 64-thread intra worker handles in a single pass
  * (at most 16x16, i.e. at most 4x4 cells). */
 static inline bool intra_small(const IntraGeom &g)
@@ -416,7 +417,7 @@ static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32
     const size_t ow = (W + 3) / 4, cells = ow * ((H + 3) / 4);
     thread_local StampMap map;
     thread_local std::vector<uint32_t> lev, hgt;
-    thread_local std::vector<uint64_t> key;
+    thread_local std::vector<uint64_t> key, tmp;
     lev.resize(ni);
     hgt.resize(ni);
     uint32_t depth = 0;
@@ -460,41 +461,33 @@ static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32
                 map.set(c, std::max(map.get(c), h + 1));
             }
     }
-    /* one sort on (level, size class, longest chain first, list index) */
+    /* one stable sort on the key (level, size class, longest chain first):
+     * three passes of an 11-bit LSD radix sort over entries in list order,
+     * so equal keys keep list order */
     key.resize(ni);
+    tmp.resize(ni);
     level_small.assign(depth, 0);
     for (uint32_t i = 0; i < ni; i++) {
         const uint32_t small = intra_small(g[i]) ? 1u : 0u;
         level_small[lev[i]] += small;
-        key[i] = (uint64_t)lev[i] << 48 | (uint64_t)(1 - small) << 47 |
-                 (uint64_t)(0xffff - std::min(hgt[i], 0xffffu)) << 31 | i;
+        key[i] = ((uint64_t)lev[i] << 17 | (uint64_t)(1 - small) << 16 |
+                  (uint64_t)(0xffff - std::min(hgt[i], 0xffffu))) << 32 | i;
     }
-    std::sort(key.begin(), key.end());
+    for (int pass = 0; pass < 3; pass++) {
+        const int sh = 32 + 11 * pass;
+        uint32_t count[2049] = {0};
+        for (uint32_t i = 0; i < ni; i++) count[((key[i] >> sh) & 2047) + 1]++;
+        for (int d = 0; d < 2048; d++) count[d + 1] += count[d];
+        for (uint32_t i = 0; i < ni; i++) tmp[count[(key[i] >> sh) & 2047]++] = key[i];
+        key.swap(tmp);
+    }
     levels.assign(depth + 1, 0);
     for (uint32_t i = 0; i < ni; i++) {
-        const uint32_t at = (uint32_t)(key[i] & 0x7fffffffu);
+        const uint32_t at = (uint32_t)key[i];
         intra[i] = g[at].b;
         levels[lev[at] + 1]++;
     }
     for (uint32_t l = 0; l < depth; l++) levels[l + 1] += levels[l];
-}
-
-/* The intra blocks of pf in list order with their local block indices
- * (blocks clamped away are not counted), for order_intra. */
-static void intra_geometry(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H,
-                           std::vector<IntraGeom> &g)
-{
-    g.clear();
-    uint32_t n = 0;
-    for (uint32_t i = 0; i < pf->num_blocks; i++) {
-        const vp9_macroblock_info_t &b = pf->blocks[i];
-        const Clamped cl = clamp_block(b, (int)W, (int)H);
-        if (cl.bw <= 0 || cl.bh <= 0) continue;
-        if (b.is_intra)
-            g.push_back(IntraGeom{n, (uint16_t)(b.x >> 2), (uint16_t)(b.y >> 2),
-                                  (uint16_t)((b.x + cl.bw - 1) >> 2), (uint16_t)((b.y + cl.bh - 1) >> 2)});
-        n++;
-    }
 }
 
 namespace {
@@ -605,7 +598,12 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
      * blocks) beside the tables and the coefficient slices */
     auto order = [&]() {
         thread_local std::vector<IntraGeom> g;
-        intra_geometry(pf, L.W, L.H, g);
+        g.clear();
+        for (int c = 0; c < P; c++)
+            for (IntraGeom x : L.chunks[c].intra) {
+                x.b += at[c].n;
+                g.push_back(x);
+            }
         order_intra(g, (uint32_t *)(dst + L.off_intra), L.W, L.H, L.levels, L.level_small);
     };
     if (P == 1) {

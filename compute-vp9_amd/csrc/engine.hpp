@@ -49,12 +49,22 @@ private:
     bool stop_ = false;
 };
 
+/* An intra block as the ordering pass sees it: its local block index and
+ * its clamped rectangle in 4x4 cells, inclusive. */
+struct IntraGeom {
+    uint32_t b;
+    uint16_t cx0, cy0, cx1, cy1;
+};
+
 /* Counts of a contiguous range [b0, b1) of the parsed block list, from
- * which the planner's parallel pass finds where each range writes. */
+ * which the planner's parallel pass finds where each range writes, and the
+ * range's intra blocks (local indices counted from the range's first
+ * block). */
 struct PlanChunk {
     uint32_t b0 = 0, b1 = 0;
     uint32_t n_blocks = 0, n_intra = 0, n_mc_not0 = 0, n_mc_0 = 0;
     uint32_t ntx[4] = {0, 0, 0, 0};
+    std::vector<IntraGeom> intra;
 };
 
 /* Byte layout of a prepared frame (host staging and device copy alike). */
