@@ -78,6 +78,21 @@
  * count is padded to a multiple of it in the batch prefix, so a workgroup
  * works for one job. */
 #define CVP9_MC_WG_UNITS 16
+
+/* Grid barrier of the fused kernel: words from DBatch::bar, zeroed by the
+ * descriptor copy before every launch. The workgroups on one XCD form a
+ * group (the XCD's id read from the hardware, at most CVP9_BAR_GROUPS):
+ * each registers in its group's count at start; at a barrier each arrives
+ * on its group's counter, the last of a group arrives on the global
+ * counter and, once every group is in, releases its group through the
+ * group's own word. Words that are polled sit 128 bytes apart. */
+#define CVP9_BAR_GROUPS  16
+#define CVP9_BAR_GLOBAL  0      /* groups arrived, all phases */
+#define CVP9_BAR_STARTED 32     /* workgroups registered */
+#define CVP9_BAR_COUNT   64     /* CVP9_BAR_GROUPS words: workgroups per group */
+#define CVP9_BAR_ARRIVE  96     /* + 32 * group: arrivals, all phases */
+#define CVP9_BAR_RELEASE (CVP9_BAR_ARRIVE + 32 * CVP9_BAR_GROUPS) /* + 32 * group: last phase released */
+#define CVP9_BAR_BYTES   4608   /* >= 4 * (CVP9_BAR_RELEASE + 32 * CVP9_BAR_GROUPS), 256-aligned */
 #if defined(__HIPCC__)
 #define CVP9_HD __host__ __device__
 #else

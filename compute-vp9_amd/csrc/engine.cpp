@@ -867,12 +867,12 @@ size_t StreamState::resident_bytes() const
 
 size_t BatchBuilder::bytes_for(uint32_t njobs)
 {
-    return align256((uint64_t)njobs * sizeof(DJob)) + align256(13ull * (njobs + 1) * 4) + 256 +
+    return align256((uint64_t)njobs * sizeof(DJob)) + align256(13ull * (njobs + 1) * 4) + CVP9_BAR_BYTES +
            3ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
 }
 
 void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uint32_t njobs,
-                         uint8_t *host, uint8_t *dev, const Tune &tune, uint32_t spin_limit)
+                         uint8_t *host, uint8_t *dev, const Tune &tune, uint32_t spin_limit, bool fused)
 {
     memcpy(host, jobs, (size_t)njobs * sizeof(DJob));
     const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
@@ -918,7 +918,9 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     b_.jstart = (const uint32_t *)(dev + pre_off) + (P_N + 1) * stride;
     const uint64_t tk_off = pre_off + align256(13ull * stride * 4);
     b_.bar = (uint32_t *)(dev + tk_off);
-    memset(host + tk_off, 0, 256); /* the barrier counter starts at zero */
+    /* the fused kernel's barrier words start at zero */
+    const size_t bar_bytes = fused ? CVP9_BAR_BYTES : 256;
+    memset(host + tk_off, 0, bar_bytes);
     b_.spin_limit = spin_limit;
     b_.fused_lanes = tune.intra_lanes;
     b_.fused_workers = tune.intra_workers;
@@ -931,7 +933,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     n_.lf = pre[P_LF * stride + njobs];
     n_.nv12 = pre[P_NV12 * stride + njobs];
     n_.tune = tune;
-    used_ = tk_off + 256;
+    used_ = tk_off + bar_bytes;
 
     /* Leading levels at least tune.intra_level_min blocks wide (over the
      * batch) are launched one by one; the persistent workers take the rest,

@@ -218,9 +218,10 @@ public:
     /* descriptor bytes for njobs jobs */
     static size_t bytes_for(uint32_t njobs);
     /* host/dev point to bytes_for(njobs) bytes each; layouts[i] is the plan
-     * of jobs[i] */
+     * of jobs[i]; fused: the batch runs in the fused kernel (whose barrier
+     * words the descriptor then carries) */
     void build(const DJob *jobs, const PrepLayout *const *layouts, uint32_t njobs, uint8_t *host,
-               uint8_t *dev, const Tune &tune, uint32_t spin_limit);
+               uint8_t *dev, const Tune &tune, uint32_t spin_limit, bool fused = false);
     size_t used() const { return used_; }
     const DBatch &batch() const { return b_; }
     void set_resid_all(bool on) { b_.resid_all = on ? 1u : 0u; }

@@ -596,7 +596,7 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
     for (int i = 0; i < n; i++) lays[i] = &preps[i]->L;
     Tune tune = e->tune;
     if (e->fused) tune.intra_level_min = 0; /* the fused kernel runs persistent workers only */
-    bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, e->spin_limit);
+    bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, e->spin_limit, e->fused);
     bb.set_resid_all(resid_all);
     for (hipEvent_t ev : readers)
         if ((he = hipStreamWaitEvent(qa, ev, 0)) != hipSuccess) return to_err(he);

@@ -1766,29 +1766,87 @@ __global__ __launch_bounds__(256) void k_nv12(DBatch B)
 /* Fused persistent reconstruction: every stage of a batch in one launch.   */
 /* All workgroups are co-resident (the grid is sized from the occupancy     */
 /* query); each phase spreads its work items over the grid and ends with a  */
-/* grid barrier: a counter in the batch descriptor (zeroed by the           */
-/* descriptor copy that precedes every launch or graph replay), agent-scope */
-/* release before arriving and acquire after leaving. Waits are bounded: a  */
-/* timeout sets error bit 4 of the first job's stream and falls through.    */
+/* grid barrier. Waits are bounded: a timeout sets error byte 2 of the      */
+/* first job's stream and falls through.                                    */
 
-__device__ __forceinline__ void grid_barrier(const DBatch &B, uint32_t target)
+/* The grid barrier's per-workgroup state, held by thread 0. */
+struct BarState {
+    uint32_t grp;    /* this workgroup's group: the id of the XCD it runs on */
+    uint32_t n;      /* workgroups in the group (0: not read yet) */
+    uint32_t ngrp;   /* groups with any workgroup */
+};
+
+__device__ __forceinline__ uint32_t xcd_id()
 {
-    /* Every wave waits until its own stores have been acknowledged by L2
-     * before the workgroup barrier; thread 0's agent-scope release then
-     * writes back an L2 that holds all of the workgroup's stores. (The
-     * release orders only the issuing wave's own accesses.) */
+    /* HW_REG_XCC_ID (hardware register 20), bits 3:0 */
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (31 << 11)) & (CVP9_BAR_GROUPS - 1);
+}
+
+__device__ __forceinline__ bool bar_wait(const DBatch &B, const uint32_t *w, uint32_t target)
+{
+    uint32_t spins = 0;
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > B.spin_limit) {
+            flag_error(B.jobs[0].err, B.jobs[0].err_host, 2);
+            return false;
+        }
+    }
+    return true;
+}
+
+/* At kernel start: thread 0 registers its workgroup in its XCD's group. */
+__device__ __forceinline__ void bar_register(const DBatch &B, BarState &st)
+{
+    st.grp = xcd_id();
+    st.n = 0;
+    __hip_atomic_fetch_add(B.bar + CVP9_BAR_COUNT + st.grp, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    /* release: the count is in before the registration is */
+    __hip_atomic_fetch_add(B.bar + CVP9_BAR_STARTED, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Barrier `ph` (1, 2, ...). Every wave first waits until its own stores are
+ * acknowledged by the L2 of its XCD, which all workgroups of the group share.
+ * The last workgroup of a group to arrive makes the whole L2 visible device
+ * wide with one agent-scope release (an L2 write-back: it covers the stores
+ * of every workgroup of the group, as they are all in that L2) before it
+ * arrives on the global counter; so only it waits there, and the others
+ * wait on their group's release word: 1/ngroups of the pollers per word, and
+ * one L2 write-back per XCD instead of one per workgroup. Each workgroup
+ * then acquires at agent scope itself. */
+__device__ __forceinline__ void grid_barrier(const DBatch &B, BarState &st, uint32_t ph)
+{
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(B.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t spins = 0;
-        while (__hip_atomic_load(B.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > B.spin_limit) {
-                flag_error(B.jobs[0].err, B.jobs[0].err_host, 2);
-                break;
+        if (st.n == 0) {
+            /* the first barrier: every workgroup has registered once the
+             * grid has, and the group sizes are final */
+            if (bar_wait(B, B.bar + CVP9_BAR_STARTED, gridDim.x)) {
+                st.ngrp = 0;
+                for (uint32_t g = 0; g < CVP9_BAR_GROUPS; g++) {
+                    const uint32_t c = __hip_atomic_load(B.bar + CVP9_BAR_COUNT + g, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                    st.ngrp += c ? 1u : 0u;
+                    if (g == st.grp) st.n = c;
+                }
             }
+            if (st.n == 0) st.n = st.ngrp = 1; /* timed out: the error is set; do not divide the wait */
+        }
+        uint32_t *arrive = B.bar + CVP9_BAR_ARRIVE + 32 * st.grp;
+        uint32_t *release = B.bar + CVP9_BAR_RELEASE + 32 * st.grp;
+        const uint32_t old = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == st.n * ph) {
+            /* the group's last: its members' arrivals precede this point */
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(B.bar + CVP9_BAR_GLOBAL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bar_wait(B, B.bar + CVP9_BAR_GLOBAL, st.ngrp * ph);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(release, ph, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            bar_wait(B, release, ph);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
@@ -1819,12 +1877,14 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     const uint32_t nwg = gridDim.x;
     const int wave = threadIdx.x >> 6;
     uint32_t phase = 0;
+    BarState bar{0u, 0u, 0u};
+    if (threadIdx.x == 0) bar_register(B, bar);
 
     if (blockIdx.x == 0) assign_epochs(B); /* read by the intra phase, after a grid barrier */
     const uint32_t nfill = B.pre_fill[B.njobs];
     if (nfill) {
         for (uint32_t w = blockIdx.x; w < nfill; w += nwg) fill_item(B, w);
-        grid_barrier(B, ++phase * nwg);
+        grid_barrier(B, bar, ++phase);
     }
     /* owner map and motion compensation (independent) */
     const uint32_t nblk = B.pre_blocks[B.njobs], nmc = B.pre_inter[B.njobs];
@@ -1832,14 +1892,14 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
         owner_item(B, wv, threadIdx.x & 15, find_job(B.pre_blocks, B.njobs, wv));
     for (uint32_t u = blockIdx.x * 16 + threadIdx.x / CVP9_MC_UNIT; u < nmc; u += nwg * 16)
         mc_unit_at(B, u, threadIdx.x % CVP9_MC_UNIT);
-    grid_barrier(B, ++phase * nwg);
+    grid_barrier(B, bar, ++phase);
     /* transforms: inter ones add to the prediction, intra ones go aside */
     double *lds = (double *)smem;
     residual_phase<4, FS4>(B, lds);
     residual_phase<8, FS8>(B, lds);
     residual_phase<16, FS16>(B, lds);
     residual_phase<32, FS32>(B, lds);
-    grid_barrier(B, ++phase * nwg);
+    grid_barrier(B, bar, ++phase);
     /* neighbour records, then intra blocks in ticket order on the tuned
      * number and width of persistent workers (the other workgroups go on
      * to the barrier: more workers than the chains can feed only poll) */
@@ -1847,7 +1907,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     if (nint) {
         for (uint32_t wv = blockIdx.x * 8 + (threadIdx.x >> 5); wv < nint; wv += nwg * 8)
             nbr_item(B, wv, threadIdx.x & 31, find_job(B.pre_intra, B.njobs, wv));
-        grid_barrier(B, ++phase * nwg);
+        grid_barrier(B, bar, ++phase);
         const uint32_t nwk = B.fused_workers < 1 ? 1 : B.fused_workers;
         if (B.fused_lanes == 64) {
             const uint32_t wgs = (nwk + 3) / 4 < nwg ? (nwk + 3) / 4 : nwg;
@@ -1857,7 +1917,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
             const uint32_t wgs = nwk < nwg ? nwk : nwg;
             if (blockIdx.x < wgs) intra_worker<256, true>(B, (IntraShared *)smem, blockIdx.x, wgs);
         }
-        grid_barrier(B, ++phase * nwg);
+        grid_barrier(B, bar, ++phase);
     }
     /* deblocking windows */
     const uint32_t nlf = B.pre_lf[B.njobs];
@@ -1868,7 +1928,7 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
     /* NV12 */
     const uint32_t nnv = B.pre_nv12[B.njobs];
     if (nnv) {
-        grid_barrier(B, ++phase * nwg);
+        grid_barrier(B, bar, ++phase);
         for (uint32_t w = blockIdx.x; w < nnv; w += nwg) nv12_item(B, w);
     }
 }
