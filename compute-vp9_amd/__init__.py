@@ -16,7 +16,9 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libcvp9_hip.so")
+# CVP9_HIP_LIB names another build of the library (the sanitizer build of
+# the CPU test run); by default the in-tree one
+LIB_PATH = os.environ.get("CVP9_HIP_LIB") or os.path.join(PKG_DIR, "libcvp9_hip.so")
 
 # ---- error codes and enums (include/compute_vp9/decoder.h:21-37) ---------
 CVP9_OK = 0

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(HERE, "_build", "libcvp9_oracle.so")
+# CVP9_ORACLE_LIB names another build of the oracle (the sanitizer build)
+ORACLE_LIB = os.environ.get("CVP9_ORACLE_LIB") or os.path.join(HERE, "_build", "libcvp9_oracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libcvp9_ref.so")
 
 _orc = None
