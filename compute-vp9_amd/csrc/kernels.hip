@@ -130,15 +130,28 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
  * every long wait), and in a host-visible copy with a plain system-scope
  * byte store (no read-modify-write crosses the bus), which the host reads
  * without a copy. */
+/* A block record, as three 8-byte global loads (records are 24 bytes on an
+ * 8-byte grid). */
+__device__ __forceinline__ DBlock load_block(const DBlock *p)
+{
+    const GMEM uint64_t *w = (const GMEM uint64_t *)gm(p);
+    const uint64_t kw[3] = {w[0], w[1], w[2]};
+    DBlock k;
+    memcpy(&k, kw, sizeof(k));
+    return k;
+}
+
+/* (global address space: a generic pointer would make these flat
+ * instructions, which wait on every outstanding access of the wave) */
 __device__ __forceinline__ void flag_error(uint32_t *err, uint32_t *err_host, int byte)
 {
-    __hip_atomic_fetch_or(err, 1u << (8 * byte), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store((uint8_t *)err_host + byte, (uint8_t)1, __ATOMIC_RELAXED,
+    __hip_atomic_fetch_or(gm(err), 1u << (8 * byte), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gm((uint8_t *)err_host) + byte, (uint8_t)1, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ uint32_t read_error(const uint32_t *err)
 {
-    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gm(err), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* The batch's first kernel gives every job its frame epoch: the stream's
@@ -212,7 +225,7 @@ __device__ __forceinline__ void owner_item(const DBatch &B, uint32_t wv, uint32_
     const uint32_t j = wv < B.pre_blocks[jhint + 1] ? jhint : find_job(B.pre_blocks, B.njobs, wv);
     const DJob &J = B.jobs[j];
     const uint32_t b = wv - B.pre_blocks[j];
-    const DBlock k = J.blocks[b];
+    const DBlock k = load_block(J.blocks + b);
     const uint32_t cx0 = k.x >> 2, cy0 = k.y >> 2;
     const uint32_t ncx = ((k.x + k.bw - 1) >> 2) - cx0 + 1;
     const uint32_t ncy = ((k.y + k.bh - 1) >> 2) - cy0 + 1;
@@ -843,15 +856,15 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     const uint32_t j = wv < B.pre_intra[jhint + 1] ? jhint : find_job(B.pre_intra, B.njobs, wv);
     const DJob &J = B.jobs[j];
     const uint32_t tk = wv - B.pre_intra[j];
-    const uint32_t b = J.intra[tk];
-    const DBlock k = J.blocks[b];
+    const uint32_t b = gm(J.intra)[tk];
+    const DBlock k = load_block(J.blocks + b);
     const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
     const int cbx = x >> 1, cby = y >> 1;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
     GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
     /* fields held in registers across the record stores */
     const GMEM uint32_t *const owner = gm(J.owner);
-    const DBlock *const blocks = J.blocks;
+    const GMEM DBlock *const blocks = gm(J.blocks);
     const uint32_t ow = J.ow;
     uint64_t mask = 0, imask = 0;
     for (int c = (int)li; c <= ncx + ncy; c += 32) {
@@ -1187,6 +1200,17 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                                      (size_t)(((er >> lt) * (int)k.nx + (ec >> lt)) << (2 * lt)) +
                                      ((er & (T - 1)) << lt) + (ec & (T - 1))
                                : (const GMEM int16_t *)rec)[0];
+    /* a 64-thread worker has a second luma edge sample (sample tid + 64 of
+     * at most 128) */
+    const int i1 = tid + 64;
+    const int er1 = i1 < bw ? bh - 1 : i1 - bw, ec1 = i1 < bw ? i1 : bw - 1;
+    const bool eres1_ok = EARLY && W == 64 && i1 < bw + bh && er1 < rrows && ec1 < rcols;
+    const int eres1 =
+        !(EARLY && W == 64) ? 0
+               : (int)(eres1_ok ? jres + k.coeff +
+                                      (size_t)(((er1 >> lt) * (int)k.nx + (ec1 >> lt)) << (2 * lt)) +
+                                      ((er1 & (T - 1)) << lt) + (ec1 & (T - 1))
+                                : (const GMEM int16_t *)rec)[0];
 
     /* 1. neighbours: 127 outside the frame; resolved cells as recorded by
      *    the neighbour stage; cells of earlier intra blocks from their
@@ -1207,6 +1231,10 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         bool stale;
         uint64_t v = cell_gather(jedge, epoch, b, out, x, y, bh, kind, kind == 0 ? c : c - ncx, pend,
                                  inter, present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
+        /* the edge residuals are in registers before any wait: issued at
+         * the block's start, they would otherwise be loaded where they are
+         * used, a memory round trip after the last neighbour arrives */
+        asm volatile("" ::"v"(eres0), "v"(eres1));
         /* a neighbour still running: poll the cell's words until it has
          * published them */
         if (stale) v = cell_from_edges(jedge, epoch, jerr, jerr_host, cell, B.spin_limit);
@@ -1303,6 +1331,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                 S.nb[2][kind][2 * q + e] = (uint8_t)(v >> (48 + 8 * e));
             }
         }
+    } else {
+        asm volatile("" ::"v"(eres0)); /* likewise, while the first wave gathers */
     }
     worker_sync<W>();
 
@@ -1325,7 +1355,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             if (p == 0) {
                 const int r = dir ? e : bh - 1, c = dir ? bw - 1 : e;
                 v = intra_pred(k.ymode, S.nb[0][0][c], S.nb[0][1][r], S.tl[0], dc0);
-                const int rv = i == tid ? (eres_ok ? eres0 : 0) : resid_at(r, c);
+                const int rv = i == tid ? (eres_ok ? eres0 : 0)
+                               : (W == 64 && i == i1 ? (eres1_ok ? eres1 : 0) : resid_at(r, c));
                 v = clampi(v + rv, 0, 255);
             } else {
                 const int r = dir ? e : cbh - 1, c = dir ? cbw - 1 : e;
@@ -1741,18 +1772,42 @@ __device__ __forceinline__ void nv12_item(const DBatch &B, uint32_t w)
     const uint32_t W = J.out.w, H = J.out.h, cw = (W + 1) / 2, ch = (H + 1) / 2;
     const uint32_t tx = (W + 63) / 64;
     const uint32_t ty0 = (l / tx) * 64, tx0 = (l % tx) * 64;
-    uint8_t *dst = J.nv12;
+    GMEM uint8_t *const dst = gm(J.nv12);
     const DFrame F = J.out; /* held in registers across the stores */
-    for (uint32_t i = threadIdx.x; i < 64 * 64; i += 256) {
-        const uint32_t r = ty0 + (i >> 6), c = tx0 + (i & 63);
-        if (r < H && c < W) dst[(size_t)r * W + c] = F.y[(size_t)r * F.sy + c];
+    /* luma: 4 samples per thread and pass, one aligned load (rows are
+     * 64-byte aligned); one 4-byte store where the target row allows it */
+    const bool y4 = (W & 3) == 0;
+    for (uint32_t i = threadIdx.x; i < 64 * 16; i += 256) {
+        const uint32_t r = ty0 + (i >> 4), c = tx0 + 4 * (i & 15);
+        if (r >= H || c >= W) continue;
+        const uint32_t q = *(const GMEM uint32_t *)(gm(F.y) + (size_t)r * F.sy + c);
+        GMEM uint8_t *o = dst + (size_t)r * W + c;
+        if (y4 && c + 4 <= W) {
+            *(GMEM uint32_t *)o = q;
+        } else {
+            for (uint32_t e = 0; e < 4 && c + e < W; e++) o[e] = (uint8_t)(q >> (8 * e));
+        }
     }
-    uint8_t *uv = dst + (size_t)W * H;
-    for (uint32_t i = threadIdx.x; i < 32 * 32; i += 256) {
-        const uint32_t r = ty0 / 2 + (i >> 5), c = tx0 / 2 + (i & 31);
-        if (r < ch && c < cw) {
-            uv[(size_t)r * 2 * cw + 2 * c] = F.u[(size_t)r * F.suv + c];
-            uv[(size_t)r * 2 * cw + 2 * c + 1] = F.v[(size_t)r * F.suv + c];
+    /* chroma: 2 samples of each plane per thread and pass, interleaved
+     * U V U V into one 4-byte store where the target row allows it */
+    GMEM uint8_t *const uv = dst + (size_t)W * H;
+    const bool c4 = (cw & 1) == 0;
+    for (uint32_t i = threadIdx.x; i < 32 * 16; i += 256) {
+        const uint32_t r = ty0 / 2 + (i >> 4), c = tx0 / 2 + 2 * (i & 15);
+        if (r >= ch || c >= cw) continue;
+        const uint32_t u = *(const GMEM uint16_t *)(gm(F.u) + (size_t)r * F.suv + c);
+        const uint32_t v = *(const GMEM uint16_t *)(gm(F.v) + (size_t)r * F.suv + c);
+        const uint32_t pair0 = (u & 0xff) | (v & 0xff) << 8, pair1 = (u >> 8) | (v & 0xff00);
+        GMEM uint8_t *o = uv + (size_t)r * 2 * cw + 2 * c;
+        if (c + 2 <= cw) {
+            if (c4) {
+                *(GMEM uint32_t *)o = pair0 | pair1 << 16;
+            } else {
+                *(GMEM uint16_t *)o = (uint16_t)pair0;
+                *(GMEM uint16_t *)(o + 2) = (uint16_t)pair1;
+            }
+        } else {
+            *(GMEM uint16_t *)o = (uint16_t)pair0;
         }
     }
 }
