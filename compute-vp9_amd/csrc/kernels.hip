@@ -120,16 +120,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
     v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true); /* row_shr:2 */
     v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true); /* row_shr:4 */
     v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true); /* row_shr:8 */
-    return (uint32_t)(__builtin_amdgcn_readlane(v, 15) + __builtin_amdgcn_readlane(v, 31) +
-                      __builtin_amdgcn_readlane(v, 47) + __builtin_amdgcn_readlane(v, 63));
+    /* lane 15 of each row holds the row's sum: row_bcast:15 adds it into
+     * the next row, row_bcast:31 adds lane 31 into rows 2 and 3, so lane 63
+     * ends with the total */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
+    return (uint32_t)__builtin_amdgcn_readlane(v, 63);
 }
 
-/* Error reporting. Each cause sets its own byte of the stream's error
- * word: in device memory, which waits poll to give up early once another
- * wait of the stream has (a host-memory word there would put a bus read in
- * every long wait), and in a host-visible copy with a plain system-scope
- * byte store (no read-modify-write crosses the bus), which the host reads
- * without a copy. */
 /* A block record, as three 8-byte global loads (records are 24 bytes on an
  * 8-byte grid). */
 __device__ __forceinline__ DBlock load_block(const DBlock *p)
@@ -141,8 +139,14 @@ __device__ __forceinline__ DBlock load_block(const DBlock *p)
     return k;
 }
 
-/* (global address space: a generic pointer would make these flat
- * instructions, which wait on every outstanding access of the wave) */
+/* Error reporting. Each cause sets its own byte of the stream's error
+ * word: in device memory, which waits poll to give up early once another
+ * wait of the stream has (a host-memory word there would put a bus read in
+ * every long wait), and in a host-visible copy with a plain system-scope
+ * byte store (no read-modify-write crosses the bus), which the host reads
+ * without a copy. Both go through the global address space: through a
+ * generic pointer they would be flat instructions, which wait on every
+ * outstanding access of the wave. */
 __device__ __forceinline__ void flag_error(uint32_t *err, uint32_t *err_host, int byte)
 {
     __hip_atomic_fetch_or(gm(err), 1u << (8 * byte), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -986,11 +990,21 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
  * truncated product with a float reciprocal (v_rcp_f32, within 1 ulp) is
  * within one of the quotient, and one step each way corrects it (exact over
  * the whole range for reciprocal errors up to 2 ulp, checked exhaustively) */
-__device__ __forceinline__ uint32_t udiv_small(uint32_t x, uint32_t n)
+__device__ __forceinline__ uint32_t udiv_small_r(uint32_t x, uint32_t n, float rcp_n)
 {
-    const uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)n));
+    const uint32_t q = (uint32_t)((float)x * rcp_n);
     const int r = (int)x - (int)(q * n);
     return q + (r >= (int)n ? 1u : 0u) - (r < 0 ? 1u : 0u);
+}
+__device__ __forceinline__ uint32_t udiv_small(uint32_t x, uint32_t n)
+{
+    return udiv_small_r(x, n, __builtin_amdgcn_rcpf((float)n));
+}
+
+/* Lane src's value of x (a byte address src * 4 taken modulo 64 lanes) */
+__device__ __forceinline__ uint32_t lane_read(uint32_t byte_addr, uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)byte_addr, (int)x);
 }
 
 /* Every candidate is computed and one selected, with no branch: the
@@ -1235,38 +1249,39 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
          * the block's start, they would otherwise be loaded where they are
          * used, a memory round trip after the last neighbour arrives */
         asm volatile("" ::"v"(eres0), "v"(eres1));
-        /* a neighbour still running: poll the cell's words until it has
-         * published them */
-        if (stale) v = cell_from_edges(jedge, epoch, jerr, jerr_host, cell, B.spin_limit);
-        /* DC sums over the w above and h left samples of each plane
-         * (vp9_reconstruct.c:67-90), reduced across the wave: Y | U << 16
-         * and V (at most 128 * 255 and 64 * 255) */
-        uint32_t syu = 0, sv = 0;
+
+        /* Everything of the DC sums and of the edge samples that does not
+         * depend on the neighbour values is worked out here, before any
+         * wait: after the last neighbour arrives, a block on a dependency
+         * chain then runs only the arithmetic on the values (a wave issues
+         * one instruction per four clocks, and the index work had doubled
+         * the length of every chain link).
+         * DC sums over the w above and h left samples of each plane
+         * (vp9_reconstruct.c:67-90): byte masks of the samples this lane's
+         * cell contributes, luma in the low word, U and V in the high word
+         * (bytes 0-1 and 2-3). */
+        uint32_t mY = 0, mUV = 0;
         if (c <= ncx + ncy && kind != 2) {
             const int q = kind == 0 ? c : c - ncx;
             const int ly = kind == 0 ? bw : bh, lc = kind == 0 ? cbw : cbh;
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                if (4 * q + e < ly) syu += (uint32_t)(v >> (8 * e)) & 0xff;
+                if (4 * q + e < ly) mY |= 0xffu << (8 * e);
 #pragma unroll
             for (int e = 0; e < 2; e++)
-                if (2 * q + e < lc) {
-                    syu += ((uint32_t)(v >> (32 + 8 * e)) & 0xff) << 16;
-                    sv += (uint32_t)(v >> (48 + 8 * e)) & 0xff;
-                }
+                if (2 * q + e < lc) mUV |= 0x00ff00ffu << (8 * e);
         }
-        syu = wave_sum(syu);
-        sv = wave_sum(sv);
+        /* edge sample i = tid (fast blocks), numbered as in the edge loop
+         * below: Y bottom row, Y right column, then U and V likewise. Its
+         * above, left and above-left samples come from the lanes holding
+         * those cells (an above or left cell is 4 luma or 2 chroma samples
+         * wide; bytes 0-3 luma, 4-5 U, 6-7 V; the above-left cell's samples
+         * in bytes 3, 5, 7): their lanes as byte addresses, the shifts that
+         * pick the sample, the plane. Edge word w = tid: the lanes holding
+         * its four samples and the mask of those inside the edge. */
+        uint32_t f_adr = 0, f_sh = 0, w_adr = 0, w_msk = 0;
+        int f_mode = 0, f_add = 0;
         if (EARLY && fast) {
-            /* edge sample i = tid, numbered as in the edge loop below: Y
-             * bottom row, Y right column, then U and V likewise. Its above,
-             * left and above-left samples come from the lanes holding those
-             * cells (an above or left cell is 4 luma or 2 chroma samples
-             * wide; bytes 0-3 luma, 4-5 U, 6-7 V; the above-left cell's
-             * samples in bytes 3, 5, 7). */
-            const int dy = (int)(uint8_t)udiv_small((syu & 0xffff) + (uint32_t)((bw + bh) >> 1), (uint32_t)(bw + bh));
-            const int du = (int)(uint8_t)udiv_small((syu >> 16) + (uint32_t)((cbw + cbh) >> 1), (uint32_t)(cbw + cbh));
-            const int dv = (int)(uint8_t)udiv_small(sv + (uint32_t)((cbw + cbh) >> 1), (uint32_t)(cbw + cbh));
             const int i = tid;
             int p, dir, e;
             if (i < bw + bh) { p = 0; dir = i >= bw; e = dir ? i - bw : i; }
@@ -1279,36 +1294,73 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             }
             const int r = p == 0 ? (dir ? e : bh - 1) : (dir ? e : cbh - 1);
             const int cc = p == 0 ? (dir ? bw - 1 : e) : (dir ? cbw - 1 : e);
-            const int qa = min(p == 0 ? cc >> 2 : cc >> 1, 63);
-            const int ql = min(ncx + (p == 0 ? r >> 2 : r >> 1), 63);
-            const int qt = ncx + ncy;
-            const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-            const uint32_t alo = __shfl(lo, qa), ahi = __shfl(hi, qa);
-            const uint32_t llo = __shfl(lo, ql), lhi = __shfl(hi, ql);
-            const uint32_t tlo = __shfl(lo, qt), thi = __shfl(hi, qt);
+            const uint32_t qa = (uint32_t)min(p == 0 ? cc >> 2 : cc >> 1, 63);
+            const uint32_t ql = (uint32_t)min(ncx + (p == 0 ? r >> 2 : r >> 1), 63);
+            const uint32_t qt = (uint32_t)(ncx + ncy);
             const uint32_t sa = p == 0 ? 8 * (cc & 3) : 8 * (cc & 1) + (p == 2 ? 16 : 0);
             const uint32_t sl = p == 0 ? 8 * (r & 3) : 8 * (r & 1) + (p == 2 ? 16 : 0);
-            const int a = (int)(((p == 0 ? alo : ahi) >> sa) & 0xff);
-            const int l = (int)(((p == 0 ? llo : lhi) >> sl) & 0xff);
-            const int t = (int)(p == 0 ? tlo >> 24 : (p == 1 ? (thi >> 8) & 0xff : thi >> 24));
-            int es = intra_pred(p == 0 ? k.ymode : k.uvmode, a, l, t, p == 0 ? dy : (p == 1 ? du : dv));
-            if (p == 0) es = clampi(es + (eres_ok ? eres0 : 0), 0, 255);
-            /* edge word w = tid: its four samples from the lanes holding
-             * them, those past the edge's length zero */
+            const uint32_t st = p == 1 ? 8 : 24;
+            f_adr = qa << 2 | ql << 10 | qt << 18;
+            f_sh = sa | sl << 8 | st << 16 | (uint32_t)p << 24;
+            f_mode = p == 0 ? k.ymode : k.uvmode;
+            f_add = p == 0 && eres_ok ? eres0 : 0;
             const int w = tid;
             int wp, wdir, wi, len;
             if (w < 32) { wp = 0; wdir = w >= 16; wi = w & 15; len = wdir ? bh : bw; }
             else { wp = (w < 48) ? 1 : 2; wdir = (w & 15) >= 8; wi = w & 7; len = wdir ? cbh : cbw; }
             const int base = wp == 0 ? (wdir ? bw : 0) : bw + bh + (wp - 1) * (cbw + cbh) + (wdir ? cbw : 0);
+            w_adr = (uint32_t)(base + 4 * wi) << 2;
+            const int n = len - 4 * wi;
+            w_msk = n >= 4 ? ~0u : (n > 0 ? (1u << (8 * n)) - 1 : 0u);
+        }
+        const bool dc_y = k.ymode == 0, dc_uv = k.uvmode == 0;
+        const int nY = bw + bh, nC = cbw + cbh;
+        const float rY = __builtin_amdgcn_rcpf((float)nY), rC = __builtin_amdgcn_rcpf((float)nC);
+        asm volatile("" ::"v"(mY), "v"(mUV), "v"(f_adr), "v"(f_sh), "v"(w_adr), "v"(w_msk), "v"(rY),
+                     "v"(rC));
+
+        /* a neighbour still running: poll the cell's words until it has
+         * published them */
+        if (stale) v = cell_from_edges(jedge, epoch, jerr, jerr_host, cell, B.spin_limit);
+
+        /* the sums, reduced across the wave (only for a DC mode: the other
+         * predictors do not read them) */
+        const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+        const uint32_t huv = hi & mUV;
+        uint32_t syu = __builtin_amdgcn_sad_u8(lo & mY, 0u, 0u) |
+                       __builtin_amdgcn_sad_u8(huv & 0xffffu, 0u, 0u) << 16;
+        uint32_t sv = __builtin_amdgcn_sad_u8(huv >> 16, 0u, 0u);
+        if (dc_y || dc_uv) {
+            syu = wave_sum(syu);
+            sv = wave_sum(sv);
+        }
+        if (EARLY && fast) {
+            const uint32_t p = f_sh >> 24;
+            /* both words of each source lane: which one a lane needs
+             * depends on its own plane */
+            const uint32_t a_at = f_adr & 0xff, l_at = (f_adr >> 8) & 0xff, t_at = (f_adr >> 16) & 0xff;
+            const uint32_t alo = lane_read(a_at, lo), ahi = lane_read(a_at, hi);
+            const uint32_t llo = lane_read(l_at, lo), lhi = lane_read(l_at, hi);
+            const uint32_t tlo = lane_read(t_at, lo), thi = lane_read(t_at, hi);
+            const uint32_t aw = p ? ahi : alo, lw = p ? lhi : llo, tw = p ? thi : tlo;
+            const int a = (int)((aw >> (f_sh & 31)) & 0xff);
+            const int l = (int)((lw >> ((f_sh >> 8) & 31)) & 0xff);
+            const int t = (int)((tw >> ((f_sh >> 16) & 31)) & 0xff);
+            int dc = 0;
+            if (p == 0 ? dc_y : dc_uv) {
+                const uint32_t sum = p == 0 ? syu & 0xffff : (p == 1 ? syu >> 16 : sv);
+                const uint32_t n = p == 0 ? (uint32_t)nY : (uint32_t)nC;
+                dc = (int)(uint8_t)udiv_small_r(sum + (n >> 1), n, p == 0 ? rY : rC);
+            }
+            const int es = clampi(intra_pred(f_mode, a, l, t, dc) + f_add, 0, 255);
             uint32_t word = 0;
 #pragma unroll
-            for (int e2 = 0; e2 < 4; e2++) {
-                const uint32_t x2 = __shfl((uint32_t)es, min(base + 4 * wi + e2, 63));
-                if (4 * wi + e2 < len) word |= (x2 & 0xff) << (8 * e2);
-            }
-            if (4 * wi < len)
-                __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + w,
-                                   (uint64_t)epoch << 32 | word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int e2 = 0; e2 < 4; e2++)
+                word |= (lane_read(w_adr + 4 * e2, (uint32_t)es) & 0xff) << (8 * e2);
+            if (w_msk)
+                __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + tid,
+                                   (uint64_t)epoch << 32 | (word & w_msk), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         if (c == 0) {
             S.sum[0] = syu & 0xffff;
