@@ -106,6 +106,22 @@ CVP9_HD static inline uint32_t cvp9_mc_unit_rows(uint32_t bw)
     return p2 >= CVP9_MC_UNIT ? CVP9_MC_RUN : CVP9_MC_RUN * CVP9_MC_UNIT / p2;
 }
 
+/* The neighbours an intra block's prediction reads (vp9_reconstruct.c:
+ * 63-91): DC the above row and the left column, V the above row, H the left
+ * column, TM both and the above-left sample; a block reads the union over
+ * its luma and chroma modes (mode classes 0 DC, 1 V, 2 H, 3 TM). A cell the
+ * prediction does not read is no dependency: the planner orders blocks by
+ * the cells they read, and the neighbour stage makes only those wait. */
+#define CVP9_READS_ABOVE  1u
+#define CVP9_READS_LEFT   2u
+#define CVP9_READS_CORNER 4u
+CVP9_HD static inline uint32_t cvp9_mode_reads(uint32_t ymode, uint32_t uvmode)
+{
+    const uint32_t r[4] = {CVP9_READS_ABOVE | CVP9_READS_LEFT, CVP9_READS_ABOVE, CVP9_READS_LEFT,
+                           CVP9_READS_ABOVE | CVP9_READS_LEFT | CVP9_READS_CORNER};
+    return r[ymode & 3] | r[uvmode & 3];
+}
+
 /* block flags */
 #define DB_INTRA   1u
 #define DB_SKIP    2u   /* no residual (skip, or transform grid empty) */

@@ -252,7 +252,8 @@ void measure_range(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H, uint32_
         r.covered += (uint64_t)(cx1 - cx0 + 1) * (cy1 - cy0 + 1);
         if (intra) {
             c.intra.push_back(IntraGeom{c.n_blocks, (uint16_t)cx0, (uint16_t)cy0, (uint16_t)cx1,
-                                        (uint16_t)cy1});
+                                        (uint16_t)cy1,
+                                        cvp9_mode_reads(mode_class(b.y_mode), mode_class(b.uv_mode))});
             c.n_intra++;
         }
         c.n_blocks++;
@@ -427,9 +428,11 @@ static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32
         uint32_t m = 0;
         if (k.cy0 > 0) {
             const size_t row = (size_t)(k.cy0 - 1) * ow;
-            for (uint32_t cx = k.cx0 ? k.cx0 - 1 : 0; cx <= k.cx1; cx++) m = std::max(m, map.get(row + cx));
+            if (k.cx0 > 0 && (k.reads & CVP9_READS_CORNER)) m = std::max(m, map.get(row + k.cx0 - 1));
+            if (k.reads & CVP9_READS_ABOVE)
+                for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) m = std::max(m, map.get(row + cx));
         }
-        if (k.cx0 > 0)
+        if (k.cx0 > 0 && (k.reads & CVP9_READS_LEFT))
             for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) m = std::max(m, map.get((size_t)cy * ow + k.cx0 - 1));
         lev[i] = m; /* level of this block */
         depth = std::max(depth, m + 1);
@@ -450,16 +453,15 @@ static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32
         for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) h = std::max(h, map.get((size_t)k.cy1 * ow + cx));
         for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) h = std::max(h, map.get((size_t)cy * ow + k.cx1));
         hgt[i] = h + 1;
+        auto mark = [&](size_t c) { map.set(c, std::max(map.get(c), h + 1)); };
         if (k.cy0 > 0) {
             const size_t row = (size_t)(k.cy0 - 1) * ow;
-            for (uint32_t cx = k.cx0 ? k.cx0 - 1 : 0; cx <= k.cx1; cx++)
-                map.set(row + cx, std::max(map.get(row + cx), h + 1));
+            if (k.cx0 > 0 && (k.reads & CVP9_READS_CORNER)) mark(row + k.cx0 - 1);
+            if (k.reads & CVP9_READS_ABOVE)
+                for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) mark(row + cx);
         }
-        if (k.cx0 > 0)
-            for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) {
-                const size_t c = (size_t)cy * ow + k.cx0 - 1;
-                map.set(c, std::max(map.get(c), h + 1));
-            }
+        if (k.cx0 > 0 && (k.reads & CVP9_READS_LEFT))
+            for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) mark((size_t)cy * ow + k.cx0 - 1);
     }
     /* one stable sort on the key (level, size class, longest chain first):
      * three passes of an 11-bit LSD radix sort over entries in list order,

@@ -54,6 +54,7 @@ private:
 struct IntraGeom {
     uint32_t b;
     uint16_t cx0, cy0, cx1, cy1;
+    uint32_t reads; /* CVP9_READS_* of its modes */
 };
 
 /* Counts of a contiguous range [b0, b1) of the parsed block list, from

@@ -870,14 +870,16 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
     const GMEM uint32_t *const owner = gm(J.owner);
     const GMEM DBlock *const blocks = gm(J.blocks);
     const uint32_t ow = J.ow;
+    /* cells the block's modes do not read are no dependency: left as 128 */
+    const uint32_t reads = cvp9_mode_reads(k.ymode, k.uvmode);
     uint64_t mask = 0, imask = 0;
     for (int c = (int)li; c <= ncx + ncy; c += 32) {
         bool dep, inter = false;
         int kind, q = 0;
         bool present;
-        if (c < ncx) { kind = 0; q = c; present = y > 0; }
-        else if (c < ncx + ncy) { kind = 1; q = c - ncx; present = x > 0; }
-        else { kind = 2; present = x > 0 && y > 0; }
+        if (c < ncx) { kind = 0; q = c; present = y > 0 && (reads & CVP9_READS_ABOVE); }
+        else if (c < ncx + ncy) { kind = 1; q = c - ncx; present = x > 0 && (reads & CVP9_READS_LEFT); }
+        else { kind = 2; present = x > 0 && y > 0 && (reads & CVP9_READS_CORNER); }
         uint64_t val = 0;
         dep = false;
         if (present) {

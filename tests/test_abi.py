@@ -213,10 +213,22 @@ def test_generator_coefficient_density(pkg):
 
 # ---- intra work order (what keeps the persistent intra workers deadlock-free) ----
 
+def mode_reads(y_mode, uv_mode):
+    """(above, left, above-left) read by the predictions of a block's modes
+    (vp9_reconstruct.c:63-91: DC above and left, V above, H left, any other
+    value TM: all three), the union over luma and chroma."""
+    def one(m):
+        m = m if m <= 2 else 3
+        return (m in (0, 1, 3), m in (0, 2, 3), m == 3)
+    a, b = one(int(y_mode)), one(int(uv_mode))
+    return tuple(x or y for x, y in zip(a, b))
+
+
 def intra_dependencies(b, w, h):
     """For each intra block i (index into b): the earlier intra blocks owning
     a neighbour sample its prediction reads, luma and chroma, sample by
-    sample (vp9_reconstruct.c:224-247, :272-311)."""
+    sample (vp9_reconstruct.c:224-247, :272-311), for the samples its modes
+    read (above row, left column, above-left sample)."""
     owner = -np.ones(((h + 3) // 4, (w + 3) // 4), np.int64)
     for i, k in enumerate(b):
         x, y = int(k["x"]), int(k["y"])
@@ -230,14 +242,15 @@ def intra_dependencies(b, w, h):
         if not k["is_intra"] or x >= w or y >= h:
             continue
         bw, bh = min(int(k["width"]), w - x), min(int(k["height"]), h - y)
+        above, left, corner = mode_reads(k["y_mode"], k["uv_mode"])
         samples = []
         for sh, (px, py, pw, ph) in ((2, (x, y, bw, bh)),
                                      (1, (x >> 1, y >> 1, (bw + 1) >> 1, (bh + 1) >> 1))):
-            if py > 0:
+            if py > 0 and above:
                 samples += [(px + e, py - 1, sh) for e in range(pw)]
-            if px > 0:
+            if px > 0 and left:
                 samples += [(px - 1, py + r, sh) for r in range(ph)]
-            if px > 0 and py > 0:
+            if px > 0 and py > 0 and corner:
                 samples.append((px - 1, py - 1, sh))
         d = set()
         for sx, sy, sh in samples:
