@@ -20,6 +20,14 @@ using namespace cvp9;
 namespace {
 
 constexpr int kBatchSlots = 4;
+/* Flags of the engine's ordering events (batch done, phase and queue order,
+ * graph replay). They order work on the device and tell the host when a
+ * batch's descriptor slot is free; the host reads no data through them, and
+ * the kernels' own acquire and release fences make each kernel's stores
+ * visible to the kernels after it on any queue. Without the system-scope
+ * fence, a marker skips its cache write-back and invalidate, which cost
+ * 2.5 us of the gap between consecutive frames on one 1080p stream. */
+constexpr unsigned kSyncEvent = hipEventDisableTiming | hipEventDisableSystemFence;
 constexpr uint32_t kSpinLimit = 1u << 21;
 
 struct BatchSlot {
@@ -265,22 +273,22 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
         return CVP9_ERR_GPU;
     }
     if (hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&e->xev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->xev[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->xev[0], kSyncEvent) != hipSuccess ||
+        hipEventCreateWithFlags(&e->xev[1], kSyncEvent) != hipSuccess ||
         hipStreamCreateWithFlags(&e->astream[0], hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->astream[1], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&e->aev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->aev[1], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->oev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->oev[1], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->oev[2], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->gev, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->aev[0], kSyncEvent) != hipSuccess ||
+        hipEventCreateWithFlags(&e->aev[1], kSyncEvent) != hipSuccess ||
+        hipEventCreateWithFlags(&e->oev[0], kSyncEvent) != hipSuccess ||
+        hipEventCreateWithFlags(&e->oev[1], kSyncEvent) != hipSuccess ||
+        hipEventCreateWithFlags(&e->oev[2], kSyncEvent) != hipSuccess ||
+        hipEventCreateWithFlags(&e->gev, kSyncEvent) != hipSuccess) {
         cvp9_engine_destroy(e);
         return CVP9_ERR_GPU;
     }
     e->fused_wgs = cvp9_fused_max_wgs(device);
     for (BatchSlot &s : e->slots) {
-        if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+        if (hipEventCreateWithFlags(&s.done, kSyncEvent) != hipSuccess) {
             cvp9_engine_destroy(e);
             return CVP9_ERR_GPU;
         }
@@ -625,13 +633,22 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
         }
     }
     /* stages in order; late: the second phase (its motion compensation takes
-     * the late units) */
-    auto run_stages = [&](const int *order, int count, hipStream_t qx, bool late) -> hipError_t {
+     * the late units). done: an event for the phase's end; its last launch
+     * records it by its own completion when that launch is a single kernel
+     * (the marker packet of a separate record sits on the chain between
+     * frames), else it stays set for the caller to record. */
+    auto run_stages = [&](const int *order, int count, hipStream_t qx, bool late,
+                          hipEvent_t *done) -> hipError_t {
+        auto has_work = [&](int st) {
+            return late && st == STAGE_MC ? bb.counts().inter2 != 0 : stage_has_work(st, bb.counts());
+        };
+        int last = -1;
+        for (int i = 0; i < count; i++)
+            if (has_work(order[i])) last = i;
         for (int i = 0; i < count; i++) {
             const int st = order[i];
-            const bool timed =
-                timing && (late && st == STAGE_MC ? bb.counts().inter2 != 0
-                                                  : stage_has_work(st, bb.counts()));
+            const bool timed = timing && has_work(st);
+            hipEvent_t *const d = i == last && !timing ? done : nullptr;
             TimedLaunch t{st, nullptr, nullptr};
             if (timed) {
                 t.a = e->take_event();
@@ -642,11 +659,11 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
             hipError_t r;
             int span = 1;
             if (late && st == STAGE_MC) {
-                r = cvp9_launch_mc_late(bb.batch(), bb.counts(), qx);
+                r = cvp9_launch_mc_late(bb.batch(), bb.counts(), qx, d);
             } else {
                 /* outside per-stage timing, independent stages share launches */
                 span = timing ? 1 : cvp9_stage_span(st, bb.batch(), bb.counts());
-                r = cvp9_launch_stages(st, span, bb.batch(), bb.counts(), qx);
+                r = cvp9_launch_stages(st, span, bb.batch(), bb.counts(), qx, d);
             }
             if (r != hipSuccess) return r;
             i += span - 1;
@@ -657,22 +674,24 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
         }
         return hipSuccess;
     };
+    hipEvent_t done = bs ? bs->done : nullptr; /* cleared once a kernel records it */
     if (!e->fused) {
         if (!resid_all) {
-            he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, false);
+            he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, false, &done);
         } else {
-            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, false);
+            hipEvent_t aev = overlap ? e->aev[q] : nullptr;
+            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, false, &aev);
             if (he == hipSuccess && overlap) {
-                he = hipEventRecord(e->aev[q], qa);
+                if (aev) he = hipEventRecord(aev, qa);
                 if (he == hipSuccess) he = hipStreamWaitEvent(qs, e->aev[q], 0);
             }
             if (he == hipSuccess)
-                he = run_stages(kOrderB, (int)(sizeof(kOrderB) / sizeof(int)), qs, true);
+                he = run_stages(kOrderB, (int)(sizeof(kOrderB) / sizeof(int)), qs, true, &done);
         }
         if (he != hipSuccess) return to_err(he);
     }
     if (bs) {
-        if ((he = hipEventRecord(bs->done, qs)) != hipSuccess) return to_err(he);
+        if (done && (he = hipEventRecord(done, qs)) != hipSuccess) return to_err(he);
         bs->used = true;
         bs->serial = serial;
     }

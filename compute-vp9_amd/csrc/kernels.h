@@ -50,7 +50,12 @@ Tune cvp9_default_tune(int cus);
 hipError_t cvp9_set_idct_tables(const double *m4, const double *m8, const double *m16,
                                 const double *m32);
 hipError_t cvp9_launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t s);
-hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s);
+/* done (optional): when *done is set and the stage is one kernel (motion
+ * compensation, deblocking, NV12 pack), that kernel's own completion records
+ * the event, with no marker behind it on the queue, and *done is cleared;
+ * otherwise the caller records it. Give it only to a phase's last launch. */
+hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s,
+                             hipEvent_t *done = nullptr);
 
 /* Consecutive stages that can share one launch, starting at `stage`: the
  * owner map with motion compensation (2), and all four transform sizes (4)
@@ -59,10 +64,11 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
 #define CVP9_RES_SMALL_MAX_WGS 2048
 int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n);
 hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageCounts &n,
-                              hipStream_t s);
+                              hipStream_t s, hipEvent_t *done = nullptr);
 /* Motion compensation of each job's late units [n_mc_a, n_mc) (STAGE_MC
  * runs the early ones). */
-hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s);
+hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s,
+                               hipEvent_t *done = nullptr);
 
 /* Fused persistent kernel: every stage of the batch in one launch of `wgs`
  * co-resident workgroups (at most cvp9_fused_max_wgs()). */

@@ -17,6 +17,7 @@
  *   -> k_intra -> k_lf -> k_nv12
  */
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include "dev_types.h"
 #include "kernels.h"
 
@@ -2187,6 +2188,19 @@ static inline uint3 res_small_starts(const StageCounts &n, uint32_t *total)
     return make_uint3(a, b, c);
 }
 
+/* Launches kernel K; with *done set, the event is recorded at the kernel's
+ * completion by its own dispatch (hipExtLaunchKernel's stop event) and
+ * *done is cleared. */
+#define CVP9_LAUNCH(done, K, grid, block, s, ...)                                              \
+    do {                                                                                       \
+        if ((done) && *(done)) {                                                               \
+            hipExtLaunchKernelGGL(K, grid, block, 0, s, nullptr, *(done), 0, __VA_ARGS__);     \
+            *(done) = nullptr;                                                                 \
+        } else {                                                                               \
+            hipLaunchKernelGGL(K, grid, block, 0, s, __VA_ARGS__);                             \
+        }                                                                                      \
+    } while (0)
+
 int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n)
 {
     /* owner map + motion compensation (with resid_all, motion compensation
@@ -2201,7 +2215,7 @@ int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n)
 }
 
 hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageCounts &n,
-                              hipStream_t s)
+                              hipStream_t s, hipEvent_t *done)
 {
     if (span == 2 && stage == STAGE_OWNER) {
         const uint32_t ow = n.blocks ? div_up(n.blocks, 16) : 1;
@@ -2214,18 +2228,18 @@ hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageC
         if (total) hipLaunchKernelGGL(k_res_small, dim3(total), dim3(256), 0, s, b, wg);
         return hipGetLastError();
     }
-    return span == 1 ? cvp9_launch_stage(stage, b, n, s) : hipErrorInvalidValue;
+    return span == 1 ? cvp9_launch_stage(stage, b, n, s, done) : hipErrorInvalidValue;
 }
 
-hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s)
+hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s, hipEvent_t *done)
 {
-    if (n.inter2)
-        hipLaunchKernelGGL(k_mc, dim3(n.inter2 / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, 1);
+    if (n.inter2) CVP9_LAUNCH(done, k_mc, dim3(n.inter2 / CVP9_MC_WG_UNITS), dim3(256), s, b, 1);
     return hipGetLastError();
 }
 
 
-hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s)
+hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, hipStream_t s,
+                             hipEvent_t *done)
 {
     switch (stage) {
     case STAGE_FILL:
@@ -2235,7 +2249,7 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         hipLaunchKernelGGL(k_owner, dim3(n.blocks ? div_up(n.blocks, 16) : 1), dim3(256), 0, s, b);
         break;
     case STAGE_MC:
-        if (n.inter) hipLaunchKernelGGL(k_mc, dim3(n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, 0);
+        if (n.inter) CVP9_LAUNCH(done, k_mc, dim3(n.inter / CVP9_MC_WG_UNITS), dim3(256), s, b, 0);
         break;
     case STAGE_RES4:
         return launch_residual<4>(b, n, 0, s);
@@ -2265,11 +2279,11 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         }
         break;
     case STAGE_LF:
-        if (n.lf >= LF_WAVE_WINDOWS) hipLaunchKernelGGL(k_lf<64>, dim3(n.lf), dim3(64), 0, s, b);
-        else if (n.lf) hipLaunchKernelGGL(k_lf<256>, dim3(n.lf), dim3(256), 0, s, b);
+        if (n.lf >= LF_WAVE_WINDOWS) CVP9_LAUNCH(done, k_lf<64>, dim3(n.lf), dim3(64), s, b);
+        else if (n.lf) CVP9_LAUNCH(done, k_lf<256>, dim3(n.lf), dim3(256), s, b);
         break;
     case STAGE_NV12:
-        if (n.nv12) hipLaunchKernelGGL(k_nv12, dim3(n.nv12), dim3(256), 0, s, b);
+        if (n.nv12) CVP9_LAUNCH(done, k_nv12, dim3(n.nv12), dim3(256), s, b);
         break;
     default:
         return hipErrorInvalidValue;
