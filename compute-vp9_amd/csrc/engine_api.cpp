@@ -679,10 +679,12 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
         if (!resid_all) {
             he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, false, &done);
         } else {
-            hipEvent_t aev = overlap ? e->aev[q] : nullptr;
-            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, false, &aev);
+            /* (phase A's end is recorded by a marker: it is off the chain
+             * between frames, and a kernel-recorded event costs the host
+             * more per launch) */
+            he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, false, nullptr);
             if (he == hipSuccess && overlap) {
-                if (aev) he = hipEventRecord(aev, qa);
+                he = hipEventRecord(e->aev[q], qa);
                 if (he == hipSuccess) he = hipStreamWaitEvent(qs, e->aev[q], 0);
             }
             if (he == hipSuccess)
