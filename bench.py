@@ -40,7 +40,7 @@ METRIC = "recon frames/s @1080p (1 vs 64 streams), p50/p99 frame latency, HBM GB
 MIX = [(640, 360), (854, 480), (1280, 720), (1920, 1080)]
 
 
-KERNEL_OF = {"intra": "k_intra", "neighbours": "k_nbr", "mc": "k_mc", "deblock": "k_lf<256>",
+KERNEL_OF = {"intra": "k_intra", "neighbours": "k_nbr", "mc": "k_mc", "deblock": "k_lf",
              "owner": "k_owner",
              "fill": "k_fill", "nv12": "k_nv12", "residual4": "k_residual<4>",
              "residual8": "k_residual<8>", "residual16": "k_residual<16>",
@@ -122,6 +122,15 @@ def stage_bytes(P, frame):
     return out
 
 
+def tune_args(a):
+    """(res_split, intra_lanes, intra_workers, intra_level_min, intra_defer)
+    from 7 or 8 integers."""
+    if len(a) not in (7, 8):
+        raise SystemExit("a launch shape is 7 or 8 integers: res_split x4, intra_lanes, "
+                         "intra_workers, intra_level_min[, intra_defer]")
+    return a[:4], a[4], a[5], a[6], (a[7] if len(a) == 8 else 0)
+
+
 def single_stream(P, args):
     W, H = 1920, 1080
     F = args.frames
@@ -130,7 +139,10 @@ def single_stream(P, args):
     eng = P.Engine(0)
     preps = [eng.prepare(f) for f in frames]
     tune = None
-    if not args.no_tune:  # launch shape for this workload, on a tuning stream
+    if args.stune:  # a fixed launch shape (profiling and A/B runs)
+        eng.set_tune(*tune_args(args.stune))
+        tune = eng.get_tune().as_dict()
+    elif not args.no_tune:  # launch shape for this workload, on a tuning stream
         tsid = eng.open_stream()
         eng.run([tsid], [preps[0]])
         tune = eng.autotune([tsid], [preps[1]], iters=5).as_dict()
@@ -180,7 +192,7 @@ def single_stream(P, args):
     algo = sum(frames[i].algo_bytes() for i in order)
     eng.close()
     lv = [P.intra_levels(f) for f in frames]
-    dropin_fps, dropin_us = dropin(P, frames, args.steps)
+    dropin_fps, dropin_us = dropin(P, frames, args.steps) if not args.no_dropin else (None, None)
     return {"clean_s": clean_s, "timed_s": timed_s, "stats": per_stage, "algo_bytes": algo,
             "frames": frames, "tune": tune, "depth": [max(len(x) - 1, 0) for x in lv],
             "width0": [int(x[1] - x[0]) if len(x) > 1 else 0 for x in lv],
@@ -249,8 +261,7 @@ def multi_stream(P, args):
     del frames
     tune = None
     if args.mtune:  # a fixed launch shape (profiling runs)
-        a = args.mtune
-        eng.set_tune(a[:4], a[4], a[5], a[6])
+        eng.set_tune(*tune_args(args.mtune))
         tune = eng.get_tune().as_dict()
     elif not args.no_tune:  # tune on streams opened for it, then drop them
         tsids = [eng.open_stream() for _ in layout]
@@ -653,9 +664,13 @@ def main():
                     help="frames per stream of the all-core CPU baseline of the 64-stream mix")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-stream phase (for profiling the others); value is null")
-    ap.add_argument("--mtune", type=int, nargs=7, default=None,
+    ap.add_argument("--mtune", type=int, nargs="+", default=None,
                     help="fixed multi-stream launch shape: res_split x4, intra_lanes, "
-                         "intra_workers, intra_level_min (no autotuning)")
+                         "intra_workers, intra_level_min[, intra_defer] (no autotuning)")
+    ap.add_argument("--stune", type=int, nargs="+", default=None,
+                    help="fixed single-stream launch shape, as --mtune")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in boundary pass of the single-stream phase")
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
@@ -753,8 +768,10 @@ def main():
         "data": "synthetic parsed frames (seeded generator), inputs resident in HBM",
         "config": {"workload": SINGLE_WORKLOAD, "frames": args.frames,
                    "step": "one frame"},
-        "dropin_host_pcie_inclusive_fps": round(ss["dropin_fps"], 1) if ss else None,
-        "dropin_decode_call_us_p50": round(ss["dropin_decode_us"], 1) if ss else None,
+        "dropin_host_pcie_inclusive_fps":
+            round(ss["dropin_fps"], 1) if ss and ss["dropin_fps"] else None,
+        "dropin_decode_call_us_p50":
+            round(ss["dropin_decode_us"], 1) if ss and ss["dropin_decode_us"] else None,
         "timed_with_events_fps": round(args.steps / ss["timed_s"], 2) if ss else None,
         "algo_GBps": round(ss["algo_bytes"] / ss["clean_s"] / 1e9, 1) if ss else None,
         "stages": stats,

@@ -138,11 +138,13 @@ class FrameBuf(C.Structure):
 class Tune(C.Structure):
     """cvp9_tune_t (include/cvp9_hip/engine.h)."""
     _fields_ = [("res_split", C.c_uint8 * 4), ("intra_lanes", C.c_uint16),
-                ("intra_workers", C.c_uint32), ("intra_level_min", C.c_uint32)]
+                ("intra_workers", C.c_uint32), ("intra_level_min", C.c_uint32),
+                ("intra_defer", C.c_uint32)]
 
     def as_dict(self):
         return {"res_split": list(self.res_split), "intra_lanes": self.intra_lanes,
-                "intra_workers": self.intra_workers, "intra_level_min": self.intra_level_min}
+                "intra_workers": self.intra_workers, "intra_level_min": self.intra_level_min,
+                "intra_defer": self.intra_defer}
 
 
 class SynthParams(C.Structure):
@@ -599,8 +601,9 @@ class Engine:
         check(lib().cvp9_engine_get_tune(self.e, C.byref(t)), "get_tune")
         return t
 
-    def set_tune(self, res_split, intra_lanes, intra_workers, intra_level_min=0):
-        t = Tune((C.c_uint8 * 4)(*res_split), intra_lanes, intra_workers, intra_level_min)
+    def set_tune(self, res_split, intra_lanes, intra_workers, intra_level_min=0, intra_defer=0):
+        t = Tune((C.c_uint8 * 4)(*res_split), intra_lanes, intra_workers, intra_level_min,
+                 intra_defer)
         check(lib().cvp9_engine_set_tune(self.e, C.byref(t)), "set_tune")
 
     def autotune(self, streams, preps, iters=3):

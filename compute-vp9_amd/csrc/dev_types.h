@@ -181,7 +181,8 @@ struct DJob {
     uint32_t ow, oh;
     uint32_t epoch;                /* written by the batch's first kernel: *ectr + 1 */
     int32_t  lf_thr;               /* 0: no deblocking */
-    uint32_t lf_wx, lf_wy;         /* deblocking windows per row / column */
+    uint32_t lf_wx, lf_wy;         /* deblocking: waves of 64 units (8x8 at 8k-2, 8m-4) per
+                                      band of 8 rows, and bands */
     uint32_t fill;                 /* 1: frame not fully covered by blocks */
 };
 
@@ -195,7 +196,7 @@ struct DBatch {
     const uint32_t *pre_inter2;    /* motion compensation, late units */
     const uint32_t *pre_grp[4];    /* residual: groups of 64/N transforms */
     const uint32_t *pre_intra;     /* intra blocks per job */
-    const uint32_t *pre_lf;        /* deblocking windows */
+    const uint32_t *pre_lf;        /* deblocking: waves of 64 units */
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
     uint32_t       *jticket;       /* per-job intra ticket counters (zeroed by the descriptor

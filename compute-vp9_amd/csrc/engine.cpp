@@ -850,8 +850,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.oh = (L.H + 3) / 4;
     j.epoch = 0; /* set on the device by the batch's first kernel */
     j.lf_thr = L.lf_thr;
-    j.lf_wx = (L.W + 2 + 63) / 64;
-    j.lf_wy = (L.H + 2 + 63) / 64;
+    j.lf_wx = ((L.W + 3) / 8 + 1 + 63) / 64; /* waves of 64 deblocking units per band */
+    j.lf_wy = (L.H + 1) / 8 + 1;             /* bands of 8 rows at 8k - 2 */
     j.fill = L.fill;
     return j;
 }

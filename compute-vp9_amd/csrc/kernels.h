@@ -30,6 +30,8 @@ struct Tune {
                                  blocks over the batch run as one launch per level (up
                                  to CVP9_MAX_HEAD_LEVELS), the rest on persistent
                                  workers; 0: persistent workers only */
+    uint32_t intra_defer;    /* 1: the ordered intra stage publishes edges only and a
+                                parallel pass (k_intra_px) writes the blocks' samples */
 };
 
 /* Total work items of each stage over the whole batch (the last element of

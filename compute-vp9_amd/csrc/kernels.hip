@@ -1136,7 +1136,10 @@ struct IntraShared {
 /* One intra block: entry tk of job J's intra list, W threads (tid) sharing
  * S. EARLY publishes the edge record before writing the block (for workers
  * that others wait on); otherwise the edges are collected while the block
- * is written and published at the end. */
+ * is written and published at the end. DEFER (with EARLY) writes no sample
+ * of the block: after publishing the edge record it leaves the block's
+ * resolved neighbour cells and DC values in the block's neighbour record,
+ * from which k_intra_px later writes every intra block at once. */
 /* A job's frame epoch. The per-stage kernels read it from the descriptor
  * (written by the batch's first kernel, an earlier launch); inside the
  * fused kernel another workgroup of the same launch writes it before a grid
@@ -1151,10 +1154,11 @@ __device__ __forceinline__ uint32_t job_epoch(const DJob &J)
         return J.epoch;
 }
 
-template <int W, bool EARLY>
+template <int W, bool EARLY, bool DEFER = false>
 __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint32_t tk,
                                             IntraShared &S, int tid, uint32_t epoch)
 {
+    static_assert(EARLY || !DEFER, "a deferred block publishes its edges first");
     /* the job's fields, loaded once, together: after any store the
      * compiler would otherwise reload them (a store could alias the
      * descriptor as far as it knows), one memory round trip each */
@@ -1196,7 +1200,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     /* this thread's first luma quad of residual, loaded with no branch
      * around it (a thread without residual reads the neighbour record and
      * ignores it): a branch would wait for it */
-    const GMEM int16_t *rs0 = tid < nq * bh ? resid_quad(tid) : nullptr;
+    const GMEM int16_t *rs0 = !DEFER && tid < nq * bh ? resid_quad(tid) : nullptr;
     const GMEM int16_t *rp0 = rs0 ? rs0 : (const GMEM int16_t *)rec;
     int16_t rq0[4];
 #pragma unroll
@@ -1365,13 +1369,26 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                                    (uint64_t)epoch << 32 | (word & w_msk), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (c == 0) {
+        if constexpr (DEFER) {
+            /* the samples of the block are written later, by k_intra_px,
+             * from its resolved cells and DC values, kept in the block's
+             * neighbour record (which nothing else reads after this) */
+            GMEM uint64_t *recw = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
+            if (c <= ncx + ncy) recw[c] = v;
+            if (c == 0) {
+                const uint32_t dY = dc_y ? udiv_small_r((syu & 0xffff) + (uint32_t)(nY >> 1), (uint32_t)nY, rY) : 0u;
+                const uint32_t dU = dc_uv ? udiv_small_r((syu >> 16) + (uint32_t)(nC >> 1), (uint32_t)nC, rC) : 0u;
+                const uint32_t dV = dc_uv ? udiv_small_r(sv + (uint32_t)(nC >> 1), (uint32_t)nC, rC) : 0u;
+                recw[CVP9_NB_MASK] = (uint64_t)((dY & 0xff) | (dU & 0xff) << 8 | (dV & 0xff) << 16);
+            }
+        }
+        if (c == 0 && !(DEFER && fast)) {
             S.sum[0] = syu & 0xffff;
             S.sum[1] = syu >> 16;
             S.sum[2] = sv;
         }
-        if (c > ncx + ncy) {
-            /* no cell */
+        if (c > ncx + ncy || (DEFER && fast)) {
+            /* no cell, or no LDS copy needed (the edges are published) */
         } else if (kind == 2) {
             S.tl[0] = (uint8_t)(v >> 24);
             S.tl[1] = (uint8_t)(v >> 40);
@@ -1440,6 +1457,8 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
             }
         }
     }
+
+    if constexpr (DEFER) return; /* k_intra_px writes the block */
 
     /* 4. the whole block: prediction + residual, 4 luma samples per item,
      *    four items' residuals loaded before any is used (so a 64-thread
@@ -1584,7 +1603,7 @@ __device__ __forceinline__ uint32_t find_open_job(const DBatch &B, uint32_t j, u
  * worker) is served through its ticket counter by the workers that have
  * finished their own entries: the tickets a worker holds are its current
  * one and a later one of the same job. */
-template <int W, bool FUSED>
+template <int W, bool FUSED, bool DEFER = false>
 __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
                                              uint32_t nwk)
 {
@@ -1630,7 +1649,7 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
          * wrap limit) */
         uint32_t next = 0;
         if (counted && tid == 0) next = atomicInc(B.jticket + j, 0xffffffffu);
-        intra_block<W, true>(B, J, tk, S, tid, job_epoch<FUSED>(J));
+        intra_block<W, true, DEFER>(B, J, tk, S, tid, job_epoch<FUSED>(J));
 
         /* next entry; the sync also orders LDS reuse */
         if (tid == 0) S.ticket = next;
@@ -1639,11 +1658,12 @@ __device__ __forceinline__ void intra_worker(const DBatch &B, IntraShared *shm, 
     }
 }
 
-template <int W>
+template <int W, bool DEFER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_intra(DBatch B)
 {
     __shared__ IntraShared shm[256 / W];
-    intra_worker<W, false>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W, gridDim.x * (256 / W));
+    intra_worker<W, false, DEFER>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W,
+                                  gridDim.x * (256 / W));
 }
 
 /* Leading dependency levels: one launch per level, one worker per intra
@@ -1653,6 +1673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
  * workgroups take the level's small blocks (at most 16x16: one pass of a
  * 64-thread worker), four to a workgroup; the rest take one large block
  * each on all 256 threads. */
+template <bool DEFER>
 __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level, uint32_t small_wgs)
 {
     __shared__ IntraShared shm[4];
@@ -1663,157 +1684,244 @@ __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level, u
         const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
         if (g >= ps[B.njobs]) return; /* uniform per worker */
         const uint32_t j = find_job_wave(ps, B.njobs, g);
-        intra_block<64, false>(B, B.jobs[j], start[j] + (g - ps[j]), shm[threadIdx.x / 64],
-                               threadIdx.x % 64, job_epoch<false>(B.jobs[j]));
+        intra_block<64, DEFER, DEFER>(B, B.jobs[j], start[j] + (g - ps[j]), shm[threadIdx.x / 64],
+                                      threadIdx.x % 64, job_epoch<false>(B.jobs[j]));
     } else {
         const uint32_t g = blockIdx.x - small_wgs; /* < pl[njobs] by the grid */
         const uint32_t j = find_job_wave(pl, B.njobs, g);
-        intra_block<256, false>(B, B.jobs[j], start[j] + (ps[j + 1] - ps[j]) + (g - pl[j]), shm[0],
-                                threadIdx.x, job_epoch<false>(B.jobs[j]));
+        intra_block<256, DEFER, DEFER>(B, B.jobs[j], start[j] + (ps[j + 1] - ps[j]) + (g - pl[j]),
+                                       shm[0], threadIdx.x, job_epoch<false>(B.jobs[j]));
     }
 }
 
-/* ------------------------------------------------------------------------ */
-/* Deblocking (vp9_reconstruct.c:134-191): windows [64i-2, 64i+62) in both  */
-/* axes. Each window holds every sample its own edges read or write, and    */
-/* the vertical pass reads the window's own horizontal-pass output, so one  */
-/* workgroup filters a window in LDS with no cross-window traffic.          */
+/* Samples of the intra blocks in the deferred form (Tune.intra_defer). The
+ * ordered stage published each block's edge record and left its resolved
+ * neighbour cells (bytes 0-3 luma, 4-5 U, 6-7 V; the above-left cell's
+ * samples in bytes 3, 5, 7) and its DC values (Y, U, V in bytes 0-2 of word
+ * CVP9_NB_MASK) in the block's neighbour record. Here every intra block of
+ * the batch is written at once, one wave each: prediction
+ * (vp9_reconstruct.c:63-91) plus the luma residual, clamped (:336-363);
+ * chroma prediction only (:266-311). The other blocks of the frame read an
+ * intra block's samples only through its edge record, so nothing waits for
+ * this pass but deblocking. A sample's above and left neighbours come from
+ * the lanes holding their cells (ds_bpermute, with the whole wave active);
+ * the above-left samples and DC values are wave-uniform. */
+__device__ __forceinline__ void intra_px_item(const DBatch &B, uint32_t w)
+{
+    const uint32_t j = find_job_wave(B.pre_intra, B.njobs, w);
+    const DJob &J = B.jobs[j];
+    const uint32_t tk = w - B.pre_intra[j];
+    const DFrame out = J.out;
+    const GMEM int16_t *const jres = gm(J.resid);
+    const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
+    const uint32_t lane = __lane_id();
+    const uint64_t cell = rec[min(lane, (uint32_t)CVP9_NB_CELLS - 1)];
+    const uint32_t dcw = (uint32_t)rec[CVP9_NB_MASK];
+    DBlock k;
+    {
+        const uint64_t kw[3] = {rec[CVP9_NB_BLOCK], rec[CVP9_NB_BLOCK + 1], rec[CVP9_NB_BLOCK + 2]};
+        memcpy(&k, kw, sizeof(k));
+    }
+    const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
+    const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
+    const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
+    const uint32_t lo = (uint32_t)cell, hi = (uint32_t)(cell >> 32);
+    const int tc = __builtin_amdgcn_readfirstlane(ncx + ncy);
+    const uint32_t tlo = (uint32_t)__builtin_amdgcn_readlane((int)lo, tc);
+    const uint32_t thi = (uint32_t)__builtin_amdgcn_readlane((int)hi, tc);
 
-__device__ __forceinline__ void lf_edge(uint8_t &p1, uint8_t &p0, uint8_t &q0, uint8_t &q1, int thr)
+    /* luma: 4 samples per item, four items' residuals loaded before any
+     * is used; lanes past the block compute and store nothing useful */
+    const int lt = k.tx + 2, T = 1 << lt;
+    const bool res = !(k.flags & DB_SKIP);
+    const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
+    const int nq = (bw + 3) >> 2, nitems = nq * bh;
+    const int ymode = k.ymode, tlY = (int)(tlo >> 24), dcY = (int)(dcw & 0xff);
+    uint8_t *const Y = out.y + (size_t)y * out.sy + x;
+    for (int i0 = 0; i0 < nitems; i0 += 256) {
+        int16_t rq[4][4];
+        bool has[4];
+        int rr[4], cc[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = min(i0 + u * 64 + (int)lane, nitems - 1);
+            const int r = (int)udiv_small((uint32_t)i, (uint32_t)nq), c0 = (i - r * nq) * 4;
+            rr[u] = r;
+            cc[u] = c0;
+            has[u] = r < rrows && c0 < rcols;
+            /* (a quad with residual lies inside one transform row) */
+            const GMEM int16_t *rs =
+                has[u] ? jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
+                             ((r & (T - 1)) << lt) + (c0 & (T - 1))
+                       : jres;
+#pragma unroll
+            for (int e = 0; e < 4; e++) rq[u][e] = rs[has[u] ? e : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int r = rr[u], c0 = cc[u];
+            const uint32_t aw = lane_read(4u * (uint32_t)(c0 >> 2), lo);
+            const uint32_t lw = lane_read(4u * (uint32_t)(ncx + (r >> 2)), lo);
+            const int l = (int)((lw >> (8 * (r & 3))) & 0xff);
+            uint32_t word = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int v = intra_pred(ymode, (int)((aw >> (8 * e)) & 0xff), l, tlY, dcY);
+                if (has[u]) v = clampi(v + rq[u][e], 0, 255);
+                word |= (uint32_t)v << (8 * e);
+            }
+            if (i0 + u * 64 + (int)lane < nitems) store_quad(Y + (size_t)r * out.sy + c0, word, bw - c0);
+        }
+    }
+
+    /* chroma: U items then V items, 4 samples each */
+    const int nqc = (cbw + 3) >> 2, nci = nqc * cbh;
+    const int uvmode = k.uvmode;
+    for (int i0 = 0; i0 < 2 * nci; i0 += 64) {
+        const int i = min(i0 + (int)lane, 2 * nci - 1);
+        const bool pu = i < nci;
+        const int q = pu ? i : i - nci;
+        const int r = (int)udiv_small((uint32_t)q, (uint32_t)nqc), c0 = (q - r * nqc) * 4;
+        const uint32_t sh = pu ? 0u : 16u;
+        const uint32_t a0 = lane_read(4u * (uint32_t)(c0 >> 1), hi);
+        const uint32_t a1 = lane_read(4u * (uint32_t)min((c0 >> 1) + 1, 63), hi);
+        const uint32_t lw = lane_read(4u * (uint32_t)(ncx + (r >> 1)), hi);
+        const int l = (int)((lw >> (sh + 8 * (r & 1))) & 0xff);
+        const int tl = pu ? (int)((thi >> 8) & 0xff) : (int)(thi >> 24);
+        const int dc = pu ? (int)((dcw >> 8) & 0xff) : (int)((dcw >> 16) & 0xff);
+        const uint32_t av = ((a0 >> sh) & 0xffff) | ((a1 >> sh) & 0xffff) << 16;
+        uint32_t word = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            word |= (uint32_t)intra_pred(uvmode, (int)((av >> (8 * e)) & 0xff), l, tl, dc) << (8 * e);
+        if (i0 + (int)lane < 2 * nci) {
+            uint8_t *pl = pu ? out.u : out.v;
+            store_quad(pl + (size_t)(cby + r) * out.suv + cbx + c0, word, cbw - c0);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_intra_px(DBatch B)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= B.pre_intra[B.njobs]) return; /* uniform per wave */
+    intra_px_item(B, w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deblocking (vp9_reconstruct.c:134-191), luma only, in place.             */
+/*                                                                          */
+/* The horizontal pass filters rows y-2..y+1 around each edge y = 8, 16, ..  */
+/* (< H), column by column; the vertical pass then filters columns x-2..x+1 */
+/* around each edge x = 8, 16, .. (< W), row by row, on the horizontal      */
+/* pass's output. Split the plane into units of 8 rows x 8 columns at       */
+/* (8k - 2, 8m - 4): the rows the horizontal pass touches are the first     */
+/* four of a unit (8k-2 .. 8k+1), the columns the vertical pass touches its */
+/* middle four (8m-2 .. 8m+1). So every sample a unit's filters read or     */
+/* write is the unit's own: one lane filters its unit in registers, both    */
+/* passes in the reference's order, and no two lanes share a byte. Nothing  */
+/* goes through LDS and nothing waits; each byte of the plane is read and   */
+/* written once.                                                            */
+
+__device__ __forceinline__ void lf_edge(int &p1, int &p0, int &q0, int &q1, int thr)
 {
     const int a = p1, b = p0, c = q0, d = q1;
     if (abs(b - c) * 2 + abs(a - d) / 2 <= thr) {
         const int delta = clampi((3 * (c - b) + 4) >> 3, -thr, thr);
-        p0 = (uint8_t)clampi(b + delta, 0, 255);
-        q0 = (uint8_t)clampi(c - delta, 0, 255);
-        p1 = (uint8_t)clampi(a + delta / 2, 0, 255);
-        q1 = (uint8_t)clampi(d - delta / 2, 0, 255);
+        p0 = clampi(b + delta, 0, 255);
+        q0 = clampi(c - delta, 0, 255);
+        p1 = clampi(a + delta / 2, 0, 255);
+        q1 = clampi(d - delta / 2, 0, 255);
     }
 }
 
-/* Synchronises the T threads filtering one window: a whole workgroup, or a
- * single wave (whose LDS accesses execute in order; the fences keep the
- * compiler from reordering them). */
-template <int T>
-__device__ __forceinline__ void lf_sync()
+__device__ __forceinline__ int byte_of(uint32_t w, int e)
 {
-    if constexpr (T == 64) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-        __syncthreads();
-    }
+    return (int)((w >> (8 * e)) & 0xff);
+}
+__device__ __forceinline__ uint32_t with_byte(uint32_t w, int e, int v)
+{
+    return (w & ~(0xffu << (8 * e))) | (uint32_t)v << (8 * e);
 }
 
-/* Window w of the batch on T threads (64: one wave per window, as k_lf
- * runs it, so a CU keeps four times the windows in flight that 256 threads
- * per window allow; 256 in the fused kernel). */
-template <int T>
-__device__ __forceinline__ void lf_item(const DBatch &B, uint32_t w, uint8_t (*tile)[65])
+/* Units per job: lf_wy bands of 8 rows, each lf_wx waves of 64 units (see
+ * the job descriptor); wave w of the batch takes 64 units side by side in
+ * one band, lane = unit, a unit row being two dwords (columns 8m-4 .. 8m-1
+ * and 8m .. 8m+3). */
+__device__ __forceinline__ void lf_units(const DBatch &B, uint32_t w)
 {
     const uint32_t j = find_job_wave(B.pre_lf, B.njobs, w);
     const DJob &J = B.jobs[j];
     const uint32_t l = w - B.pre_lf[j];
-    const int wy = (int)(l / J.lf_wx), wx = (int)(l - (uint32_t)wy * J.lf_wx);
-    const int W = (int)J.out.w, H = (int)J.out.h;
-    const int r0 = max(64 * wy - 2, 0), r1 = min(64 * wy + 62, H);
-    const int c0 = max(64 * wx - 2, 0), c1 = min(64 * wx + 62, W);
-    const int nr = r1 - r0, nc = c1 - c0;
-    const int thr = J.lf_thr;
-    GMEM uint8_t *const Y = gm(J.out.y);
-    const uint32_t sy = J.out.sy;
-    const int tid = threadIdx.x % T;
-
-    /* rows move as aligned dwords: the window starts 2 bytes into one (rows
-     * are 64-byte aligned and every dword that meets [c0, c1) lies inside
-     * the row's pitch); nd <= 17 and indices below 64 * 17, inside the
-     * range where udiv_small is exact */
-    const int a0 = c0 & ~3, nd = (c1 - a0 + 3) >> 2;
-    /* at most 64 rows of 17 dwords, every load issued before the first is
-     * used (past the window's end a thread loads the window's last dword
-     * again and discards it: unconditional loads need no wait before the
-     * join) */
-    constexpr int LF_ITEMS = (64 * 17 + T - 1) / T;
-    uint32_t lv[LF_ITEMS];
+    const int k = (int)(l / J.lf_wx);
+    const int m = (int)(l - (uint32_t)k * J.lf_wx) * 64 + (int)__lane_id();
+    const int W = (int)J.out.w, H = (int)J.out.h, thr = J.lf_thr;
+    const int nu = (W + 3) / 8 + 1, dmax = (W - 1) >> 2;
+    GMEM uint32_t *const Y = (GMEM uint32_t *)gm(J.out.y);
+    const size_t sw = J.out.sy >> 2;
+    /* dwords 2m-1 and 2m of each row; the first holds a sample of the plane
+     * for every unit m >= 1 up to nu - 1 (8m - 4 <= W - 1), the second when
+     * 8m < W; loads at clamped places, stores only where they belong */
+    const int d0 = max(2 * m - 1, 0), d1 = min(2 * m, dmax);
+    const bool ok0 = m >= 1 && m < nu, ok1 = m < nu && 2 * m <= dmax;
+    const int y0 = 8 * k - 2;
+    uint32_t lo[8], hi[8];
 #pragma unroll
-    for (int u = 0; u < LF_ITEMS; u++) {
-        const int i = min(tid + T * u, nr * nd - 1);
-        const int r = (int)udiv_small((uint32_t)i, (uint32_t)nd), p = a0 + 4 * (i - r * nd);
-        lv[u] = *(const GMEM uint32_t *)(Y + (size_t)(r0 + r) * sy + p);
+    for (int r = 0; r < 8; r++) {
+        const size_t row = (size_t)clampi(y0 + r, 0, H - 1) * sw;
+        lo[r] = Y[row + d0];
+        hi[r] = Y[row + d1];
     }
-#pragma unroll
-    for (int u = 0; u < LF_ITEMS; u++) {
-        const int i = tid + T * u;
-        if (i < nr * nd) {
-            const int r = (int)udiv_small((uint32_t)i, (uint32_t)nd), p = a0 + 4 * (i - r * nd);
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int c = p + e - c0;
-                if (c >= 0 && c < nc) tile[r][c] = (uint8_t)(lv[u] >> (8 * e));
-            }
-        }
-    }
-    lf_sync<T>();
-    for (int i = tid; i < 8 * 64; i += T) { /* horizontal edges */
-        const int e = i >> 6, c = i & 63;
-        const int ey = 64 * wy + 8 * e;
-        if (ey >= 8 && ey < H && c < nc) {
-            const int lr = ey - r0;
-            lf_edge(tile[lr - 2][c], tile[lr - 1][c], tile[lr][c], tile[lr + 1][c], thr);
-        }
-    }
-    lf_sync<T>();
-    for (int i = tid; i < 8 * 64; i += T) { /* vertical edges */
-        const int e = i >> 6, r = i & 63;
-        const int ex = 64 * wx + 8 * e;
-        if (ex >= 8 && ex < W && r < nr) {
-            const int lc = ex - c0;
-            lf_edge(tile[r][lc - 2], tile[r][lc - 1], tile[r][lc], tile[r][lc + 1], thr);
-        }
-    }
-    lf_sync<T>();
-    /* whole dwords where the window covers them; the partial dwords at the
-     * window's ends (their other bytes belong to the neighbouring window)
-     * byte by byte */
-    for (int i = tid; i < nr * nd; i += T) {
-        const int r = (int)udiv_small((uint32_t)i, (uint32_t)nd), p = a0 + 4 * (i - r * nd);
-        uint32_t v = 0, mask = 0;
+    /* horizontal edge y = 8k (rows 8k-2 .. 8k+1 lie inside the plane for an
+     * accepted frame: H is not 1 mod 8), all 8 columns */
+    if (k >= 1 && 8 * k < H) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const int c = p + e - c0;
-            if (c >= 0 && c < nc) {
-                v |= (uint32_t)tile[r][c] << (8 * e);
-                mask |= 1u << e;
-            }
+            int a = byte_of(lo[0], e), b = byte_of(lo[1], e), c = byte_of(lo[2], e), d = byte_of(lo[3], e);
+            lf_edge(a, b, c, d, thr);
+            lo[0] = with_byte(lo[0], e, a);
+            lo[1] = with_byte(lo[1], e, b);
+            lo[2] = with_byte(lo[2], e, c);
+            lo[3] = with_byte(lo[3], e, d);
+            a = byte_of(hi[0], e), b = byte_of(hi[1], e), c = byte_of(hi[2], e), d = byte_of(hi[3], e);
+            lf_edge(a, b, c, d, thr);
+            hi[0] = with_byte(hi[0], e, a);
+            hi[1] = with_byte(hi[1], e, b);
+            hi[2] = with_byte(hi[2], e, c);
+            hi[3] = with_byte(hi[3], e, d);
         }
-        GMEM uint8_t *d = Y + (size_t)(r0 + r) * sy + p;
-        if (mask == 15) {
-            *(GMEM uint32_t *)d = v;
-        } else {
+    }
+    /* vertical edge x = 8m, all 8 rows (rows outside the plane are never
+     * stored) */
+    if (m >= 1 && 8 * m < W) {
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                if ((mask >> e) & 1) d[e] = (uint8_t)(v >> (8 * e));
+        for (int r = 0; r < 8; r++) {
+            int a = byte_of(lo[r], 2), b = byte_of(lo[r], 3), c = byte_of(hi[r], 0), d = byte_of(hi[r], 1);
+            lf_edge(a, b, c, d, thr);
+            lo[r] = (lo[r] & 0xffffu) | (uint32_t)a << 16 | (uint32_t)b << 24;
+            hi[r] = (hi[r] & 0xffff0000u) | (uint32_t)c | (uint32_t)d << 8;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int y = y0 + r;
+        if (y >= 0 && y < H) {
+            const size_t row = (size_t)y * sw;
+            if (ok0) Y[row + d0] = lo[r];
+            if (ok1) Y[row + d1] = hi[r];
         }
     }
 }
 
-/* Windows next to each other share the cache lines of their overlapping
- * columns (a window's rows start 4 bytes left of its first column, and its
- * first rows are the last ones of the window above), so each XCD takes one
- * contiguous run of the list and neighbours meet in its L2. T threads per
- * window: 64 when the batch has windows enough to fill the device several
- * times over (a CU then keeps 32 windows in flight instead of 8), else 256
- * (fewer windows, each finished sooner). */
-template <int T>
-__global__ __launch_bounds__(T) void k_lf(DBatch B)
+/* Four waves per workgroup, each its own item; workgroups mapped so that
+ * each XCD takes one contiguous run of the bands. */
+__global__ __launch_bounds__(256) void k_lf(DBatch B)
 {
-    __shared__ uint8_t tile[64][65];
-    lf_item<T>(B, xcd_block(blockIdx.x, gridDim.x), tile);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 +
+                                                      (threadIdx.x >> 6));
+    if (w >= B.pre_lf[B.njobs]) return; /* uniform per wave */
+    lf_units(B, w);
 }
-
-/* windows from which one wave per window is the faster form */
-constexpr uint32_t LF_WAVE_WINDOWS = 4096;
 
 /* ------------------------------------------------------------------------ */
 /* I420 -> NV12 into the job's render target, pitch W (vp9_nv12_pack and    */
@@ -2029,12 +2137,10 @@ __global__ __launch_bounds__(256) void k_fused(DBatch B)
         }
         grid_barrier(B, bar, ++phase);
     }
-    /* deblocking windows */
+    /* deblocking: one item of 64 units per wave */
     const uint32_t nlf = B.pre_lf[B.njobs];
-    for (uint32_t w = blockIdx.x; w < nlf; w += nwg) {
-        lf_item<256>(B, w, (uint8_t(*)[65])smem);
-        __syncthreads();
-    }
+    for (uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); w < nlf; w += nwg * 4)
+        lf_units(B, w);
     /* NV12 */
     const uint32_t nnv = B.pre_nv12[B.njobs];
     if (nnv) {
@@ -2057,7 +2163,7 @@ hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s)
  * they end and give their slots up; the only other launch that may wait is
  * the intra stage of a batch on the engine's other queue, which is held to
  * the other half. */
-template <int W>
+template <int W, bool DEFER>
 static uint32_t intra_resident_wgs()
 {
     static uint32_t cache[16] = {};
@@ -2065,7 +2171,7 @@ static uint32_t intra_resident_wgs()
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
     if (!cache[dev]) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra<W>, 256, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra<W, DEFER>, 256, 0) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             per_cu < 1 || cus < 1)
             return 128;
@@ -2159,7 +2265,31 @@ Tune cvp9_default_tune(int cus)
     t.intra_lanes = 256;
     t.intra_workers = 4u * (uint32_t)(cus > 0 ? cus : 256);
     t.intra_level_min = 0;
+    t.intra_defer = 0;
     return t;
+}
+
+/* The intra stage: leading levels one launch each, the persistent workers
+ * for the rest, and with DEFER the pass that writes the blocks' samples. */
+template <bool DEFER>
+static hipError_t launch_intra(const DBatch &b, const StageCounts &n, hipStream_t s)
+{
+    for (uint32_t l = 0; l < n.levels; l++) {
+        const uint32_t sw = div_up(n.level_ns[l], 4);
+        hipLaunchKernelGGL(k_intra_level<DEFER>, dim3(sw + n.level_n[l]), dim3(256), 0, s, b, l, sw);
+    }
+    if (n.intra_tail) {
+        const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;
+        if (n.tune.intra_lanes == 64) {
+            const uint32_t g = div_up(w, 4), cap = intra_resident_wgs<64, DEFER>();
+            hipLaunchKernelGGL((k_intra<64, DEFER>), dim3(g < cap ? g : cap), dim3(256), 0, s, b);
+        } else {
+            const uint32_t cap = intra_resident_wgs<256, DEFER>();
+            hipLaunchKernelGGL((k_intra<256, DEFER>), dim3(w < cap ? w : cap), dim3(256), 0, s, b);
+        }
+    }
+    if (DEFER && n.intra) hipLaunchKernelGGL(k_intra_px, dim3(div_up(n.intra, 4)), dim3(256), 0, s, b);
+    return hipGetLastError();
 }
 
 template <int N>
@@ -2263,24 +2393,9 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         if (n.intra) hipLaunchKernelGGL(k_nbr, dim3(div_up(n.intra, 8)), dim3(256), 0, s, b);
         break;
     case STAGE_INTRA:
-        for (uint32_t l = 0; l < n.levels; l++) {
-            const uint32_t sw = div_up(n.level_ns[l], 4);
-            hipLaunchKernelGGL(k_intra_level, dim3(sw + n.level_n[l]), dim3(256), 0, s, b, l, sw);
-        }
-        if (n.intra_tail) {
-            const uint32_t w = n.intra_tail < n.tune.intra_workers ? n.intra_tail : n.tune.intra_workers;
-            if (n.tune.intra_lanes == 64) {
-                const uint32_t g = div_up(w, 4), cap = intra_resident_wgs<64>();
-                hipLaunchKernelGGL(k_intra<64>, dim3(g < cap ? g : cap), dim3(256), 0, s, b);
-            } else {
-                const uint32_t cap = intra_resident_wgs<256>();
-                hipLaunchKernelGGL(k_intra<256>, dim3(w < cap ? w : cap), dim3(256), 0, s, b);
-            }
-        }
-        break;
+        return n.tune.intra_defer ? launch_intra<true>(b, n, s) : launch_intra<false>(b, n, s);
     case STAGE_LF:
-        if (n.lf >= LF_WAVE_WINDOWS) CVP9_LAUNCH(done, k_lf<64>, dim3(n.lf), dim3(64), s, b);
-        else if (n.lf) CVP9_LAUNCH(done, k_lf<256>, dim3(n.lf), dim3(256), s, b);
+        if (n.lf) CVP9_LAUNCH(done, k_lf, dim3(div_up(n.lf, 4)), dim3(256), s, b);
         break;
     case STAGE_NV12:
         if (n.nv12) CVP9_LAUNCH(done, k_nv12, dim3(n.nv12), dim3(256), s, b);

@@ -200,8 +200,8 @@ cvp9_err_t cvp9_engine_run_static(const cvp9_prep_t *prep, const cvp9_frame_buf_
     j.ow = (L.W + 3) / 4;
     j.oh = (L.H + 3) / 4;
     j.lf_thr = L.lf_thr;
-    j.lf_wx = (L.W + 2 + 63) / 64;
-    j.lf_wy = (L.H + 2 + 63) / 64;
+    j.lf_wx = ((L.W + 3) / 8 + 1 + 63) / 64; /* waves of 64 deblocking units per band */
+    j.lf_wy = (L.H + 1) / 8 + 1;             /* bands of 8 rows at 8k - 2 */
     j.fill = L.fill;
 
     /* the descriptor image, built on the stack: one job, persistent intra

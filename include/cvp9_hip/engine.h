@@ -89,16 +89,20 @@ cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int stream_id, uint8_t *dst, uint6
 
 /* Launch shape of the stage kernels: waves sharing one transform group for
  * N = 4, 8, 16, 32 (1, 2 or 4), threads per intra block (64 or 256), the
- * number of persistent intra workers, and the width (intra blocks over the
+ * number of persistent intra workers, the width (intra blocks over the
  * batch) from which a leading dependency level gets a launch of its own
  * instead of going to the persistent workers (0: persistent workers only;
- * at most 64 leading levels). Output does not depend on it;
- * cvp9_engine_autotune picks it for a workload class. */
+ * at most 64 leading levels), and whether the ordered intra stage only
+ * publishes each block's edge samples and leaves the blocks' samples to a
+ * parallel pass after it (intra_defer 1) or writes every block itself (0).
+ * Output does not depend on it; cvp9_engine_autotune picks it for a
+ * workload class. */
 typedef struct {
     uint8_t  res_split[4];
     uint16_t intra_lanes;
     uint32_t intra_workers;
     uint32_t intra_level_min;
+    uint32_t intra_defer;
 } cvp9_tune_t;
 cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t);
 cvp9_err_t cvp9_engine_set_tune(cvp9_engine_t *e, const cvp9_tune_t *t);

@@ -21,10 +21,16 @@ def shuffled_with_holes(pkg, frame, rng, drop=0.1):
 def set_mode(eng, mode):
     """plain: per-stage kernels with persistent intra workers; fused: one
     persistent kernel; levels: per-stage kernels, one intra launch per
-    dependency level (the first 64; persistent workers after)."""
+    dependency level (the first 64; persistent workers after); *_defer: the
+    same with the blocks' samples written by the parallel pass after the
+    ordered stage."""
     eng.set_fused(mode == "fused")
     if mode == "levels":
         eng.set_tune([1, 1, 2, 4], 64, 512, 1)
+    elif mode == "levels_defer":
+        eng.set_tune([1, 1, 2, 4], 64, 512, 1, 1)
+    elif mode == "plain_defer":
+        eng.set_tune([1, 1, 2, 4], 64, 512, 0, 1)
 
 
 def run_engine(pkg, frames, mode, w, h):
@@ -49,7 +55,7 @@ def test_fused_bit_exact(pkg, orc, gpu_ready, preset, w, h):
         assert np.array_equal(g, x), f"frame {i}"
 
 
-@pytest.mark.parametrize("mode", ["plain", "fused", "levels"])
+@pytest.mark.parametrize("mode", ["plain", "fused", "levels", "plain_defer", "levels_defer"])
 def test_shuffled_blocks_with_holes(pkg, orc, gpu_ready, mode):
     rng = np.random.default_rng(11)
     w, h = 200, 136
