@@ -281,46 +281,50 @@ __global__ __launch_bounds__(256) void k_owner(DBatch B)
  * a load inside a branch is waited for before the branch ends, which made
  * each row one memory round trip. */
 struct McWin {
-    uint64_t w[CVP9_MC_RUN + 1];
-    uint32_t sh[5];
+    uint32_t lo[CVP9_MC_RUN + 1], hi[CVP9_MC_RUN + 1]; /* bytes 0-3, 4-7 of each row */
+    uint32_t sel[4]; /* byte selectors of taps (0,1), (2,3), (1,2), (3,4), as 16-bit pairs */
 };
+
+/* Samples in packed 16-bit lanes: the filters run two samples per
+ * instruction (v_pk_mad_u16 and friends; every intermediate fits 16 bits:
+ * a * (8 - f) + b * f + 4 <= 2044). */
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
 
 __device__ __forceinline__ void mc_win(McWin &q, const uint8_t *ref, uint32_t rs, int rw, int rh,
                                        int sx, int y0, int m)
 {
     const int b = clampi(sx, 0, max(rw - 5, 0)) & ~3;
+    uint32_t o[5];
 #pragma unroll
-    for (int e = 0; e < 5; e++) q.sh[e] = 8u * (uint32_t)(clampi(sx + e, 0, rw - 1) - b);
+    for (int e = 0; e < 5; e++) o[e] = (uint32_t)(clampi(sx + e, 0, rw - 1) - b);
+    /* v_perm: selector byte i picks byte i of the result from the 8-byte
+     * window (0-3 the low dword, 4-7 the high one), 0x0c gives 0 */
+    q.sel[0] = o[0] | 0x0cu << 8 | o[1] << 16 | 0x0cu << 24;
+    q.sel[1] = o[2] | 0x0cu << 8 | o[3] << 16 | 0x0cu << 24;
+    q.sel[2] = o[1] | 0x0cu << 8 | o[2] << 16 | 0x0cu << 24;
+    q.sel[3] = o[3] | 0x0cu << 8 | o[4] << 16 | 0x0cu << 24;
     const GMEM uint8_t *base = gm(ref) + b;
 #pragma unroll
     for (int t = 0; t <= CVP9_MC_RUN; t++) {
         const int y = clampi(y0 + min(t, m), 0, rh - 1);
         const GMEM uint32_t *p = (const GMEM uint32_t *)(base + (size_t)y * rs);
-        q.w[t] = (uint64_t)p[1] << 32 | p[0];
+        q.lo[t] = p[0];
+        q.hi[t] = p[1];
     }
 }
 
-/* horizontal pass of window row t: 4 samples, 8 bits each */
-__device__ __forceinline__ uint32_t mc_hpass(const McWin &q, int t, uint32_t fx)
+/* horizontal pass of window row t: samples 0,1 and 2,3 in 16-bit lanes */
+__device__ __forceinline__ void mc_hpass(const McWin &q, int t, u16x2 fx, u16x2 gx, u16x2 &h01,
+                                         u16x2 &h23)
 {
-    uint32_t a[5];
-#pragma unroll
-    for (int e = 0; e < 5; e++) a[e] = (uint32_t)(q.w[t] >> q.sh[e]) & 0xff;
-    uint32_t h = 0;
-#pragma unroll
-    for (int e = 0; e < 4; e++) h |= ((a[e] * (8 - fx) + a[e + 1] * fx + 4) >> 3) << (8 * e);
-    return h;
-}
-
-__device__ __forceinline__ uint32_t mc_vpass(uint32_t t, uint32_t b, uint32_t fy)
-{
-    uint32_t out = 0;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const uint32_t x = (t >> (8 * e)) & 0xff, y = (b >> (8 * e)) & 0xff;
-        out |= ((x * (8 - fy) + y * fy + 4) >> 3) << (8 * e);
-    }
-    return out;
+    const u16x2 l01 = as_u16x2(__builtin_amdgcn_perm(q.hi[t], q.lo[t], q.sel[0]));
+    const u16x2 l23 = as_u16x2(__builtin_amdgcn_perm(q.hi[t], q.lo[t], q.sel[1]));
+    const u16x2 r01 = as_u16x2(__builtin_amdgcn_perm(q.hi[t], q.lo[t], q.sel[2]));
+    const u16x2 r23 = as_u16x2(__builtin_amdgcn_perm(q.hi[t], q.lo[t], q.sel[3]));
+    h01 = (l01 * gx + (r01 * fx + (u16x2)4)) >> (u16x2)3;
+    h23 = (l23 * gx + (r23 * fx + (u16x2)4)) >> (u16x2)3;
 }
 
 __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
@@ -335,29 +339,41 @@ __device__ __forceinline__ void store_quad(uint8_t *d, uint32_t v, int n)
 
 /* The m filtered rows of a run (flat 127 for an empty slot,
  * vp9_reconstruct.c:258-263, :328-333), the residual rows res added when
- * add, stored at d with pitch ds, nv samples of the quad valid; nothing is
- * stored by a lane with on false, whose values are computed all the same
- * (so that the loads they use stay ahead of every branch). */
+ * add (saturating 16-bit add, then the clamp to 0..255: the same as the
+ * reference's int sum clamped), stored at d with pitch ds, nv samples of the
+ * quad valid; nothing is stored by a lane with on false, whose values are
+ * computed all the same (so that the loads they use stay ahead of every
+ * branch). */
 __device__ __forceinline__ void mc_emit(const McWin &q, bool on, int m, bool flat, uint32_t fx,
                                         uint32_t fy, const uint64_t *res, bool add, uint8_t *d,
                                         uint32_t ds, int nv)
 {
-    uint32_t top = mc_hpass(q, 0, fx);
+    const u16x2 fx2 = (u16x2)(unsigned short)fx, gx2 = (u16x2)(unsigned short)(8 - fx);
+    const u16x2 fy2 = (u16x2)(unsigned short)fy, gy2 = (u16x2)(unsigned short)(8 - fy);
+    u16x2 t01, t23;
+    mc_hpass(q, 0, fx2, gx2, t01, t23);
 #pragma unroll
     for (int t = 0; t < CVP9_MC_RUN; t++) {
-        const uint32_t bot = mc_hpass(q, t + 1, fx);
-        uint32_t v = flat ? 0x7f7f7f7fu : mc_vpass(top, bot, fy);
-        top = bot;
+        u16x2 b01, b23;
+        mc_hpass(q, t + 1, fx2, gx2, b01, b23);
+        u16x2 v01 = (t01 * gy2 + (b01 * fy2 + (u16x2)4)) >> (u16x2)3;
+        u16x2 v23 = (t23 * gy2 + (b23 * fy2 + (u16x2)4)) >> (u16x2)3;
+        t01 = b01;
+        t23 = b23;
+        if (flat) v01 = v23 = (u16x2)127;
         if (add) {
-            uint32_t o = 0;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int p = (int)((v >> (8 * e)) & 0xff);
-                const int rv = (int16_t)(uint16_t)(res[t] >> (16 * e));
-                o |= (uint32_t)clampi(p + rv, 0, 255) << (8 * e);
-            }
-            v = o;
+            i16x2 s01 = __builtin_elementwise_add_sat(__builtin_bit_cast(i16x2, v01),
+                                                      __builtin_bit_cast(i16x2, (uint32_t)res[t]));
+            i16x2 s23 = __builtin_elementwise_add_sat(__builtin_bit_cast(i16x2, v23),
+                                                      __builtin_bit_cast(i16x2, (uint32_t)(res[t] >> 32)));
+            s01 = __builtin_elementwise_min(__builtin_elementwise_max(s01, (i16x2)0), (i16x2)255);
+            s23 = __builtin_elementwise_min(__builtin_elementwise_max(s23, (i16x2)0), (i16x2)255);
+            v01 = __builtin_bit_cast(u16x2, s01);
+            v23 = __builtin_bit_cast(u16x2, s23);
         }
+        /* low bytes of the four 16-bit lanes */
+        const uint32_t v = __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, v23),
+                                                 __builtin_bit_cast(uint32_t, v01), 0x06040200u);
         if (on && t < m) store_quad(d + (size_t)t * ds, v, nv);
     }
 }
