@@ -235,6 +235,8 @@ SIGNATURES = {
     "cvp9_hip_idct_table": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "cvp9_hip_plan_check": (C.c_int, [PF_P]),
     "cvp9_hip_plan_image": (C.c_int64, [PF_P, C.c_int, C.POINTER(C.c_uint8), C.c_uint64]),
+    "cvp9_hip_plan_image_compact": (C.c_int64, [PF_P, C.c_int, C.POINTER(C.c_uint8), C.c_uint64,
+                                                C.POINTER(C.c_uint64)]),
     "cvp9_hip_route_check": (C.c_int, [C.POINTER(PF_P), C.c_int, C.POINTER(C.c_int)]),
     "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_intra_levels": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
@@ -439,6 +441,23 @@ def plan_image(frame, threads=0):
     return out
 
 
+def plan_image_compact(frame, threads=0):
+    """The host part of the frame's device image with compact coefficients
+    (cvp9_hip_plan_image_compact): (image bytes, info dict)."""
+    info = (C.c_uint64 * 8)()
+    n = lib().cvp9_hip_plan_image_compact(frame.ptr, threads, None, 0, info)
+    if n < 0:
+        raise Cvp9Error(int(n), "cvp9_hip_plan_image_compact")
+    out = np.zeros(n, np.uint8)
+    r = lib().cvp9_hip_plan_image_compact(frame.ptr, threads, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          n, info)
+    if r < 0:
+        raise Cvp9Error(int(r), "cvp9_hip_plan_image_compact")
+    keys = ("off_cmask", "off_cstart", "off_cvals", "nnz", "n_chunks", "upload", "off_coeffs",
+            "bytes")
+    return out, dict(zip(keys, (int(x) for x in info)))
+
+
 def route_check(frames):
     """Codes hip_decode_frame would return for a sequence before any device
     work (validator + the decline-until-KEY rule); needs no device."""
@@ -537,7 +556,7 @@ class HipBackend:
 
 # ---- multi-stream engine (include/cvp9_hip/engine.h) ------------------------------
 STAGE_NAMES = ["fill", "owner", "mc", "residual4", "residual8", "residual16", "residual32",
-               "neighbours", "intra", "deblock", "nv12", "fused"]
+               "neighbours", "intra", "deblock", "nv12", "fused", "expand"]
 
 
 class Engine:

@@ -60,6 +60,12 @@ struct Backend {
      * it on the CPU, so the device reference pool lacks it. Every frame is
      * declined until the next KEY frame, which writes all 8 slots. */
     bool declined = false;
+    /* Coefficients staged compact (nonzero masks and values, expanded on
+     * the device) instead of verbatim: about a third of the PCIe bytes at
+     * 25% nonzero coefficients, for more host time per decode call (the
+     * values are packed on the host). Chosen per context at init from the
+     * environment: CVP9_HIP_STAGING=compact. */
+    bool compact = false;
 };
 
 /* An export buffer: dedicated VRAM with its DMA-BUF, and the CPU view the
@@ -191,6 +197,8 @@ cvp9_err_t hip_backend_init(void **ctx)
     Backend *b = new (std::nothrow) Backend();
     if (!b) return CVP9_ERR_NOMEM;
     b->device = dev;
+    const char *staging = getenv("CVP9_HIP_STAGING");
+    b->compact = staging && strcmp(staging, "compact") == 0;
     cvp9_err_t ce = cvp9_engine_create(dev, &b->eng);
     if (ce == CVP9_OK) ce = cvp9_engine_stream_open(b->eng, &b->sid);
     if (ce != CVP9_OK) {
@@ -241,6 +249,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     if (b->count == CVP9_HIP_INFLIGHT) return CVP9_ERR_AGAIN;
     const bool key = pf->hdr.frame_type == VP9_FRAME_KEY;
     PrepLayout L;
+    L.compact = b->compact;
     cvp9_err_t err = route(b->declined, pf, L);
     if (err != CVP9_OK) return err;
     /* an armed NV12 target must hold the whole frame */
@@ -256,7 +265,7 @@ cvp9_err_t hip_decode_frame(void *ctx, const vp9_parsed_frame_t *pf, int64_t pts
     Slot &s = b->slots[(b->head + b->count) % CVP9_HIP_INFLIGHT];
     const size_t need = L.bytes;
     hipError_t e;
-    if ((e = grow_pinned(&s.stage_host, &s.stage_host_cap, need)) != hipSuccess) return to_err(e);
+    if ((e = grow_pinned(&s.stage_host, &s.stage_host_cap, L.host_bytes)) != hipSuccess) return to_err(e);
     if ((e = grow_device(&s.stage_dev, &s.stage_dev_cap, need)) != hipSuccess) return to_err(e);
     const size_t ob = i420_bytes(L.W, L.H);
     if ((e = grow_pinned(&s.out_host, &s.out_host_cap, ob)) != hipSuccess) return to_err(e);
@@ -496,6 +505,30 @@ int64_t cvp9_hip_plan_image(const vp9_parsed_frame_t *pf, int threads, uint8_t *
         plan_write(pf, L, dst);
     }
     return (int64_t)L.bytes;
+}
+
+int64_t cvp9_hip_plan_image_compact(const vp9_parsed_frame_t *pf, int threads, uint8_t *dst,
+                                    uint64_t cap, uint64_t info[8])
+{
+    PrepLayout L;
+    L.compact = true;
+    cvp9_err_t err = plan_measure(pf, L, threads);
+    if (err != CVP9_OK) return err;
+    if (dst && cap >= L.host_bytes) {
+        memset(dst, 0, L.host_bytes);
+        plan_write(pf, L, dst);
+    }
+    if (info) {
+        info[0] = L.off_cmask;
+        info[1] = L.off_cstart;
+        info[2] = L.off_cvals;
+        info[3] = L.nnz;
+        info[4] = L.n_chunks;
+        info[5] = L.upload;
+        info[6] = L.off_coeffs;
+        info[7] = L.bytes;
+    }
+    return (int64_t)L.host_bytes;
 }
 
 int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap)

@@ -122,6 +122,10 @@ CVP9_HD static inline uint32_t cvp9_mode_reads(uint32_t ymode, uint32_t uvmode)
     return r[ymode & 3] | r[uvmode & 3];
 }
 
+/* Compact coefficients: chunks of this many pool entries, one 64-bit
+ * nonzero mask each. */
+#define CVP9_COEF_CHUNK 64
+
 /* block flags */
 #define DB_INTRA   1u
 #define DB_SKIP    2u   /* no residual (skip, or transform grid empty) */
@@ -174,6 +178,13 @@ struct DJob {
     uint32_t       *err_host;      /* the same flags in host-visible memory, set with
                                       them, so the host reads them without a copy */
     uint32_t       *ectr;          /* the stream's epoch counter (device) */
+    /* compact coefficients (n_chunks > 0): per chunk of 64 pool entries its
+     * nonzero mask and the position of its first value among cvals; the
+     * expansion stage writes the dense pool (coeffs, n_chunks * 64 entries) */
+    const uint64_t *cmask;
+    const uint32_t *cstart;
+    const int16_t  *cvals;
+    uint32_t        n_chunks;
     uint32_t n_blocks, n_inter, n_intra, n_mc;
     uint32_t n_mc_a;               /* motion-compensation units [0, n_mc_a) run in the
                                       early launch, the rest in the late one */
@@ -199,6 +210,7 @@ struct DBatch {
     const uint32_t *pre_lf;        /* deblocking: waves of 64 units */
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
+    const uint32_t *pre_exp;       /* compact coefficients: waves of 4 chunks */
     uint32_t       *jticket;       /* per-job intra ticket counters (zeroed by the descriptor
                                       copy), for jobs no persistent worker starts on */
     const uint32_t *jstart;        /* per job, the intra-list entry where the persistent

@@ -271,7 +271,9 @@ void measure_range(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H, uint32_
 cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads)
 {
     if (!pf) return CVP9_ERR_INVALID_DATA;
+    const bool compact = L.compact;
     L = PrepLayout();
+    L.compact = compact;
     const uint32_t W = pf->hdr.width, H = pf->hdr.height;
     /* Sizes the kernels take. The CPU deblock reads one row (column) past
      * the plane when H (W) is 1 mod 8, and the chroma bilinear clamp
@@ -354,7 +356,19 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads
     for (int k = 0; k < 4; k++) {
         L.off_tx_tab[k] = off; off = align256(off + (uint64_t)L.cls_ntx[k] * 8);
     }
-    L.off_coeffs = off; off = align256(off + (uint64_t)L.num_coeffs * 2);
+    if (L.compact) {
+        L.n_chunks = (L.num_coeffs + 63) / 64;
+        L.off_cmask = off;  off = align256(off + (uint64_t)L.n_chunks * 8);
+        L.off_cstart = off; off = align256(off + (uint64_t)L.n_chunks * 4);
+        L.off_cvals = off;  off = align256(off + (uint64_t)L.num_coeffs * 2 + 2);
+        L.host_bytes = off;
+        L.off_coeffs = off; off = align256(off + (uint64_t)L.n_chunks * 64 * 2);
+        L.upload = L.host_bytes; /* until plan_write knows the values' count */
+    } else {
+        L.n_chunks = 0;
+        L.off_coeffs = off; off = align256(off + (uint64_t)L.num_coeffs * 2);
+        L.host_bytes = L.upload = off;
+    }
     L.bytes = off;
     return CVP9_OK;
 }
@@ -575,6 +589,51 @@ void write_range(const vp9_parsed_frame_t *pf, const PrepLayout &L, uint8_t *dst
     }
 }
 
+/* Nonzero mask of 64 pool entries: compares and byte masks of whole
+ * vector registers (a boolean vector's bits are its lanes). */
+typedef short v64s __attribute__((ext_vector_type(64)));
+typedef bool v64b __attribute__((ext_vector_type(64)));
+inline uint64_t nonzero_mask64(const int16_t *p)
+{
+    v64s v;
+    memcpy(&v, p, sizeof(v));
+    const v64b nz = __builtin_convertvector(v != 0, v64b);
+    return __builtin_bit_cast(uint64_t, nz);
+}
+
+/* Compact coefficients, first pass: the nonzero mask of each chunk
+ * [c0, c1) of 64 pool entries (entries past the pool's end count as
+ * zero); returns the chunks' nonzero count. */
+uint32_t chunk_masks(const int16_t *pool, uint32_t n, uint32_t c0, uint32_t c1, uint64_t *mask)
+{
+    uint32_t cnt = 0;
+    for (uint32_t c = c0; c < c1; c++) {
+        const uint32_t a = c * 64, e = std::min(n, a + 64);
+        uint64_t m = 0;
+        if (e - a == 64) {
+            m = nonzero_mask64(pool + a);
+        } else {
+            for (uint32_t i = a; i < e; i++)
+                if (pool[i]) m |= 1ull << (i - a);
+        }
+        mask[c] = m;
+        cnt += (uint32_t)__builtin_popcountll(m);
+    }
+    return cnt;
+}
+
+/* Second pass: each chunk's position among the packed values, and the
+ * values, from position pos on. */
+void chunk_values(const int16_t *pool, uint32_t c0, uint32_t c1, const uint64_t *mask, uint32_t pos,
+                  uint32_t *start, int16_t *vals)
+{
+    for (uint32_t c = c0; c < c1; c++) {
+        start[c] = pos;
+        const int16_t *p = pool + (size_t)c * 64;
+        for (uint64_t m = mask[c]; m; m &= m - 1) vals[pos++] = p[__builtin_ctzll(m)];
+    }
+}
+
 } // namespace
 
 void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
@@ -590,9 +649,30 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
         acc.nm_0 += k.n_mc_0;
         for (int t = 0; t < 4; t++) acc.tx[t] += k.ntx[t];
     }
-    /* the parser's coefficients, verbatim, in P slices */
+    /* the parser's coefficients, verbatim, in P slices; or, compact, each
+     * slice's chunk masks and then its values, which start where the
+     * previous slice's end: a slice waits for the previous one's count
+     * (tasks are taken in index order, and a serial run takes them in
+     * that order too, so the slice it waits for is running or done), then
+     * packs its values while its coefficients are still in cache */
     const size_t cbytes = (size_t)L.num_coeffs * 2;
+    const int16_t *pool = (const int16_t *)pf->coeffs;
+    uint64_t *cmask = (uint64_t *)(dst + L.off_cmask);
+    std::vector<uint32_t> slice_end((size_t)P + 1, 0);
+    std::atomic<int> counted{0}; /* slices whose end position is known */
     auto copy_slice = [&](int c) {
+        if (L.compact) {
+            const uint32_t a = (uint32_t)((uint64_t)L.n_chunks * c / P),
+                           b = (uint32_t)((uint64_t)L.n_chunks * (c + 1) / P);
+            const uint32_t cnt = chunk_masks(pool, L.num_coeffs, a, b, cmask);
+            while (counted.load(std::memory_order_acquire) < c) __builtin_ia32_pause();
+            const uint32_t base = slice_end[c];
+            slice_end[c + 1] = base + cnt;
+            counted.store(c + 1, std::memory_order_release);
+            chunk_values(pool, a, b, cmask, base, (uint32_t *)(dst + L.off_cstart),
+                         (int16_t *)(dst + L.off_cvals));
+            return;
+        }
         const size_t a = cbytes * c / P, b = cbytes * (c + 1) / P;
         if (b > a) memcpy(dst + L.off_coeffs + a, (const uint8_t *)pf->coeffs + a, b - a);
     };
@@ -618,6 +698,12 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
             else if (t <= P) write_range(pf, L, dst, L.chunks[t - 1], at[t - 1]);
             else copy_slice(t - 1 - P);
         });
+    }
+    if (L.compact) {
+        L.nnz = slice_end[P];
+        /* at least one value: the device reads entry 0 for zero entries */
+        if (L.nnz == 0) *(int16_t *)(dst + L.off_cvals) = 0;
+        L.upload = L.off_cvals + 2ull * (L.nnz ? L.nnz : 1);
     }
 }
 
@@ -841,6 +927,12 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.err = errd_;
     j.err_host = err_dev_;
     j.ectr = ectr_;
+    if (L.compact && L.n_chunks) {
+        j.cmask = (const uint64_t *)(dev + L.off_cmask);
+        j.cstart = (const uint32_t *)(dev + L.off_cstart);
+        j.cvals = (const int16_t *)(dev + L.off_cvals);
+        j.n_chunks = L.n_chunks;
+    }
     j.n_blocks = L.n_blocks;
     j.n_inter = L.n_inter;
     j.n_mc = L.n_mc;
@@ -869,7 +961,7 @@ size_t StreamState::resident_bytes() const
 
 size_t BatchBuilder::bytes_for(uint32_t njobs)
 {
-    return align256((uint64_t)njobs * sizeof(DJob)) + align256(13ull * (njobs + 1) * 4) + CVP9_BAR_BYTES +
+    return align256((uint64_t)njobs * sizeof(DJob)) + align256(14ull * (njobs + 1) * 4) + CVP9_BAR_BYTES +
            3ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
 }
 
@@ -880,8 +972,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
     uint32_t *pre = (uint32_t *)(host + pre_off);
     const uint32_t stride = njobs + 1;
-    enum { P_FILL, P_BLK, P_INTER, P_INTER2, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_N };
-    static_assert(P_N + 2 <= 13, "prefix rows, the ticket row and the start row fit the descriptor");
+    enum { P_FILL, P_BLK, P_INTER, P_INTER2, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_EXP, P_N };
+    static_assert(P_N + 2 <= 14, "prefix rows, the ticket row and the start row fit the descriptor");
     for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
 
     for (uint32_t i = 0; i < njobs; i++) {
@@ -903,6 +995,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
         c[P_INTRA] = J.n_intra;
         c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;
         c[P_NV12] = J.nv12 ? div_up(J.out.w, 64) * div_up(J.out.h, 64) : 0;
+        c[P_EXP] = div_up(J.n_chunks, 4u);
         for (int a = 0; a < P_N; a++) pre[a * stride + i + 1] = pre[a * stride + i] + c[a];
     }
     const uint32_t *dpre = (const uint32_t *)(dev + pre_off);
@@ -916,9 +1009,10 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     b_.pre_intra = dpre + P_INTRA * stride;
     b_.pre_lf = dpre + P_LF * stride;
     b_.pre_nv12 = dpre + P_NV12 * stride;
+    b_.pre_exp = dpre + P_EXP * stride;
     b_.jticket = (uint32_t *)(dev + pre_off) + P_N * stride;
     b_.jstart = (const uint32_t *)(dev + pre_off) + (P_N + 1) * stride;
-    const uint64_t tk_off = pre_off + align256(13ull * stride * 4);
+    const uint64_t tk_off = pre_off + align256(14ull * stride * 4);
     b_.bar = (uint32_t *)(dev + tk_off);
     /* the fused kernel's barrier words start at zero */
     const size_t bar_bytes = fused ? CVP9_BAR_BYTES : 256;
@@ -934,6 +1028,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     n_.intra = pre[P_INTRA * stride + njobs];
     n_.lf = pre[P_LF * stride + njobs];
     n_.nv12 = pre[P_NV12 * stride + njobs];
+    n_.exp = pre[P_EXP * stride + njobs];
     n_.tune = tune;
     used_ = tk_off + bar_bytes;
 
@@ -1000,6 +1095,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
 
 hipError_t BatchBuilder::launch(const DBatch &b, const StageCounts &n, hipStream_t s)
 {
+    if (hipError_t e = cvp9_launch_stage(STAGE_EXPAND, b, n, s)) return e;
     for (int st = 0; st < STAGE_FUSED; st++) {
         hipError_t e = cvp9_launch_stage(st, b, n, s);
         if (e != hipSuccess) return e;

@@ -84,6 +84,18 @@ struct PrepLayout {
     uint64_t off_blocks = 0, off_intra = 0, off_mc_tab = 0;
     uint64_t off_tx_tab[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
+    /* Compact coefficients (set compact before plan_measure): the parser's
+     * pool in chunks of 64, a 64-bit mask of each chunk's nonzero entries
+     * (off_cmask), the position of each chunk's first nonzero value among
+     * the packed values (off_cstart) and the nonzero values in pool order
+     * (off_cvals, room for the whole pool); the dense pool at off_coeffs,
+     * n_chunks * 64 entries, is written on the device from them. Only the
+     * image's first `upload` bytes (set by plan_write) need to reach the
+     * device, and only its first `host_bytes` are written on the host. */
+    bool compact = false;
+    uint32_t n_chunks = 0, nnz = 0;
+    uint64_t off_cmask = 0, off_cstart = 0, off_cvals = 0;
+    uint64_t upload = 0, host_bytes = 0;
     /* written by plan_write: intra-list position where each dependency
      * level starts, then the list's end (empty without intra blocks), and
      * the number of small blocks (at most 16x16, first in their level) */
@@ -94,7 +106,8 @@ struct PrepLayout {
     std::vector<PlanChunk> chunks;
 };
 
-/* Validates pf and fills L. Returns CVP9_OK, CVP9_ERR_UNSUPPORTED (layout
+/* Validates pf and fills L (keeping L.compact, the caller's choice of
+ * coefficient layout). Returns CVP9_OK, CVP9_ERR_UNSUPPORTED (layout
  * this backend does not take: the caller falls back to the CPU path) or
  * CVP9_ERR_INVALID_DATA (coefficient references outside the pool): the code
  * of the first offending block in list order, as a serial pass finds it.

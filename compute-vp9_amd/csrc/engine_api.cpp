@@ -44,10 +44,10 @@ struct BatchSlot {
  * transforms to int16 residuals, motion compensation of the early units),
  * the second on the previous frame (motion compensation of the late units,
  * with the residual add like the early ones, intra, deblocking). */
-constexpr int kOrderPlain[] = {STAGE_FILL, STAGE_OWNER, STAGE_MC, STAGE_RES4, STAGE_RES8,
+constexpr int kOrderPlain[] = {STAGE_EXPAND, STAGE_FILL, STAGE_OWNER, STAGE_MC, STAGE_RES4, STAGE_RES8,
                                STAGE_RES16, STAGE_RES32, STAGE_NBR, STAGE_INTRA, STAGE_LF,
                                STAGE_NV12};
-constexpr int kOrderA[] = {STAGE_FILL, STAGE_OWNER, STAGE_NBR, STAGE_RES4, STAGE_RES8,
+constexpr int kOrderA[] = {STAGE_EXPAND, STAGE_FILL, STAGE_OWNER, STAGE_NBR, STAGE_RES4, STAGE_RES8,
                            STAGE_RES16, STAGE_RES32, STAGE_MC};
 constexpr int kOrderB[] = {STAGE_MC, STAGE_INTRA, STAGE_LF, STAGE_NV12};
 
@@ -70,6 +70,7 @@ bool stage_has_work(int st, const StageCounts &n)
     case STAGE_INTRA: return n.intra != 0;
     case STAGE_LF: return n.lf != 0;
     case STAGE_NV12: return n.nv12 != 0;
+    case STAGE_EXPAND: return n.exp != 0;
     }
     return false;
 }
@@ -480,7 +481,7 @@ cvp9_err_t engine_run_staged(cvp9_engine_t *e, int sid, const PrepLayout &L, con
     p.L = L;
     p.dev = dev;
     cvp9_prep_t *pp = &p;
-    const Upload up{host, dev, (size_t)L.bytes};
+    const Upload up{host, dev, (size_t)L.upload};
     const cvp9_err_t err = run_batch(e, 1, &sid, &pp, nv12 ? &nv12 : nullptr, &up);
     p.dev = nullptr; /* not the engine's to free */
     if (err != CVP9_OK) return err;
@@ -626,6 +627,10 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
             if (!t.a || !t.b) return CVP9_ERR_GPU;
             hipEventRecord(t.a, qs);
         }
+        /* compact coefficients are expanded by a launch of their own first */
+        if (bb.counts().exp &&
+            (he = cvp9_launch_stage(STAGE_EXPAND, bb.batch(), bb.counts(), qs)) != hipSuccess)
+            return to_err(he);
         if ((he = cvp9_launch_fused(bb.batch(), e->fused_wgs, qs)) != hipSuccess) return to_err(he);
         if (timing) {
             hipEventRecord(t.b, qs);

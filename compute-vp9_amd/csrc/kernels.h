@@ -18,6 +18,7 @@ enum Stage {
     STAGE_LF,
     STAGE_NV12,
     STAGE_FUSED, /* the fused kernel (all of the above in one launch) */
+    STAGE_EXPAND, /* compact coefficients to the dense pool (runs first) */
     STAGE_COUNT
 };
 
@@ -37,6 +38,7 @@ struct Tune {
 /* Total work items of each stage over the whole batch (the last element of
  * each prefix array in DBatch), plus launch parameters. */
 struct StageCounts {
+    uint32_t exp;             /* compact-coefficient waves (4 chunks each) */
     uint32_t fill, blocks, inter, inter2, grp[4], intra, lf, nv12; /* inter: early motion-
                                       compensation units, inter2: late ones (padded) */
     uint32_t levels;          /* leading levels launched one by one */

@@ -221,6 +221,50 @@ __global__ __launch_bounds__(256) void k_fill(DBatch B)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Compact coefficients (a frame staged through the drop-in boundary): the  */
+/* parser's pool arrives as a nonzero mask per chunk of 64 entries, each    */
+/* chunk's first position among the packed values, and the nonzero values; */
+/* this stage writes the dense pool the transforms read. One wave per four  */
+/* chunks, a lane 4 entries (one 8-byte store): its values sit at the       */
+/* chunk's start plus the nonzero count below its first entry, and its     */
+/* four loads are issued together at clamped places.                        */
+
+__device__ __forceinline__ void expand_item(const DBatch &B, uint32_t w)
+{
+    const uint32_t j = find_job_wave(B.pre_exp, B.njobs, w);
+    const DJob &J = B.jobs[j];
+    const uint32_t lane = __lane_id();
+    const uint32_t c = (w - B.pre_exp[j]) * 4 + (lane >> 4);
+    const uint32_t cc = min(c, J.n_chunks - 1);
+    const uint64_t m = gm(J.cmask)[cc];
+    const uint32_t st = gm(J.cstart)[cc];
+    const uint32_t b = (lane & 15) * 4;
+    const uint32_t nib = (uint32_t)(m >> b) & 15;
+    uint32_t k = st + (uint32_t)__builtin_popcountll(m & ((1ull << b) - 1));
+    const GMEM int16_t *vals = gm(J.cvals);
+    uint32_t v[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const bool nz = (nib >> e) & 1;
+        const int16_t x = vals[nz ? k : 0];
+        v[e] = nz ? (uint32_t)(uint16_t)x : 0u;
+        k += nz;
+    }
+    if (c < J.n_chunks) {
+        GMEM uint32_t *d = (GMEM uint32_t *)gm((int16_t *)J.coeffs + (size_t)c * CVP9_COEF_CHUNK + b);
+        d[0] = v[0] | v[1] << 16;
+        d[1] = v[2] | v[3] << 16;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_expand(DBatch B)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= B.pre_exp[B.njobs]) return; /* uniform per wave */
+    expand_item(B, w);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Owner map: cell (4x4 luma) -> local index of the block covering it.      */
 
 /* block wv of the batch on 16 lanes (lane = 0..15); jhint: the job of the
@@ -2415,6 +2459,9 @@ hipError_t cvp9_launch_stage(int stage, const DBatch &b, const StageCounts &n, h
         break;
     case STAGE_NV12:
         if (n.nv12) CVP9_LAUNCH(done, k_nv12, dim3(n.nv12), dim3(256), s, b);
+        break;
+    case STAGE_EXPAND:
+        if (n.exp) hipLaunchKernelGGL(k_expand, dim3(div_up(n.exp, 4)), dim3(256), 0, s, b);
         break;
     default:
         return hipErrorInvalidValue;

@@ -140,8 +140,10 @@ void       cvp9_engine_graph_free(cvp9_engine_t *e, cvp9_graph_t *g);
  * each of the CVP9_ENGINE_STAGES stages, the summed milliseconds and the
  * number of launches since the last reset (it synchronises). Stage order:
  * fill, owner, mc, residual4, residual8, residual16, residual32,
- * neighbours (of intra blocks), intra, deblock, nv12, fused. */
-#define CVP9_ENGINE_STAGES 12
+ * neighbours (of intra blocks), intra, deblock, nv12, fused, expand (the
+ * compact coefficients of a frame staged through the drop-in boundary,
+ * written out as the dense pool before the transforms). */
+#define CVP9_ENGINE_STAGES 13
 cvp9_err_t cvp9_engine_set_timing(cvp9_engine_t *e, int on);
 cvp9_err_t cvp9_engine_stage_stats(cvp9_engine_t *e, double *ms, uint32_t *launches,
                                    int reset);

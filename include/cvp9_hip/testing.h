@@ -37,6 +37,17 @@ cvp9_err_t cvp9_hip_plan_check(const vp9_parsed_frame_t *pf);
  * returns its size in bytes, or a negative cvp9_err_t. The image never
  * depends on the number of threads. Needs no device. */
 int64_t cvp9_hip_plan_image(const vp9_parsed_frame_t *pf, int threads, uint8_t *dst, uint64_t cap);
+/* The host part of pf's device image with compact coefficients, as the
+ * drop-in boundary stages it: the tables as above, then per chunk of 64
+ * pool entries a 64-bit nonzero mask (at info[0]) and the position of its
+ * first value among the packed nonzero values (u32, at info[1]), then the
+ * values (int16, at info[2]); info[3] the number of values, info[4] of
+ * chunks, info[5] the bytes that cross to the device, info[6] the offset of
+ * the dense pool the device writes from them and info[7] the whole image's
+ * size. Writes dst (when cap is large enough) and returns the host part's
+ * size, or a negative cvp9_err_t. Needs no device. */
+int64_t cvp9_hip_plan_image_compact(const vp9_parsed_frame_t *pf, int threads, uint8_t *dst,
+                                    uint64_t cap, uint64_t info[8]);
 /* The order in which intra workers take pf's intra blocks: indices into
  * pf->blocks, by dependency level (no earlier intra block among the
  * above/left/above-left neighbours: level 0), then list index. Returns the
