@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <new>
 #include <vector>
 
@@ -18,6 +19,8 @@ using namespace cvp9;
 
 
 namespace {
+
+std::atomic<int> g_live_engines{0};
 
 constexpr int kBatchSlots = 4;
 /* Flags of the engine's ordering events (batch done, phase and queue order,
@@ -176,6 +179,7 @@ struct cvp9_engine {
     bool fused = false;
     uint32_t fused_wgs = 0;
     cvp9_graph *capture = nullptr; /* recording batches into a graph */
+    bool counted = false;          /* counted in g_live_engines */
 
     hipStream_t queue(int q) const { return q ? stream2 : stream; }
 
@@ -250,6 +254,11 @@ struct cvp9_engine {
     }
 };
 
+int cvp9_live_engines()
+{
+    return g_live_engines.load(std::memory_order_relaxed);
+}
+
 extern "C" {
 
 cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
@@ -294,6 +303,8 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
             return CVP9_ERR_GPU;
         }
     }
+    g_live_engines.fetch_add(1, std::memory_order_relaxed);
+    e->counted = true;
     *out = e;
     return CVP9_OK;
 }
@@ -301,6 +312,7 @@ cvp9_err_t cvp9_engine_create(int device, cvp9_engine_t **out)
 void cvp9_engine_destroy(cvp9_engine_t *e)
 {
     if (!e) return;
+    if (e->counted) g_live_engines.fetch_sub(1, std::memory_order_relaxed);
     hipSetDevice(e->device);
     if (e->stream) e->sync_all();
     for (StreamState *s : e->streams) delete s;
@@ -631,7 +643,11 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
         if (bb.counts().exp &&
             (he = cvp9_launch_stage(STAGE_EXPAND, bb.batch(), bb.counts(), qs)) != hipSuccess)
             return to_err(he);
-        if ((he = cvp9_launch_fused(bb.batch(), e->fused_wgs, qs)) != hipSuccess) return to_err(he);
+        /* every workgroup of a fused launch must be resident at once: with
+         * several engines in the process, each takes its share */
+        const int live = cvp9_live_engines();
+        const uint32_t wgs = live > 1 ? std::max(e->fused_wgs / (uint32_t)live, 8u) : e->fused_wgs;
+        if ((he = cvp9_launch_fused(bb.batch(), wgs, qs)) != hipSuccess) return to_err(he);
         if (timing) {
             hipEventRecord(t.b, qs);
             e->timed.push_back(t);

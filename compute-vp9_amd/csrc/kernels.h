@@ -74,6 +74,11 @@ hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageC
 hipError_t cvp9_launch_mc_late(const DBatch &b, const StageCounts &n, hipStream_t s,
                                hipEvent_t *done = nullptr);
 
+/* Engines alive in this process (each may have a waiting launch on each of
+ * its two queues): the persistent launches of all of them share the
+ * device's resident capacity. */
+int cvp9_live_engines();
+
 /* Fused persistent kernel: every stage of the batch in one launch of `wgs`
  * co-resident workgroups (at most cvp9_fused_max_wgs()). */
 hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s);

@@ -2215,16 +2215,8 @@ hipError_t cvp9_launch_fused(const DBatch &b, uint32_t wgs, hipStream_t s)
     return hipGetLastError();
 }
 
-/* Half the workgroups of k_intra<W> that the current device holds at once
- * when nothing else runs (cached per device). Persistent workers own their
- * intra entries by stride, and a worker may wait on an entry of any other
- * worker of its job, so every workgroup of a launch must become resident
- * while the others wait. The kernels that run beside it never wait, so
- * they end and give their slots up; the only other launch that may wait is
- * the intra stage of a batch on the engine's other queue, which is held to
- * the other half. */
 template <int W, bool DEFER>
-static uint32_t intra_resident_wgs()
+static uint32_t intra_capacity()
 {
     static uint32_t cache[16] = {};
     int dev = 0;
@@ -2234,10 +2226,29 @@ static uint32_t intra_resident_wgs()
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra<W, DEFER>, 256, 0) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             per_cu < 1 || cus < 1)
-            return 128;
-        cache[dev] = (uint32_t)(per_cu * cus) / 2;
+            return 256;
+        cache[dev] = (uint32_t)(per_cu * cus);
     }
     return cache[dev];
+}
+
+/* The workgroups of k_intra<W> a launch may have. Persistent workers own
+ * their intra entries by stride, and a worker may wait on an entry of any
+ * other worker of its job, so every workgroup of a launch must become
+ * resident while the others wait. The kernels that run beside it never
+ * wait, so they end and give their slots up; the other launches that may
+ * wait are the intra stages on the other queue of the same engine and on
+ * the queues of the other engines of the process. So a launch is held to
+ * the device's resident capacity for the kernel (when nothing else runs;
+ * cached per device) divided over two queues per live engine. Engines of
+ * other processes on the same device are not counted: a wait that then
+ * cannot end gives up at its bound and is reported as a device error. */
+template <int W, bool DEFER>
+static uint32_t intra_resident_wgs()
+{
+    const int live = cvp9_live_engines();
+    const uint32_t per = intra_capacity<W, DEFER>() / (2u * (uint32_t)(live > 1 ? live : 1));
+    return per ? per : 1u;
 }
 
 uint32_t cvp9_fused_max_wgs(int device)
