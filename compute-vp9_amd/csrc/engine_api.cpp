@@ -50,8 +50,8 @@ struct BatchSlot {
 constexpr int kOrderPlain[] = {STAGE_EXPAND, STAGE_FILL, STAGE_OWNER, STAGE_MC, STAGE_RES4, STAGE_RES8,
                                STAGE_RES16, STAGE_RES32, STAGE_NBR, STAGE_INTRA, STAGE_LF,
                                STAGE_NV12};
-constexpr int kOrderA[] = {STAGE_EXPAND, STAGE_FILL, STAGE_OWNER, STAGE_NBR, STAGE_RES4, STAGE_RES8,
-                           STAGE_RES16, STAGE_RES32, STAGE_MC};
+constexpr int kOrderA[] = {STAGE_EXPAND, STAGE_FILL, STAGE_OWNER, STAGE_RES4, STAGE_RES8,
+                           STAGE_RES16, STAGE_RES32, STAGE_NBR, STAGE_MC};
 constexpr int kOrderB[] = {STAGE_MC, STAGE_INTRA, STAGE_LF, STAGE_NV12};
 
 struct TimedLaunch {
@@ -619,8 +619,12 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
     if (e->fused) tune.intra_level_min = 0; /* the fused kernel runs persistent workers only */
     bb.build(jobs.data(), lays.data(), (uint32_t)n, dhost, ddev, tune, e->spin_limit, e->fused);
     bb.set_resid_all(resid_all);
+    /* (a reader that has finished needs no wait: a queue wait costs the
+     * host several microseconds, a query a fraction of one, and when the
+     * host is what limits the rate the device is usually done) */
     for (hipEvent_t ev : readers)
-        if ((he = hipStreamWaitEvent(qa, ev, 0)) != hipSuccess) return to_err(he);
+        if (hipEventQuery(ev) != hipSuccess && (he = hipStreamWaitEvent(qa, ev, 0)) != hipSuccess)
+            return to_err(he);
     if (overlap && e->graph_pending[q]) {
         if ((he = hipStreamWaitEvent(qa, e->gev, 0)) != hipSuccess) return to_err(he);
         e->graph_pending[q] = false;

@@ -867,14 +867,18 @@ constexpr size_t RES_SMALL_LDS =
     cmax(cmax(ResShape<4, 1>::LDS_DOUBLES, ResShape<8, 1>::LDS_DOUBLES),
          cmax(ResShape<16, 2>::LDS_DOUBLES, ResShape<32, 4>::LDS_DOUBLES));
 
-__global__ __launch_bounds__(256) void k_res_small(DBatch B, uint3 wg)
+__device__ __forceinline__ void res_small_wg(const DBatch &B, uint32_t b, uint3 wg, double *lds)
 {
-    __shared__ double lds[RES_SMALL_LDS];
-    const uint32_t b = blockIdx.x;
     if (b < wg.x) residual_unit<4, 1>(B, b, lds);
     else if (b < wg.y) residual_unit<8, 1>(B, b - wg.x, lds);
     else if (b < wg.z) residual_unit<16, 2>(B, b - wg.y, lds);
     else residual_unit<32, 4>(B, b - wg.z, lds);
+}
+
+__global__ __launch_bounds__(256) void k_res_small(DBatch B, uint3 wg)
+{
+    __shared__ double lds[RES_SMALL_LDS];
+    res_small_wg(B, blockIdx.x, wg, lds);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -997,15 +1001,44 @@ __device__ __forceinline__ void nbr_item(const DBatch &B, uint32_t wv, uint32_t 
         rec[CVP9_NB_BLOCK + li - 3] = ((const GMEM uint64_t *)gm(blocks + b))[li - 3];
 }
 
-__global__ __launch_bounds__(256) void k_nbr(DBatch B)
+/* workgroup wg of nwg: 8 intra blocks, each XCD a contiguous run */
+__device__ __forceinline__ void nbr_wg(const DBatch &B, uint32_t wg, uint32_t nwg)
 {
     const uint32_t total = B.pre_intra[B.njobs];
-    const uint32_t w0 =
-        __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 6) * 2);
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(xcd_block(wg, nwg) * 8 + (threadIdx.x >> 6) * 2);
     if (w0 >= total) return;
     const uint32_t jh = find_job_wave(B.pre_intra, B.njobs, w0); /* the wave's first block */
     const uint32_t wv = w0 + ((threadIdx.x & 63) >> 5);
     if (wv < total) nbr_item(B, wv, threadIdx.x & 31, jh);
+}
+
+__global__ __launch_bounds__(256) void k_nbr(DBatch B)
+{
+    nbr_wg(B, blockIdx.x, gridDim.x);
+}
+
+/* The pipelined first phase in two launches instead of four: the owner map
+ * beside every transform size (k_owner_res: they touch disjoint data), then
+ * the neighbour records beside the early motion compensation (k_nbr_mc:
+ * the records read the owner map, motion compensation the residuals). A
+ * launch costs the host a few microseconds, and one stream's frame rate is
+ * bound by the host's enqueue time. Each launch's first part is padded to
+ * whole groups of 8 workgroups, so that workgroup b of its second part
+ * still runs on XCD b % 8. */
+__global__ __launch_bounds__(256) void k_owner_res(DBatch B, uint32_t owner_wgs, uint3 wg)
+{
+    __shared__ double lds[RES_SMALL_LDS];
+    if (blockIdx.x < owner_wgs) owner_wg(B, blockIdx.x);
+    else res_small_wg(B, blockIdx.x - owner_wgs, wg, lds);
+}
+
+__global__ __launch_bounds__(256) void k_nbr_mc(DBatch B, uint32_t nbr_wgs, uint32_t nbr_used)
+{
+    if (blockIdx.x < nbr_wgs) {
+        if (blockIdx.x < nbr_used) nbr_wg(B, blockIdx.x, nbr_used);
+    } else {
+        mc_wg(B, blockIdx.x - nbr_wgs, gridDim.x - nbr_wgs, false);
+    }
 }
 
 /* The samples of an intra cell in the resolved layout (bytes 0-3 luma,
@@ -2402,11 +2435,21 @@ static inline uint3 res_small_starts(const StageCounts &n, uint32_t *total)
         }                                                                                      \
     } while (0)
 
+static inline uint32_t pad8(uint32_t x) { return (x + 7) & ~7u; }
+
 int cvp9_stage_span(int stage, const DBatch &b, const StageCounts &n)
 {
     /* owner map + motion compensation (with resid_all, motion compensation
      * follows the transforms instead) */
     if (stage == STAGE_OWNER && n.inter && !b.resid_all) return 2;
+    /* with resid_all (the pipelined first phase): owner map + the four
+     * transform sizes, then neighbour records + early motion compensation */
+    if (stage == STAGE_OWNER && b.resid_all) {
+        uint32_t total;
+        res_small_starts(n, &total);
+        return total && total <= CVP9_RES_SMALL_MAX_WGS ? 5 : 1;
+    }
+    if (stage == STAGE_NBR && b.resid_all && n.intra && n.inter) return 2;
     if (stage == STAGE_RES4) {
         uint32_t total;
         res_small_starts(n, &total);
@@ -2421,6 +2464,18 @@ hipError_t cvp9_launch_stages(int stage, int span, const DBatch &b, const StageC
     if (span == 2 && stage == STAGE_OWNER) {
         const uint32_t ow = n.blocks ? div_up(n.blocks, 16) : 1;
         hipLaunchKernelGGL(k_front, dim3(ow + n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, ow);
+        return hipGetLastError();
+    }
+    if (span == 5 && stage == STAGE_OWNER) {
+        uint32_t total;
+        const uint3 wg = res_small_starts(n, &total);
+        const uint32_t ow = pad8(n.blocks ? div_up(n.blocks, 16) : 1);
+        hipLaunchKernelGGL(k_owner_res, dim3(ow + total), dim3(256), 0, s, b, ow, wg);
+        return hipGetLastError();
+    }
+    if (span == 2 && stage == STAGE_NBR) {
+        const uint32_t used = div_up(n.intra, 8), nw = pad8(used);
+        hipLaunchKernelGGL(k_nbr_mc, dim3(nw + n.inter / CVP9_MC_WG_UNITS), dim3(256), 0, s, b, nw, used);
         return hipGetLastError();
     }
     if (span == 4 && stage == STAGE_RES4) {
