@@ -123,12 +123,12 @@ def stage_bytes(P, frame):
 
 
 def tune_args(a):
-    """(res_split, intra_lanes, intra_workers, intra_level_min, intra_defer)
-    from 7 or 8 integers."""
-    if len(a) not in (7, 8):
-        raise SystemExit("a launch shape is 7 or 8 integers: res_split x4, intra_lanes, "
-                         "intra_workers, intra_level_min[, intra_defer]")
-    return a[:4], a[4], a[5], a[6], (a[7] if len(a) == 8 else 0)
+    """(res_split, intra_lanes, intra_workers, intra_level_min, intra_defer,
+    intra_chains) from 7 to 9 integers."""
+    if len(a) not in (7, 8, 9):
+        raise SystemExit("a launch shape is 7 to 9 integers: res_split x4, intra_lanes, "
+                         "intra_workers, intra_level_min[, intra_defer[, intra_chains]]")
+    return a[:4], a[4], a[5], a[6], (a[7] if len(a) > 7 else 0), (a[8] if len(a) > 8 else 0)
 
 
 def single_stream(P, args):

@@ -139,12 +139,12 @@ class Tune(C.Structure):
     """cvp9_tune_t (include/cvp9_hip/engine.h)."""
     _fields_ = [("res_split", C.c_uint8 * 4), ("intra_lanes", C.c_uint16),
                 ("intra_workers", C.c_uint32), ("intra_level_min", C.c_uint32),
-                ("intra_defer", C.c_uint32)]
+                ("intra_defer", C.c_uint32), ("intra_chains", C.c_uint32)]
 
     def as_dict(self):
         return {"res_split": list(self.res_split), "intra_lanes": self.intra_lanes,
                 "intra_workers": self.intra_workers, "intra_level_min": self.intra_level_min,
-                "intra_defer": self.intra_defer}
+                "intra_defer": self.intra_defer, "intra_chains": self.intra_chains}
 
 
 class SynthParams(C.Structure):
@@ -240,6 +240,7 @@ SIGNATURES = {
     "cvp9_hip_route_check": (C.c_int, [C.POINTER(PF_P), C.c_int, C.POINTER(C.c_int)]),
     "cvp9_hip_intra_order": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_intra_levels": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
+    "cvp9_hip_intra_chains": (C.c_int, [PF_P, C.POINTER(C.c_uint32), C.c_uint32]),
     "cvp9_hip_export_read": (C.c_int, [C.c_void_p, C.POINTER(ExportBuffer), C.POINTER(C.c_uint8),
                                        C.c_uint64]),
     "cvp9_hip_dmabuf_read": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint8),
@@ -620,9 +621,10 @@ class Engine:
         check(lib().cvp9_engine_get_tune(self.e, C.byref(t)), "get_tune")
         return t
 
-    def set_tune(self, res_split, intra_lanes, intra_workers, intra_level_min=0, intra_defer=0):
+    def set_tune(self, res_split, intra_lanes, intra_workers, intra_level_min=0, intra_defer=0,
+                 intra_chains=0):
         t = Tune((C.c_uint8 * 4)(*res_split), intra_lanes, intra_workers, intra_level_min,
-                 intra_defer)
+                 intra_defer, intra_chains)
         check(lib().cvp9_engine_set_tune(self.e, C.byref(t)), "set_tune")
 
     def autotune(self, streams, preps, iters=3):

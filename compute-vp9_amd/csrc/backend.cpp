@@ -531,6 +531,22 @@ int64_t cvp9_hip_plan_image_compact(const vp9_parsed_frame_t *pf, int threads, u
     return (int64_t)L.host_bytes;
 }
 
+int cvp9_hip_intra_chains(const vp9_parsed_frame_t *pf, uint32_t *out, uint32_t cap)
+{
+    PrepLayout L;
+    L.chains = true;
+    cvp9_err_t err = plan_measure(pf, L);
+    if (err != CVP9_OK) return err;
+    std::vector<uint8_t> img(L.bytes);
+    plan_write(pf, L, img.data());
+    const uint32_t n = L.n_chains + 1 + L.n_intra;
+    if (out) {
+        if (cap < n) return CVP9_ERR_INVALID_DATA;
+        memcpy(out, img.data() + L.off_chain, (size_t)n * 4);
+    }
+    return (int)L.n_chains;
+}
+
 int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap)
 {
     PrepLayout L;

@@ -185,6 +185,11 @@ struct DJob {
     const uint32_t *cstart;
     const int16_t  *cvals;
     uint32_t        n_chunks;
+    /* chains of intra blocks for chain workers (n_chains > 0): n_chains + 1
+     * offsets, the chained intra-list entries chain by chain, then every
+     * other entry in list order */
+    const uint32_t *chain;
+    uint32_t        n_chains;
     uint32_t n_blocks, n_inter, n_intra, n_mc;
     uint32_t n_mc_a;               /* motion-compensation units [0, n_mc_a) run in the
                                       early launch, the rest in the late one */

@@ -271,9 +271,10 @@ void measure_range(const vp9_parsed_frame_t *pf, uint32_t W, uint32_t H, uint32_
 cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads)
 {
     if (!pf) return CVP9_ERR_INVALID_DATA;
-    const bool compact = L.compact;
+    const bool compact = L.compact, chains = L.chains;
     L = PrepLayout();
     L.compact = compact;
+    L.chains = chains;
     const uint32_t W = pf->hdr.width, H = pf->hdr.height;
     /* Sizes the kernels take. The CPU deblock reads one row (column) past
      * the plane when H (W) is 1 mod 8, and the chroma bilinear clamp
@@ -353,6 +354,10 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads
     L.off_blocks = off; off = align256(off + (uint64_t)L.n_blocks * sizeof(DBlock));
     L.off_mc_tab = off; off = align256(off + (uint64_t)L.n_mc * 16);
     L.off_intra = off;  off = align256(off + (uint64_t)L.n_intra * 4);
+    if (L.chains) {
+        L.off_chain = off;
+        off = align256(off + ((uint64_t)L.n_intra + kMaxChains + 1) * 4);
+    }
     for (int k = 0; k < 4; k++) {
         L.off_tx_tab[k] = off; off = align256(off + (uint64_t)L.cls_ntx[k] * 8);
     }
@@ -416,17 +421,118 @@ struct StampMap {
  * order: a cell without a mark of this pass when block b reads it is not
  * owned by an earlier intra block. g holds the intra blocks in list
  * order. */
+
+/* Chains over the intra blocks (g in list order; lev, hgt their levels and
+ * heights; key the sorted keys, whose low 32 bits are indices into g).
+ * Each block continues the chain of one of its parents, the one of highest
+ * level (then greatest height) whose chain has no next block yet; a parent
+ * owns a cell the block's modes read, and such a cell always lies on the
+ * parent's bottom row or right column, so only those cells carry a mark.
+ * The chains of at least two blocks, the most critical first (level plus
+ * height of the first block, then length), at most kMaxChains of them, are
+ * written to `out` as intra-list positions after n_chains + 1 offsets; every
+ * other position follows in list order. A worker that takes one chain in
+ * order, or a subsequence of the list, meets levels in non-decreasing
+ * order, which keeps the persistent stage free of deadlock. */
+static void chain_intra(const std::vector<IntraGeom> &g, const std::vector<uint32_t> &lev,
+                        const std::vector<uint32_t> &hgt, const std::vector<uint64_t> &key,
+                        size_t ow, size_t cells, uint32_t *out, uint32_t &n_chains,
+                        uint32_t &n_chained)
+{
+    constexpr uint32_t NONE = 0xffffffffu;
+    const uint32_t ni = (uint32_t)g.size();
+    thread_local std::vector<uint64_t> own; /* cell -> pass << 32 | (g index + 1) */
+    thread_local uint32_t pass = 0;
+    if (own.size() < cells) own.assign(cells, 0);
+    if (++pass == 0) {
+        std::fill(own.begin(), own.end(), 0ull);
+        pass = 1;
+    }
+    thread_local std::vector<uint32_t> prev, next, pos, heads;
+    prev.assign(ni, NONE);
+    next.assign(ni, NONE);
+    pos.resize(ni);
+    for (uint32_t i = 0; i < ni; i++) pos[(uint32_t)key[i]] = i;
+    for (uint32_t i = 0; i < ni; i++) {
+        const IntraGeom &k = g[i];
+        uint32_t best = NONE;
+        auto consider = [&](size_t c) {
+            const uint64_t v = own[c];
+            if ((uint32_t)(v >> 32) != pass) return;
+            const uint32_t o = (uint32_t)v - 1;
+            if (next[o] != NONE) return;
+            if (best == NONE || lev[o] > lev[best] || (lev[o] == lev[best] && hgt[o] > hgt[best]))
+                best = o;
+        };
+        if (k.cy0 > 0) {
+            const size_t row = (size_t)(k.cy0 - 1) * ow;
+            if (k.cx0 > 0 && (k.reads & CVP9_READS_CORNER)) consider(row + k.cx0 - 1);
+            if (k.reads & CVP9_READS_ABOVE)
+                for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) consider(row + cx);
+        }
+        if (k.cx0 > 0 && (k.reads & CVP9_READS_LEFT))
+            for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) consider((size_t)cy * ow + k.cx0 - 1);
+        if (best != NONE) {
+            prev[i] = best;
+            next[best] = i;
+        }
+        const uint64_t mark = (uint64_t)pass << 32 | (i + 1);
+        for (uint32_t cx = k.cx0; cx <= k.cx1; cx++) own[(size_t)k.cy1 * ow + cx] = mark;
+        for (uint32_t cy = k.cy0; cy <= k.cy1; cy++) own[(size_t)cy * ow + k.cx1] = mark;
+    }
+    heads.clear();
+    for (uint32_t i = 0; i < ni; i++)
+        if (prev[i] == NONE && next[i] != NONE) heads.push_back(i);
+    auto len = [&](uint32_t h) {
+        uint32_t n = 0;
+        for (uint32_t i = h; i != NONE; i = next[i]) n++;
+        return n;
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> rank;
+    rank.reserve(heads.size());
+    for (uint32_t h : heads) {
+        const uint64_t crit = (uint64_t)(lev[h] + hgt[h]) << 32 | len(h);
+        rank.emplace_back(~crit, h); /* ascending sort = most critical first */
+    }
+    std::sort(rank.begin(), rank.end());
+    n_chains = (uint32_t)std::min<size_t>(rank.size(), kMaxChains);
+    uint32_t *off = out, *ent = out + n_chains + 1;
+    thread_local std::vector<uint8_t> taken;
+    taken.assign(ni, 0);
+    uint32_t n = 0;
+    for (uint32_t c = 0; c < n_chains; c++) {
+        off[c] = n;
+        for (uint32_t i = rank[c].second; i != NONE; i = next[i]) {
+            ent[n++] = pos[i];
+            taken[pos[i]] = 1;
+        }
+    }
+    off[n_chains] = n;
+    n_chained = n;
+    for (uint32_t p = 0; p < ni; p++)
+        if (!taken[p]) ent[n++] = p;
+}
+
 static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32_t W, uint32_t H,
-                        std::vector<uint32_t> &levels, std::vector<uint32_t> &level_small)
+                        std::vector<uint32_t> &levels, std::vector<uint32_t> &level_small,
+                        uint32_t *chain, uint32_t &n_chains, uint32_t &n_chained)
 {
     levels.clear();
     level_small.clear();
+    n_chains = n_chained = 0;
     const uint32_t ni = (uint32_t)g.size();
-    if (ni == 0) return;
+    if (ni == 0) {
+        if (chain) chain[0] = 0;
+        return;
+    }
     if (ni == 1) {
         intra[0] = g[0].b;
         levels = {0, 1};
         level_small = {intra_small(g[0]) ? 1u : 0u};
+        if (chain) {
+            chain[0] = 0; /* no chains; the one entry */
+            chain[1] = 0;
+        }
         return;
     }
     const size_t ow = (W + 3) / 4, cells = ow * ((H + 3) / 4);
@@ -504,6 +610,7 @@ static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32
         levels[lev[at] + 1]++;
     }
     for (uint32_t l = 0; l < depth; l++) levels[l + 1] += levels[l];
+    if (chain) chain_intra(g, lev, hgt, key, ow, cells, chain, n_chains, n_chained);
 }
 
 namespace {
@@ -686,7 +793,8 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
                 x.b += at[c].n;
                 g.push_back(x);
             }
-        order_intra(g, (uint32_t *)(dst + L.off_intra), L.W, L.H, L.levels, L.level_small);
+        order_intra(g, (uint32_t *)(dst + L.off_intra), L.W, L.H, L.levels, L.level_small,
+                    L.chains ? (uint32_t *)(dst + L.off_chain) : nullptr, L.n_chains, L.n_chained);
     };
     if (P == 1) {
         write_range(pf, L, dst, L.chunks[0], at[0]);
@@ -927,6 +1035,10 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.err = errd_;
     j.err_host = err_dev_;
     j.ectr = ectr_;
+    if (L.chains && L.n_intra) {
+        j.chain = (const uint32_t *)(dev + L.off_chain);
+        j.n_chains = L.n_chains;
+    }
     if (L.compact && L.n_chunks) {
         j.cmask = (const uint64_t *)(dev + L.off_cmask);
         j.cstart = (const uint32_t *)(dev + L.off_cstart);
@@ -1029,6 +1141,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     n_.lf = pre[P_LF * stride + njobs];
     n_.nv12 = pre[P_NV12 * stride + njobs];
     n_.exp = pre[P_EXP * stride + njobs];
+    n_.chains = njobs == 1 ? jobs[0].n_chains : 0;
     n_.tune = tune;
     used_ = tk_off + bar_bytes;
 

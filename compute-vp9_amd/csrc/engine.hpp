@@ -68,6 +68,9 @@ struct PlanChunk {
     std::vector<IntraGeom> intra;
 };
 
+/* Chains of intra blocks a prepared frame lists at most (see PrepLayout). */
+constexpr uint32_t kMaxChains = 256;
+
 /* Byte layout of a prepared frame (host staging and device copy alike). */
 struct PrepLayout {
     uint32_t W = 0, H = 0;
@@ -101,6 +104,15 @@ struct PrepLayout {
      * the number of small blocks (at most 16x16, first in their level) */
     std::vector<uint32_t> levels;
     std::vector<uint32_t> level_small;
+    /* Chains of intra blocks (set chains before plan_measure): up to
+     * kMaxChains chains, each block continuing the chain of its
+     * highest-level parent, the most critical first (dependency level plus
+     * height of the chain's first block). At off_chain: n_chains + 1
+     * offsets into the chained entries, the chained entries (intra-list
+     * positions in chain order), then every other entry in list order. */
+    bool chains = false;
+    uint32_t n_chains = 0, n_chained = 0;
+    uint64_t off_chain = 0;
     /* the block list in ranges, one per planner thread (one range when the
      * frame is planned on the calling thread) */
     std::vector<PlanChunk> chunks;

@@ -418,6 +418,7 @@ cvp9_err_t cvp9_engine_prepare(cvp9_engine_t *e, const vp9_parsed_frame_t *pf, c
     *out = nullptr;
     cvp9_prep_t *p = new (std::nothrow) cvp9_prep();
     if (!p) return CVP9_ERR_NOMEM;
+    p->L.chains = true; /* chain workers may take it (resident inputs: planned once) */
     cvp9_err_t err = plan_measure(pf, p->L);
     if (err != CVP9_OK) {
         delete p;
@@ -849,6 +850,7 @@ cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t)
     t->intra_workers = e->tune.intra_workers;
     t->intra_level_min = e->tune.intra_level_min;
     t->intra_defer = e->tune.intra_defer;
+    t->intra_chains = e->tune.intra_chains;
     return CVP9_OK;
 }
 
@@ -860,12 +862,13 @@ cvp9_err_t cvp9_engine_set_tune(cvp9_engine_t *e, const cvp9_tune_t *t)
             return CVP9_ERR_INVALID_DATA;
     if (t->intra_lanes != 64 && t->intra_lanes != 256) return CVP9_ERR_INVALID_DATA;
     if (t->intra_workers < 1 || t->intra_workers > 65536) return CVP9_ERR_INVALID_DATA;
-    if (t->intra_defer > 1) return CVP9_ERR_INVALID_DATA;
+    if (t->intra_defer > 1 || t->intra_chains > 1) return CVP9_ERR_INVALID_DATA;
     for (int i = 0; i < 4; i++) e->tune.res_split[i] = t->res_split[i];
     e->tune.intra_lanes = t->intra_lanes;
     e->tune.intra_workers = t->intra_workers;
     e->tune.intra_level_min = t->intra_level_min;
     e->tune.intra_defer = t->intra_defer;
+    e->tune.intra_chains = t->intra_chains;
     return CVP9_OK;
 }
 
@@ -881,14 +884,15 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
     const uint32_t cu = (uint32_t)(e->cus > 0 ? e->cus : 256);
     /* each candidate run measures every stage at once; the transform split
      * is uniform over sizes inside a run and chosen per size afterwards */
-    struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; uint32_t level_min; uint32_t defer; };
+    struct Cand { uint8_t split; uint16_t lanes; uint32_t workers; uint32_t level_min; uint32_t defer;
+                  uint32_t chains = 0; };
     /* (the intra launch holds the worker count to what stays resident) */
     const Cand cands[] = {
         {1, 256, 4 * cu, 0, 0},      {2, 256, 4 * cu, 0, 0},      {4, 256, 3 * cu, 0, 0},
         {1, 64, 8 * cu, 0, 0},       {2, 64, 16 * cu, 0, 0},      {4, 64, 12 * cu, 0, 0},
         {1, 256, 2 * cu, 4 * cu, 0}, {2, 256, 4 * cu, 4 * cu, 0}, {4, 256, 4 * cu, 16 * cu, 0},
         {1, 64, 16 * cu, 4 * cu, 0}, {1, 256, 4 * cu, 0, 1},      {2, 64, 16 * cu, 0, 1},
-        {4, 64, 8 * cu, 0, 1},
+        {4, 64, 8 * cu, 0, 1},       {1, 64, 8 * cu, 0, 0, 1},   {2, 64, 16 * cu, 0, 0, 1},
     };
     const bool was_timing = e->timing;
     const Tune saved = e->tune;
@@ -902,6 +906,7 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
         e->tune.intra_workers = c.workers;
         e->tune.intra_level_min = c.level_min;
         e->tune.intra_defer = c.defer;
+        e->tune.intra_chains = c.chains;
         e->timing = false;
         if ((err = cvp9_engine_run(e, n, streams, preps, nullptr)) != CVP9_OK) break; /* warm */
         e->timing = true;
@@ -927,6 +932,7 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
                 out.intra_workers = c.workers;
                 out.intra_level_min = c.level_min;
                 out.intra_defer = c.defer;
+                out.intra_chains = c.chains;
             }
         }
     }
@@ -938,6 +944,7 @@ cvp9_err_t cvp9_engine_autotune(cvp9_engine_t *e, int n, const int *streams,
         best->intra_workers = out.intra_workers;
         best->intra_level_min = out.intra_level_min;
         best->intra_defer = out.intra_defer;
+        best->intra_chains = out.intra_chains;
     }
     return err;
 }

@@ -33,6 +33,8 @@ struct Tune {
                                  workers; 0: persistent workers only */
     uint32_t intra_defer;    /* 1: the ordered intra stage publishes edges only and a
                                 parallel pass (k_intra_px) writes the blocks' samples */
+    uint32_t intra_chains;   /* 1: a one-job batch whose frame lists chains of intra blocks
+                                runs them on chain workers (k_intra_chain) */
 };
 
 /* Total work items of each stage over the whole batch (the last element of
@@ -45,6 +47,7 @@ struct StageCounts {
     const uint32_t *level_n;  /* host arrays, per leading level over the batch: large and */
     const uint32_t *level_ns; /* small (at most 16x16) intra blocks */
     uint32_t intra_tail;      /* intra blocks left to the persistent workers */
+    uint32_t chains;          /* chains listed by a one-job batch's frame (0 otherwise) */
     Tune tune;
 };
 

@@ -1247,11 +1247,15 @@ __device__ __forceinline__ uint32_t job_epoch(const DJob &J)
         return J.epoch;
 }
 
-template <int W, bool EARLY, bool DEFER = false>
+template <int W, bool EARLY, bool DEFER = false, bool CHAIN = false>
 __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint32_t tk,
-                                            IntraShared &S, int tid, uint32_t epoch)
+                                            IntraShared &S, int tid, uint32_t epoch,
+                                            uint32_t prev_b = 0xffffffffu, uint32_t prev_word = 0,
+                                            uint32_t *my_word = nullptr, uint32_t *my_b = nullptr)
 {
     static_assert(EARLY || !DEFER, "a deferred block publishes its edges first");
+    static_assert(!CHAIN || (W == 64 && EARLY && !DEFER), "chain workers are single waves");
+    uint32_t myw = 0; /* CHAIN: edge word tid of this block (its 4 samples) */
     /* the job's fields, loaded once, together: after any store the
      * compiler would otherwise reload them (a store could alias the
      * descriptor as far as it knows), one memory round trip each */
@@ -1345,6 +1349,22 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         bool stale;
         uint64_t v = cell_gather(jedge, epoch, b, out, x, y, bh, kind, kind == 0 ? c : c - ncx, pend,
                                  inter, present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
+        if constexpr (CHAIN) {
+            /* cells of the block this wave finished just before (its chain
+             * predecessor): from the lanes holding that block's edge words
+             * (lane t word t), with no memory access and no wait */
+            const uint32_t side = (uint32_t)(cell >> 32) & 1, lw = (uint32_t)(cell >> 33) & 15,
+                           cc = (uint32_t)(cell >> 40) & 31;
+            const uint32_t ly = lane_read(4u * ((side ? 16u : 0u) + lw), prev_word);
+            const uint32_t lu = lane_read(4u * ((side ? 40u : 32u) + (cc >> 2)), prev_word);
+            const uint32_t lv = lane_read(4u * ((side ? 56u : 48u) + (cc >> 2)), prev_word);
+            if (pend && (uint32_t)cell == prev_b) {
+                const uint32_t sh = 8 * (cc & 3);
+                v = (uint64_t)ly | (uint64_t)((lu >> sh) & 0xffff) << 32 |
+                    (uint64_t)((lv >> sh) & 0xffff) << 48;
+                stale = false;
+            }
+        }
         /* the edge residuals are in registers before any wait: issued at
          * the block's start, they would otherwise be loaded where they are
          * used, a memory round trip after the last neighbour arrives */
@@ -1461,6 +1481,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                 __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + tid,
                                    (uint64_t)epoch << 32 | (word & w_msk), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
+            myw = word & w_msk;
         }
         if constexpr (DEFER) {
             /* the samples of the block are written later, by k_intra_px,
@@ -1547,8 +1568,13 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
                 const uint64_t d = (uint64_t)epoch << 32 | (n >= 4 ? q : q & ((1u << (8 * n)) - 1));
                 __hip_atomic_store(gm(jedge) + (size_t)b * CVP9_EDGE_WORDS + t, d, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
+                myw = (uint32_t)d;
             }
         }
+    }
+    if constexpr (CHAIN) {
+        *my_word = myw;
+        *my_b = b;
     }
 
     if constexpr (DEFER) return; /* k_intra_px writes the block */
@@ -1757,6 +1783,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     __shared__ IntraShared shm[256 / W];
     intra_worker<W, false, DEFER>(B, shm, blockIdx.x * (256 / W) + threadIdx.x / W,
                                   gridDim.x * (256 / W));
+}
+
+/* Chain workers (a batch of one job whose frame lists chains of intra
+ * blocks): worker w < K takes chain w in order, each block right after its
+ * chain predecessor on the same wave, which hands that predecessor's edge
+ * words over in registers (no memory round trip and no poll on the link);
+ * the other workers deal the remaining entries by stride, as the
+ * persistent stage does. A chain rises in level block by block and the
+ * remaining entries are in list order, so every worker meets levels in
+ * non-decreasing order and, with every worker resident, every wait ends. */
+__device__ __forceinline__ void intra_chain_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
+                                                   uint32_t nwk)
+{
+    const DJob &J = B.jobs[0];
+    IntraShared &S = shm[threadIdx.x / 64];
+    const int tid = threadIdx.x % 64;
+    const uint32_t epoch = J.epoch;
+    const GMEM uint32_t *ct = gm(J.chain);
+    const uint32_t K = J.n_chains;
+    const GMEM uint32_t *ent = ct + K + 1;
+    if (wk < K) {
+        const uint32_t a = __builtin_amdgcn_readfirstlane(ct[wk]);
+        const uint32_t z = __builtin_amdgcn_readfirstlane(ct[wk + 1]);
+        uint32_t prev_b = 0xffffffffu, word = 0;
+        for (uint32_t e = a; e < z; e++) {
+            const uint32_t tk = __builtin_amdgcn_readfirstlane(ent[e]);
+            uint32_t nw = 0, nb = 0;
+            intra_block<64, true, false, true>(B, J, tk, S, tid, epoch, prev_b, word, &nw, &nb);
+            word = nw;
+            prev_b = __builtin_amdgcn_readfirstlane(nb);
+            worker_sync<64>(); /* orders the LDS reuse of the next block */
+        }
+    } else {
+        const uint32_t r0 = __builtin_amdgcn_readfirstlane(ct[K]);
+        for (uint32_t e = r0 + (wk - K); e < J.n_intra; e += nwk - K) {
+            const uint32_t tk = __builtin_amdgcn_readfirstlane(ent[e]);
+            intra_block<64, true>(B, J, tk, S, tid, epoch);
+            worker_sync<64>();
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_intra_chain(DBatch B)
+{
+    __shared__ IntraShared shm[4];
+    intra_chain_worker(B, shm, blockIdx.x * 4 + threadIdx.x / 64, gridDim.x * 4);
 }
 
 /* Leading dependency levels: one launch per level, one worker per intra
@@ -2284,6 +2356,24 @@ static uint32_t intra_resident_wgs()
     return per ? per : 1u;
 }
 
+static uint32_t chain_resident_wgs()
+{
+    static uint32_t cache[16] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    if (!cache[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intra_chain, 256, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per_cu < 1 || cus < 1)
+            return 128;
+        cache[dev] = (uint32_t)(per_cu * cus);
+    }
+    const int live = cvp9_live_engines();
+    const uint32_t per = cache[dev] / (2u * (uint32_t)(live > 1 ? live : 1));
+    return per ? per : 1u;
+}
+
 uint32_t cvp9_fused_max_wgs(int device)
 {
     int per_cu = 0, cus = 0;
@@ -2378,6 +2468,17 @@ Tune cvp9_default_tune(int cus)
 template <bool DEFER>
 static hipError_t launch_intra(const DBatch &b, const StageCounts &n, hipStream_t s)
 {
+    /* chain workers: a one-job batch that lists chains, no level launches,
+     * more workers than chains */
+    if (!DEFER && n.tune.intra_chains && n.chains && n.levels == 0 && n.intra_tail) {
+        const uint32_t cap = chain_resident_wgs();
+        uint32_t g = div_up(n.tune.intra_workers, 4);
+        if (g > cap) g = cap;
+        if (g * 4 > n.chains) {
+            hipLaunchKernelGGL(k_intra_chain, dim3(g), dim3(256), 0, s, b);
+            return hipGetLastError();
+        }
+    }
     for (uint32_t l = 0; l < n.levels; l++) {
         const uint32_t sw = div_up(n.level_ns[l], 4);
         hipLaunchKernelGGL(k_intra_level<DEFER>, dim3(sw + n.level_n[l]), dim3(256), 0, s, b, l, sw);

@@ -94,7 +94,13 @@ cvp9_err_t cvp9_engine_read(cvp9_engine_t *e, int stream_id, uint8_t *dst, uint6
  * instead of going to the persistent workers (0: persistent workers only;
  * at most 64 leading levels), and whether the ordered intra stage only
  * publishes each block's edge samples and leaves the blocks' samples to a
- * parallel pass after it (intra_defer 1) or writes every block itself (0).
+ * parallel pass after it (intra_defer 1) or writes every block itself (0),
+ * and whether a batch of one frame that lists chains of intra blocks (frames
+ * prepared by cvp9_engine_prepare do) runs its most critical chains each on
+ * a worker of its own, which takes a block right after its chain
+ * predecessor and gets that predecessor's edge samples in registers
+ * (intra_chains 1; 64-thread workers, no deferred samples, no level
+ * launches).
  * Output does not depend on it; cvp9_engine_autotune picks it for a
  * workload class. */
 typedef struct {
@@ -103,6 +109,7 @@ typedef struct {
     uint32_t intra_workers;
     uint32_t intra_level_min;
     uint32_t intra_defer;
+    uint32_t intra_chains;
 } cvp9_tune_t;
 cvp9_err_t cvp9_engine_get_tune(cvp9_engine_t *e, cvp9_tune_t *t);
 cvp9_err_t cvp9_engine_set_tune(cvp9_engine_t *e, const cvp9_tune_t *t);

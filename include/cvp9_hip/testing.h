@@ -55,6 +55,14 @@ int64_t cvp9_hip_plan_image_compact(const vp9_parsed_frame_t *pf, int threads, u
  * cvp9_err_t. Needs no device. */
 int cvp9_hip_intra_order(const vp9_parsed_frame_t *pf, uint32_t *order, uint32_t cap);
 
+/* The chains of intra blocks a prepared frame lists for chain workers:
+ * returns the number of chains K (or a negative cvp9_err_t) and with out !=
+ * NULL writes K + 1 offsets, then the chained entries (positions in the
+ * order above, chain by chain, each chain in dependency order), then every
+ * other position in list order: K + 1 + (number of intra blocks) words, at
+ * most cap. Needs no device. */
+int cvp9_hip_intra_chains(const vp9_parsed_frame_t *pf, uint32_t *out, uint32_t cap);
+
 /* Dependency levels of pf's intra blocks (the order above groups them by
  * level): returns the number of levels, and with starts != NULL writes the
  * order position where each level starts followed by the number of intra

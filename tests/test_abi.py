@@ -365,3 +365,49 @@ def test_intra_levels_match_dependencies(pkg, preset, w, h, shuffle):
             got = order[starts[L]:starts[L + 1]].tolist()
             want = by_class(b, w, h, [blk for blk, v in lev.items() if v == L], hgt)
             assert got == want, f"frame {idx} level {L}"
+
+
+# ---- chains of intra blocks (chain workers) ----
+
+@pytest.mark.parametrize("preset,w,h,shuffle", [(1, 320, 184, False), (2, 643, 362, False),
+                                                (1, 1920, 1080, False), (1, 264, 136, True),
+                                                (3, 854, 480, False)])
+def test_intra_chains(pkg, preset, w, h, shuffle):
+    """Each listed chain follows dependencies (every block depends on the one
+    before it, so levels rise along it), the chains are the most critical
+    ones (level + height of the first block, then length), at most 256, and
+    every other entry follows in list order: a chain worker and the workers
+    dealing the rest each meet levels in non-decreasing order."""
+    rng = np.random.default_rng(9)
+    p = pkg.synth_params(preset, w, h, 44)
+    for idx in range(1, 3):
+        f = pkg.SynthFrame(p, idx)
+        b = f.blocks()
+        if shuffle:
+            keep = rng.random(b.size) >= 0.1
+            b = b[keep][rng.permutation(int(keep.sum()))]
+            f = pkg.OwnedFrame(w, h, b, f.coeffs(), frame_type=f.pf.hdr.frame_type)
+        deps = intra_dependencies(b, w, h)
+        lev, hgt = brute_levels(deps), brute_heights(deps)
+        n = len(deps)
+        order = np.zeros(max(n, 1), np.uint32)
+        assert pkg.lib().cvp9_hip_intra_order(f.ptr, order.ctypes.data_as(pkg.C.POINTER(pkg.C.c_uint32)),
+                                              order.size) == n
+        buf = np.zeros(256 + 1 + max(n, 1), np.uint32)
+        K = pkg.lib().cvp9_hip_intra_chains(f.ptr, buf.ctypes.data_as(pkg.C.POINTER(pkg.C.c_uint32)),
+                                            buf.size)
+        assert 0 <= K <= 256
+        off = buf[:K + 1].tolist()
+        ent = buf[K + 1:K + 1 + n].tolist()
+        assert sorted(ent) == list(range(n)), "not a permutation of the entries"
+        assert off[0] == 0 and all(off[c] < off[c + 1] - 1 for c in range(K)), "chains of 2+ blocks"
+        rest = ent[off[K]:]
+        assert rest == sorted(rest)
+        crit = []
+        for c in range(K):
+            blks = [int(order[e]) for e in ent[off[c]:off[c + 1]]]
+            for a, z in zip(blks, blks[1:]):
+                assert a in deps[z], f"chain {c}: {z} does not depend on {a}"
+                assert lev[a] < lev[z]
+            crit.append((lev[blks[0]] + hgt[blks[0]], len(blks)))
+        assert crit == sorted(crit, reverse=True)

@@ -201,3 +201,33 @@ def test_config1_fused_1080p(pkg, orc, gpu_ready):
             eng.sync()
             got = eng.read(sid, W, H)
             assert np.array_equal(got, want[i]), f"frame {i}: {first_diff(got, want[i], W, H)}"
+
+
+@pytest.mark.parametrize("lanes,workers", [(64, 1024), (64, 300)])
+def test_config1_chain_workers_1080p(pkg, orc, gpu_ready, lanes, workers):
+    """1920x1080 preset 1 with chain workers (the 256 most critical chains
+    of intra blocks each on a wave of its own, the rest dealt by stride),
+    frames pipelined back to back: a KEY frame and four inter frames, each
+    bit-exact; with 300 workers only 44 deal the rest."""
+    W, H = 1920, 1080
+    seq = gen_all(pkg, [(1, W, H, 2031, 5)])[0]
+    want = orc.run_oracle(seq)
+    print(f"[case] config 1: 1080p chain workers, {workers} workers", flush=True)
+    with pkg.Engine(0) as eng:
+        eng.set_tune([1, 1, 4, 4], lanes, workers, 0, 0, 1)
+        sid = eng.open_stream()
+        preps = [eng.prepare(f) for f in seq]
+        for i in range(len(seq)):
+            eng.run([sid], [preps[i]])
+        eng.sync()
+        assert eng.stream_error(sid) == 0
+        got = eng.read(sid, W, H)
+        assert np.array_equal(got, want[-1]), first_diff(got, want[-1], W, H)
+    with pkg.Engine(0) as eng:
+        eng.set_tune([1, 1, 4, 4], lanes, workers, 0, 0, 1)
+        sid = eng.open_stream()
+        for i, f in enumerate(seq):
+            eng.run([sid], [eng.prepare(f)])
+            eng.sync()
+            got = eng.read(sid, W, H)
+            assert np.array_equal(got, want[i]), f"frame {i}: {first_diff(got, want[i], W, H)}"

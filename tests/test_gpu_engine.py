@@ -42,29 +42,32 @@ def test_engine_timing_counts(pkg, gpu_ready):
 
 
 # (transform split, intra lanes, persistent workers, width from which a
-# leading intra level gets its own launch, deferred intra samples): 1
-# launches every level up to 64 deep (the KEY frame's tail beyond goes to
-# the persistent workers); 4 and 12 stop at the first narrower level, mixing
-# both within one frame
-TUNES = [([1, 1, 1, 1], 256, 64, 0, 0), ([2, 2, 2, 2], 64, 512, 0, 0),
-         ([4, 4, 4, 4], 64, 37, 0, 0), ([1, 2, 4, 4], 256, 5, 0, 0),
-         ([4, 1, 2, 1], 64, 2048, 0, 0), ([1, 1, 2, 4], 64, 64, 1, 0),
-         ([2, 2, 2, 2], 256, 64, 1, 0), ([1, 1, 1, 1], 64, 512, 4, 0),
-         ([1, 1, 1, 1], 256, 3, 12, 0), ([2, 2, 2, 2], 64, 512, 0, 1),
-         ([1, 2, 4, 4], 256, 5, 0, 1), ([1, 1, 2, 4], 64, 64, 1, 1),
-         ([1, 1, 1, 1], 256, 3, 12, 1)]
+# leading intra level gets its own launch, deferred intra samples, chain
+# workers): 1 launches every level up to 64 deep (the KEY frame's tail beyond
+# goes to the persistent workers); 4 and 12 stop at the first narrower level,
+# mixing both within one frame; chain workers with plenty of workers, with
+# few left over for the other entries, and with fewer workers than chains
+# (then the plain persistent form runs)
+TUNES = [([1, 1, 1, 1], 256, 64, 0, 0, 0), ([2, 2, 2, 2], 64, 512, 0, 0, 0),
+         ([4, 4, 4, 4], 64, 37, 0, 0, 0), ([1, 2, 4, 4], 256, 5, 0, 0, 0),
+         ([4, 1, 2, 1], 64, 2048, 0, 0, 0), ([1, 1, 2, 4], 64, 64, 1, 0, 0),
+         ([2, 2, 2, 2], 256, 64, 1, 0, 0), ([1, 1, 1, 1], 64, 512, 4, 0, 0),
+         ([1, 1, 1, 1], 256, 3, 12, 0, 0), ([2, 2, 2, 2], 64, 512, 0, 1, 0),
+         ([1, 2, 4, 4], 256, 5, 0, 1, 0), ([1, 1, 2, 4], 64, 64, 1, 1, 0),
+         ([1, 1, 1, 1], 256, 3, 12, 1, 0), ([1, 1, 2, 4], 64, 2048, 0, 0, 1),
+         ([2, 2, 2, 2], 64, 40, 0, 0, 1), ([4, 1, 2, 4], 64, 4, 0, 0, 1)]
 
 
-@pytest.mark.parametrize("split,lanes,workers,level_min,defer", TUNES)
+@pytest.mark.parametrize("split,lanes,workers,level_min,defer,chains", TUNES)
 def test_every_launch_shape_bit_exact(pkg, orc, gpu_ready, split, lanes, workers, level_min,
-                                      defer):
+                                      defer, chains):
     """Launch shapes (autotuner knobs) never change the output."""
     frames = [pkg.SynthFrame(pkg.synth_params(2, 264, 200, 900 + lanes), i) for i in range(3)]
     want = orc.run_oracle(frames)
     print(f"[case] tune split={split} lanes={lanes} workers={workers} level_min={level_min} "
-          f"defer={defer}", flush=True)
+          f"defer={defer} chains={chains}", flush=True)
     with pkg.Engine(0) as eng:
-        eng.set_tune(split, lanes, workers, level_min, defer)
+        eng.set_tune(split, lanes, workers, level_min, defer, chains)
         sid = eng.open_stream()
         for i, f in enumerate(frames):
             eng.run([sid], [eng.prepare(f)])

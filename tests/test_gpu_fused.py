@@ -31,6 +31,8 @@ def set_mode(eng, mode):
         eng.set_tune([1, 1, 2, 4], 64, 512, 1, 1)
     elif mode == "plain_defer":
         eng.set_tune([1, 1, 2, 4], 64, 512, 0, 1)
+    elif mode == "chains":
+        eng.set_tune([1, 1, 2, 4], 64, 512, 0, 0, 1)
 
 
 def run_engine(pkg, frames, mode, w, h):
@@ -55,7 +57,8 @@ def test_fused_bit_exact(pkg, orc, gpu_ready, preset, w, h):
         assert np.array_equal(g, x), f"frame {i}"
 
 
-@pytest.mark.parametrize("mode", ["plain", "fused", "levels", "plain_defer", "levels_defer"])
+@pytest.mark.parametrize("mode", ["plain", "fused", "levels", "plain_defer", "levels_defer",
+                                  "chains"])
 def test_shuffled_blocks_with_holes(pkg, orc, gpu_ready, mode):
     rng = np.random.default_rng(11)
     w, h = 200, 136
