@@ -1142,6 +1142,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     n_.nv12 = pre[P_NV12 * stride + njobs];
     n_.exp = pre[P_EXP * stride + njobs];
     n_.chains = njobs == 1 ? jobs[0].n_chains : 0;
+    n_.chained = njobs == 1 && jobs[0].n_chains ? layouts[0]->n_chained : 0;
     n_.tune = tune;
     used_ = tk_off + bar_bytes;
 

@@ -48,6 +48,7 @@ struct StageCounts {
     const uint32_t *level_ns; /* small (at most 16x16) intra blocks */
     uint32_t intra_tail;      /* intra blocks left to the persistent workers */
     uint32_t chains;          /* chains listed by a one-job batch's frame (0 otherwise) */
+    uint32_t chained;         /* and the intra blocks on them */
     Tune tune;
 };
 

@@ -1127,11 +1127,19 @@ __device__ __forceinline__ int intra_pred(int mode, int a, int l, int tl, int dc
  * one memory round trip per kind of cell instead of one in all. Block
  * positions are multiples of 4, so each sample lies in the dword named
  * below; rows are 64-byte aligned, so the dword lies inside the row pitch. */
-__device__ __forceinline__ uint64_t cell_gather(const uint64_t *edge, uint32_t epoch, uint32_t self,
-                                                const DFrame &F, int x, int y, int bh, int kind,
-                                                int q, bool pend, bool inter, uint64_t d,
-                                                bool &stale)
+/* The loads of cell_gather, and the selection that follows them: split so
+ * that a chain worker can issue the loads of its next block before it
+ * finishes the current one. */
+struct CellRaw {
+    uint64_t wy, wu, wv;             /* edge words (Y, U, V) */
+    uint32_t fy[4], fu[2], fv[2];    /* frame dwords */
+};
+
+__device__ __forceinline__ CellRaw cell_load(const uint64_t *edge, uint32_t self, const DFrame &F,
+                                             int x, int y, int bh, int kind, int q, bool pend,
+                                             bool inter, uint64_t d)
 {
+    CellRaw r;
     /* edge words (a thread without an edge cell reads the block's own
      * record, and discards it) */
     const uint64_t ed = pend ? d : (uint64_t)self;
@@ -1139,12 +1147,11 @@ __device__ __forceinline__ uint64_t cell_gather(const uint64_t *edge, uint32_t e
     const uint32_t side = (uint32_t)(ed >> 32) & 1, lw = (uint32_t)(ed >> 33) & 15;
     const uint32_t cc = (uint32_t)(ed >> 40) & 31;
     const GMEM uint64_t *rec = gm(edge) + (size_t)o * CVP9_EDGE_WORDS;
-    const uint64_t wy = __hip_atomic_load(rec + (side ? 16 : 0) + lw, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t wu = __hip_atomic_load(rec + (side ? 40 : 32) + (cc >> 2), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t wv = __hip_atomic_load(rec + (side ? 56 : 48) + (cc >> 2), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+    r.wy = __hip_atomic_load(rec + (side ? 16 : 0) + lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r.wu = __hip_atomic_load(rec + (side ? 40 : 32) + (cc >> 2), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    r.wv = __hip_atomic_load(rec + (side ? 56 : 48) + (cc >> 2), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
 
     /* frame dwords, four of luma and two of each chroma plane, and the byte
      * each contributes (shift, and whether it is one of the cell's samples):
@@ -1159,42 +1166,61 @@ __device__ __forceinline__ uint64_t cell_gather(const uint64_t *edge, uint32_t e
     const int cbx = x >> 1, cby = y >> 1, cbh = (bh + 1) >> 1;
     const int colY = kind == 0 ? x + 4 * q : x - 4;
     const int cC = kind == 0 ? cbx + 2 * q : cbx - 1;
-    const int colC = cC & ~3, shC = 8 * (cC & 3);
-    uint32_t fy[4], fu[2], fv[2];
+    const int colC = cC & ~3;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        const int r = kind == 1 ? y + min(4 * q + e, bh - 1) : y - 1;
-        const size_t o = inter ? (size_t)r * F.sy + colY : 0;
-        fy[e] = *(const GMEM uint32_t *)(gm(F.y) + o);
+        const int rr = kind == 1 ? y + min(4 * q + e, bh - 1) : y - 1;
+        const size_t off = inter ? (size_t)rr * F.sy + colY : 0;
+        r.fy[e] = *(const GMEM uint32_t *)(gm(F.y) + off);
     }
 #pragma unroll
     for (int e = 0; e < 2; e++) {
-        const int r = kind == 1 ? cby + min(2 * q + e, cbh - 1) : cby - 1;
-        const size_t o = inter ? (size_t)r * F.suv + colC : 0;
-        fu[e] = *(const GMEM uint32_t *)(gm(F.u) + o);
-        fv[e] = *(const GMEM uint32_t *)(gm(F.v) + o);
+        const int rr = kind == 1 ? cby + min(2 * q + e, cbh - 1) : cby - 1;
+        const size_t off = inter ? (size_t)rr * F.suv + colC : 0;
+        r.fu[e] = *(const GMEM uint32_t *)(gm(F.u) + off);
+        r.fv[e] = *(const GMEM uint32_t *)(gm(F.v) + off);
     }
+    return r;
+}
+
+__device__ __forceinline__ uint64_t cell_value(const CellRaw &r, uint32_t epoch, int x, int bh,
+                                               int kind, int q, bool pend, bool inter, uint64_t d,
+                                               bool &stale)
+{
+    const int cbx = x >> 1, cbh = (bh + 1) >> 1;
+    const int cC = kind == 0 ? cbx + 2 * q : cbx - 1;
+    const int shC = 8 * (cC & 3);
     uint64_t v_inter = 0;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         const uint32_t sh = kind == 0 ? 8 * e : 24;
         const bool keep = kind == 0 || (kind == 1 ? 4 * q + e < bh : e == 3);
-        v_inter |= (uint64_t)(keep ? (fy[e] >> sh) & 0xff : 0u) << (8 * e);
+        v_inter |= (uint64_t)(keep ? (r.fy[e] >> sh) & 0xff : 0u) << (8 * e);
     }
 #pragma unroll
     for (int e = 0; e < 2; e++) {
         const uint32_t sh = kind == 0 ? shC + 8 * e : shC;
         const bool keep = kind == 0 || (kind == 1 ? 2 * q + e < cbh : e == 1);
-        v_inter |= (uint64_t)(keep ? (fu[e] >> sh) & 0xff : 0u) << (32 + 8 * e);
-        v_inter |= (uint64_t)(keep ? (fv[e] >> sh) & 0xff : 0u) << (48 + 8 * e);
+        v_inter |= (uint64_t)(keep ? (r.fu[e] >> sh) & 0xff : 0u) << (32 + 8 * e);
+        v_inter |= (uint64_t)(keep ? (r.fv[e] >> sh) & 0xff : 0u) << (48 + 8 * e);
     }
 
+    const uint32_t cc = (uint32_t)((pend ? d : 0) >> 40) & 31;
     const uint32_t sh = 8 * (cc & 3);
     const uint64_t v_edge =
-        (wy & 0xffffffffull) | ((wu >> sh) & 0xffff) << 32 | ((wv >> sh) & 0xffff) << 48;
-    stale = pend && ((uint32_t)(wy >> 32) != epoch || (uint32_t)(wu >> 32) != epoch ||
-                     (uint32_t)(wv >> 32) != epoch);
+        (r.wy & 0xffffffffull) | ((r.wu >> sh) & 0xffff) << 32 | ((r.wv >> sh) & 0xffff) << 48;
+    stale = pend && ((uint32_t)(r.wy >> 32) != epoch || (uint32_t)(r.wu >> 32) != epoch ||
+                     (uint32_t)(r.wv >> 32) != epoch);
     return pend ? v_edge : (inter ? v_inter : d);
+}
+
+__device__ __forceinline__ uint64_t cell_gather(const uint64_t *edge, uint32_t epoch, uint32_t self,
+                                                const DFrame &F, int x, int y, int bh, int kind,
+                                                int q, bool pend, bool inter, uint64_t d,
+                                                bool &stale)
+{
+    const CellRaw r = cell_load(edge, self, F, x, y, bh, kind, q, pend, inter, d);
+    return cell_value(r, epoch, x, bh, kind, q, pend, inter, d, stale);
 }
 
 /* Synchronises the threads of one intra worker: a whole workgroup, or a
@@ -1247,14 +1273,132 @@ __device__ __forceinline__ uint32_t job_epoch(const DJob &J)
         return J.epoch;
 }
 
-template <int W, bool EARLY, bool DEFER = false, bool CHAIN = false>
-__device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint32_t tk,
+/* An intra block's neighbour record as one lane holds it: its cell
+ * (threads past the cells read the last one, and never use it), the
+ * dependency masks, the block's local index and its block record. */
+struct IntraRec {
+    uint64_t cell, dmask, imask;
+    uint64_t kw[3];
+    uint32_t b;
+};
+
+/* The job fields the intra stage reads, held in registers by a chain
+ * worker for its whole chain (through the descriptor, every store would make
+ * the compiler load them again: a store could alias it as far as it knows). */
+struct JobView {
+    DFrame out;
+    int16_t *resid;
+    uint64_t *edge, *nbr;
+    uint32_t *err, *err_host;
+};
+
+__device__ __forceinline__ JobView job_view(const DJob &J)
+{
+    JobView v;
+    v.out = J.out;
+    v.resid = J.resid;
+    v.edge = J.edge;
+    v.nbr = J.nbr;
+    v.err = J.err;
+    v.err_host = J.err_host;
+    return v;
+}
+
+template <typename JT>
+__device__ __forceinline__ IntraRec load_intra_rec(const JT &J, uint32_t tk, int tid)
+{
+    const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
+    IntraRec R;
+    R.cell = rec[min(tid, CVP9_NB_CELLS - 1)];
+    R.dmask = rec[CVP9_NB_MASK];
+    R.imask = rec[CVP9_NB_IMASK];
+    R.b = (uint32_t)rec[CVP9_NB_INDEX];
+    R.kw[0] = rec[CVP9_NB_BLOCK];
+    R.kw[1] = rec[CVP9_NB_BLOCK + 1];
+    R.kw[2] = rec[CVP9_NB_BLOCK + 2];
+    return R;
+}
+
+/* Where this lane's cell of a block lies and what it is (thread c of the
+ * gathering wave): kind 0 above, cell q, 1 left, cell q, 2 above-left;
+ * pend: an earlier intra block's (edge record), inter: an inter block's
+ * (the frame), else the resolved value d. */
+struct CellPos {
+    int kind, q;
+    bool pend, inter;
+    uint64_t d;
+};
+
+__device__ __forceinline__ CellPos cell_pos(const IntraRec &R, const DBlock &k, int c)
+{
+    const int x = k.x, y = k.y, ncx = (k.bw + 3) >> 2, ncy = (k.bh + 3) >> 2;
+    CellPos p;
+    p.kind = c < ncx ? 0 : (c < ncx + ncy ? 1 : 2);
+    p.q = p.kind == 0 ? c : c - ncx;
+    const bool present = p.kind == 0 ? y > 0 : (p.kind == 1 ? x > 0 : (x > 0 && y > 0));
+    const bool incell = c <= ncx + ncy && present;
+    p.pend = incell && ((R.dmask >> c) & 1);
+    p.inter = incell && !p.pend && ((R.imask >> c) & 1);
+    p.d = present ? R.cell : 0x7f7f7f7f7f7f7f7full;
+    return p;
+}
+
+/* The luma residual of edge sample i of a block (bottom row, then right
+ * column; 0 where the block has none), read at a clamped place when there
+ * is none so that the load needs no branch. */
+__device__ __forceinline__ int edge_resid(const GMEM int16_t *jres, const DBlock &k, int i)
+{
+    const int bw = k.bw, bh = k.bh, lt = k.tx + 2, T = 1 << lt;
+    const bool res = !(k.flags & DB_SKIP);
+    const int rrows = res ? (bh >> lt) << lt : 0, rcols = res ? (int)k.nx << lt : 0;
+    const int er = i < bw ? bh - 1 : i - bw, ec = i < bw ? i : bw - 1;
+    const bool ok = i < bw + bh && er < rrows && ec < rcols;
+    const GMEM int16_t *p = ok ? jres + k.coeff + (size_t)(((er >> lt) * (int)k.nx + (ec >> lt)) << (2 * lt)) +
+                                     ((er & (T - 1)) << lt) + (ec & (T - 1))
+                               : jres;
+    return ok ? (int)p[0] : 0;
+}
+
+/* What a chain worker loads for its next block before it finishes the
+ * current one: the record, this lane's two luma edge residuals and its
+ * cell's loads (the cells of the current block, the next one's chain
+ * predecessor, are taken from registers instead). */
+struct BlockPre {
+    IntraRec R;
+    int eres0, eres1;
+    CellRaw raw;
+};
+
+template <typename JT>
+__device__ __forceinline__ BlockPre issue_block(const JT &J, const IntraRec &R, int tid)
+{
+    BlockPre P;
+    P.R = R;
+    DBlock k;
+    memcpy(&k, R.kw, sizeof(k));
+    const GMEM int16_t *jres = gm(J.resid);
+    P.eres0 = edge_resid(jres, k, tid);
+    P.eres1 = edge_resid(jres, k, tid + 64);
+    const CellPos cp = cell_pos(R, k, tid);
+    P.raw = cell_load(J.edge, R.b, J.out, k.x, k.y, k.bh, cp.kind, cp.q, cp.pend, cp.inter, cp.d);
+    return P;
+}
+
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+template <int W, bool EARLY, bool DEFER = false, bool CHAIN = false, typename JT = DJob,
+          typename HOOK = NoHook>
+__device__ __forceinline__ void intra_block(const DBatch &B, const JT &J, uint32_t tk,
                                             IntraShared &S, int tid, uint32_t epoch,
+                                            const BlockPre *pre = nullptr,
                                             uint32_t prev_b = 0xffffffffu, uint32_t prev_word = 0,
-                                            uint32_t *my_word = nullptr, uint32_t *my_b = nullptr)
+                                            uint32_t *my_word = nullptr, uint32_t *my_b = nullptr,
+                                            const HOOK &after_gather = HOOK())
 {
     static_assert(EARLY || !DEFER, "a deferred block publishes its edges first");
-    static_assert(!CHAIN || (W == 64 && EARLY && !DEFER), "chain workers are single waves");
+    static_assert(!CHAIN || (W == 64 && EARLY), "chain workers are single waves");
     uint32_t myw = 0; /* CHAIN: edge word tid of this block (its 4 samples) */
     /* the job's fields, loaded once, together: after any store the
      * compiler would otherwise reload them (a store could alias the
@@ -1264,19 +1408,16 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     uint64_t *const jedge = J.edge;
     uint32_t *const jerr = J.err, *const jerr_host = J.err_host;
     /* block index, block record, dependency mask and this thread's cell
-     * all come from the neighbour record, in one round of loads */
+     * all come from the neighbour record, in one round of loads (a chain
+     * worker loaded it one block ahead) */
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
-    /* (threads past the cells read the last one, and never use it) */
-    const uint64_t cell = rec[min(tid, CVP9_NB_CELLS - 1)];
-    const uint64_t dmask = rec[CVP9_NB_MASK];
-    const uint64_t imask = rec[CVP9_NB_IMASK];
-    const uint32_t b = (uint32_t)rec[CVP9_NB_INDEX];
+    IntraRec R;
+    if constexpr (CHAIN) R = pre->R;
+    else R = load_intra_rec(J, tk, tid);
+    const uint64_t cell = R.cell, dmask = R.dmask, imask = R.imask;
+    const uint32_t b = R.b;
     DBlock k;
-    {
-        const uint64_t k0 = rec[CVP9_NB_BLOCK], k1 = rec[CVP9_NB_BLOCK + 1], k2 = rec[CVP9_NB_BLOCK + 2];
-        uint64_t kw[3] = {k0, k1, k2};
-        memcpy(&k, kw, sizeof(k));
-    }
+    memcpy(&k, R.kw, sizeof(k));
     const int x = k.x, y = k.y, bw = k.bw, bh = k.bh;
     const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
 
@@ -1313,7 +1454,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const int er = tid < bw ? bh - 1 : tid - bw, ec = tid < bw ? tid : bw - 1;
     const bool eres_ok = EARLY && tid < bw + bh && er < rrows && ec < rcols;
     const int eres0 =
-        !EARLY ? 0
+        CHAIN ? pre->eres0 : !EARLY ? 0
                : (int)(eres_ok ? jres + k.coeff +
                                      (size_t)(((er >> lt) * (int)k.nx + (ec >> lt)) << (2 * lt)) +
                                      ((er & (T - 1)) << lt) + (ec & (T - 1))
@@ -1324,7 +1465,7 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
     const int er1 = i1 < bw ? bh - 1 : i1 - bw, ec1 = i1 < bw ? i1 : bw - 1;
     const bool eres1_ok = EARLY && W == 64 && i1 < bw + bh && er1 < rrows && ec1 < rcols;
     const int eres1 =
-        !(EARLY && W == 64) ? 0
+        CHAIN ? pre->eres1 : !(EARLY && W == 64) ? 0
                : (int)(eres1_ok ? jres + k.coeff +
                                       (size_t)(((er1 >> lt) * (int)k.nx + (ec1 >> lt)) << (2 * lt)) +
                                       ((er1 & (T - 1)) << lt) + (ec1 & (T - 1))
@@ -1347,8 +1488,13 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         const bool pend = incell && ((dmask >> c) & 1);
         const bool inter = incell && !pend && ((imask >> c) & 1);
         bool stale;
-        uint64_t v = cell_gather(jedge, epoch, b, out, x, y, bh, kind, kind == 0 ? c : c - ncx, pend,
-                                 inter, present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
+        uint64_t v;
+        if constexpr (CHAIN)
+            v = cell_value(pre->raw, epoch, x, bh, kind, kind == 0 ? c : c - ncx, pend, inter,
+                           present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
+        else
+            v = cell_gather(jedge, epoch, b, out, x, y, bh, kind, kind == 0 ? c : c - ncx, pend,
+                            inter, present ? cell : 0x7f7f7f7f7f7f7f7full, stale);
         if constexpr (CHAIN) {
             /* cells of the block this wave finished just before (its chain
              * predecessor): from the lanes holding that block's edge words
@@ -1442,6 +1588,10 @@ __device__ __forceinline__ void intra_block(const DBatch &B, const DJob &J, uint
         /* a neighbour still running: poll the cell's words until it has
          * published them */
         if (stale) v = cell_from_edges(jedge, epoch, jerr, jerr_host, cell, B.spin_limit);
+        /* (a chain worker issues its next block's loads here: the loads it
+         * waits for above were issued a block earlier, and what it loads
+         * now is in flight while this block is computed and published) */
+        after_gather();
 
         /* the sums, reduced across the wave (only for a DC mode: the other
          * predictors do not read them) */
@@ -1789,8 +1939,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
  * blocks): worker w < K takes chain w in order, each block right after its
  * chain predecessor on the same wave, which hands that predecessor's edge
  * words over in registers (no memory round trip and no poll on the link);
- * the other workers deal the remaining entries by stride, as the
- * persistent stage does. A chain rises in level block by block and the
+ * a chain block's samples are left to k_intra_px (deferred form), so the
+ * next link does not wait for them. The other workers deal the remaining
+ * entries by stride and write their blocks, as the persistent stage does. A chain rises in level block by block and the
  * remaining entries are in list order, so every worker meets levels in
  * non-decreasing order and, with every worker resident, every wait ends. */
 __device__ __forceinline__ void intra_chain_worker(const DBatch &B, IntraShared *shm, uint32_t wk,
@@ -1806,13 +1957,49 @@ __device__ __forceinline__ void intra_chain_worker(const DBatch &B, IntraShared 
     if (wk < K) {
         const uint32_t a = __builtin_amdgcn_readfirstlane(ct[wk]);
         const uint32_t z = __builtin_amdgcn_readfirstlane(ct[wk + 1]);
+        /* the job's fields in registers for the whole chain, and the
+         * chain's entries one per lane (64 at a time) */
+        const JobView V = job_view(J);
+        const uint32_t lane = __lane_id();
+        uint32_t base = a; /* ents holds entries [base, base + 64) */
+        uint32_t ents = ent[min(base + lane, z - 1)];
+        auto entry = [&](uint32_t e) -> uint32_t {
+            return __builtin_amdgcn_readlane(ents, (int)((min(e, z - 1) - base) & 63));
+        };
+        /* software pipeline: block e's loads (record, edge residuals, cell
+         * loads) were issued while block e - 1 ran, and block e + 1's
+         * record while block e - 1 ran too, so a link waits only for the
+         * predecessor's arithmetic (its edge words arrive in registers).
+         * The loads past the chain's end repeat its last entry: no branch
+         * around a load. */
+        uint32_t tk = entry(a);
+        BlockPre P = issue_block(V, load_intra_rec(V, tk, tid), tid);
+        IntraRec Rn = load_intra_rec(V, entry(a + 1), tid);
         uint32_t prev_b = 0xffffffffu, word = 0;
         for (uint32_t e = a; e < z; e++) {
-            const uint32_t tk = __builtin_amdgcn_readfirstlane(ent[e]);
+            const uint32_t tkn = entry(e + 1);
+            uint32_t tkn2;
+            if (e + 2 >= base + 64 && e + 2 < z) {
+                /* a chain longer than 64: its next 64 entries (a uniform
+                 * branch, taken once per 64 blocks) */
+                ents = ent[min(e + 2 + lane, z - 1)];
+                base = e + 2;
+                tkn2 = __builtin_amdgcn_readlane(ents, 0);
+            } else {
+                tkn2 = entry(e + 2);
+            }
+            BlockPre Pn;
+            auto next_loads = [&]() {
+                Pn = issue_block(V, Rn, tid);
+                Rn = load_intra_rec(V, tkn2, tid);
+            };
             uint32_t nw = 0, nb = 0;
-            intra_block<64, true, false, true>(B, J, tk, S, tid, epoch, prev_b, word, &nw, &nb);
+            intra_block<64, true, true, true>(B, V, tk, S, tid, epoch, &P, prev_b, word, &nw, &nb,
+                                              next_loads);
             word = nw;
             prev_b = __builtin_amdgcn_readfirstlane(nb);
+            P = Pn;
+            tk = tkn;
             worker_sync<64>(); /* orders the LDS reuse of the next block */
         }
     } else {
@@ -1825,7 +2012,7 @@ __device__ __forceinline__ void intra_chain_worker(const DBatch &B, IntraShared 
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_intra_chain(DBatch B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_intra_chain(DBatch B)
 {
     __shared__ IntraShared shm[4];
     intra_chain_worker(B, shm, blockIdx.x * 4 + threadIdx.x / 64, gridDim.x * 4);
@@ -1871,11 +2058,8 @@ __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level, u
  * this pass but deblocking. A sample's above and left neighbours come from
  * the lanes holding their cells (ds_bpermute, with the whole wave active);
  * the above-left samples and DC values are wave-uniform. */
-__device__ __forceinline__ void intra_px_item(const DBatch &B, uint32_t w)
+__device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
 {
-    const uint32_t j = find_job_wave(B.pre_intra, B.njobs, w);
-    const DJob &J = B.jobs[j];
-    const uint32_t tk = w - B.pre_intra[j];
     const DFrame out = J.out;
     const GMEM int16_t *const jres = gm(J.resid);
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
@@ -1970,7 +2154,19 @@ __global__ __launch_bounds__(256) void k_intra_px(DBatch B)
 {
     const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= B.pre_intra[B.njobs]) return; /* uniform per wave */
-    intra_px_item(B, w);
+    const uint32_t j = find_job_wave(B.pre_intra, B.njobs, w);
+    intra_px_entry(B.jobs[j], w - B.pre_intra[j]);
+}
+
+/* The same for the chained entries of a one-job batch (chain workers leave
+ * their blocks' samples to it): wave w takes chained entry w. */
+__global__ __launch_bounds__(256) void k_intra_px_chain(DBatch B, uint32_t n)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= n) return; /* uniform per wave */
+    const DJob &J = B.jobs[0];
+    const uint32_t tk = __builtin_amdgcn_readfirstlane(gm(J.chain)[J.n_chains + 1 + w]);
+    intra_px_entry(J, tk);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2476,6 +2672,9 @@ static hipError_t launch_intra(const DBatch &b, const StageCounts &n, hipStream_
         if (g > cap) g = cap;
         if (g * 4 > n.chains) {
             hipLaunchKernelGGL(k_intra_chain, dim3(g), dim3(256), 0, s, b);
+            if (n.chained)
+                hipLaunchKernelGGL(k_intra_px_chain, dim3(div_up(n.chained, 4)), dim3(256), 0, s, b,
+                                   n.chained);
             return hipGetLastError();
         }
     }
