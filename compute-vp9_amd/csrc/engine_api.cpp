@@ -31,6 +31,12 @@ constexpr int kBatchSlots = 4;
  * fence, a marker skips its cache write-back and invalidate, which cost
  * 2.5 us of the gap between consecutive frames on one 1080p stream. */
 constexpr unsigned kSyncEvent = hipEventDisableTiming | hipEventDisableSystemFence;
+/* 1: a batch whose stream set differs from its queue's previous batch still
+ * overlaps the other queue (see run_batch); the library is built with 0, the
+ * schedule model of the tests builds both */
+#ifndef CVP9_OVERLAP_REGROUPED
+#define CVP9_OVERLAP_REGROUPED 0
+#endif
 constexpr uint32_t kSpinLimit = 1u << 21;
 
 struct BatchSlot {
@@ -545,7 +551,9 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
      * queues) runs after the other queue's work instead: the overlapped
      * form of that schedule stalled once on MI355X with no cause found. */
     const uint64_t sig = stream_set_signature(streams, n);
-    if (!single && !other && e->qsig[q] && e->qsig[q] != sig && e->queue_busy(1 - q)) other = true;
+    if (!CVP9_OVERLAP_REGROUPED && !single && !other && e->qsig[q] && e->qsig[q] != sig &&
+        e->queue_busy(1 - q))
+        other = true;
     if (other && (he = e->order(1 - q, q)) != hipSuccess) return to_err(he);
     const hipStream_t qs = e->queue(q);
     const bool resid_all = e->pipeline && !e->fused;
