@@ -54,7 +54,8 @@ def kernel_of(stage, tune):
     names it (templates on threads per intra worker, transform size and
     waves per transform group)."""
     if stage == "intra" and tune:
-        return f"k_intra<{tune['intra_lanes']}>"
+        defer = "true" if tune.get("intra_defer") else "false"
+        return f"k_intra<{tune['intra_lanes']}, {defer}>"
     if stage.startswith("residual") and tune:
         n = int(stage[8:])
         return f"k_residual<{n}, {tune['res_split'][(4, 8, 16, 32).index(n)]}>"

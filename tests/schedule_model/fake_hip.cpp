@@ -404,6 +404,11 @@ hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned int)
     return hipSuccess;
 }
 
+hipError_t hipExtStreamCreateWithCUMask(hipStream_t *s, uint32_t, const uint32_t *)
+{
+    return hipStreamCreateWithFlags(s, 0);
+}
+
 hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 
 hipError_t hipStreamSynchronize(hipStream_t s)
