@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <new>
 #include <vector>
 
@@ -513,6 +514,32 @@ cvp9_err_t engine_run_staged(cvp9_engine_t *e, int sid, const PrepLayout &L, con
 } // namespace cvp9
 
 namespace {
+/* The second phase of a one-job batch follows its first phase on another
+ * queue. A queue wait on an event that has already finished when the wait
+ * is queued costs that queue ~0.3 us; one still pending then costs it
+ * 2.4-4.4 us at the moment the queue reaches it, which for one stream is on
+ * the chain between frames. So while the queue still has the previous
+ * batch's work ahead, the host polls the first phase's end (a fraction of a
+ * microsecond per query) and queues the second phase with no wait once it
+ * has finished. It gives up (and queues the wait) when the previous batch
+ * finishes first, when the queue has nothing ahead (a host-bound caller
+ * never spins), or after kHostWaitUs. */
+constexpr double kHostWaitUs = 2000.0;
+
+bool phase_a_done(hipEvent_t aev, hipEvent_t prev)
+{
+    if (!prev || hipEventQuery(prev) != hipErrorNotReady) return false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t r = hipEventQuery(aev);
+        if (r == hipSuccess) return true;
+        if (r != hipErrorNotReady || hipEventQuery(prev) != hipErrorNotReady) return false;
+        if (std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() >
+            kHostWaitUs)
+            return false;
+    }
+}
+
 cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *const *preps,
                      uint8_t *const *nv12, const Upload *up)
 {
@@ -709,6 +736,7 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
         return hipSuccess;
     };
     hipEvent_t done = bs ? bs->done : nullptr; /* cleared once a kernel records it */
+    const hipEvent_t prev_on_q = e->batch_event(e->qserial[q]); /* the queue's previous batch */
     if (!e->fused) {
         if (!resid_all) {
             he = run_stages(kOrderPlain, (int)(sizeof(kOrderPlain) / sizeof(int)), qs, false, &done);
@@ -719,7 +747,8 @@ cvp9_err_t run_batch(cvp9_engine_t *e, int n, const int *streams, cvp9_prep_t *c
             he = run_stages(kOrderA, (int)(sizeof(kOrderA) / sizeof(int)), qa, false, nullptr);
             if (he == hipSuccess && overlap) {
                 he = hipEventRecord(e->aev[q], qa);
-                if (he == hipSuccess) he = hipStreamWaitEvent(qs, e->aev[q], 0);
+                if (he == hipSuccess && !(n == 1 && phase_a_done(e->aev[q], prev_on_q)))
+                    he = hipStreamWaitEvent(qs, e->aev[q], 0);
             }
             if (he == hipSuccess)
                 he = run_stages(kOrderB, (int)(sizeof(kOrderB) / sizeof(int)), qs, true, &done);

@@ -12,8 +12,10 @@ can run at once never need more than the device holds resident.
 Schedules: two to four stream groups alternating over the two queues (the
 four-group form is the one that once stalled on the GPU), groups that change
 every step, one batch and several batches alternating, level launches, frame
-pipelining on and off, one queue, and a switch to the fused kernel; under
-hipEventQuery answers that are always busy, always done, or pseudo-random.
+pipelining on and off, one queue, one-job batches (whose second phase the host
+queues without a wait once it has seen the first finish), and a switch to the
+fused kernel; under hipEventQuery answers that are always busy, always done, or
+pseudo-random.
 The engine as built serialises a regrouped batch behind the other queue's
 work; the "overlap" build lets it overlap, which is checked as well.
 
@@ -59,6 +61,8 @@ SCHEDULES = [
     (8, 4, 6, "alt", 0, 2, 0, -1),
     (8, 2, 6, "alt", 1, 1, 0, -1),
     (8, 2, 6, "alt", 1, 2, 0, 3),
+    (1, 1, 8, "alt", 1, 2, 0, -1),   # one stream: one-job batches, host-side wait for phase A
+    (2, 2, 8, "alt", 1, 2, 0, -1),   # one stream per queue
 ]
 
 
