@@ -2087,10 +2087,14 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
     const int nq = (bw + 3) >> 2, nitems = nq * bh;
     const int ymode = k.ymode, tlY = (int)(tlo >> 24), dcY = (int)(dcw & 0xff);
     uint8_t *const Y = out.y + (size_t)y * out.sy + x;
+    /* a quad's 4 residuals are one 8-byte word when the block's coefficients
+     * start on a multiple of 4 (its transforms' offsets then all do) */
+    const bool ral = (k.coeff & 3) == 0;
     for (int i0 = 0; i0 < nitems; i0 += 256) {
         int16_t rq[4][4];
         bool has[4];
         int rr[4], cc[4];
+        const GMEM int16_t *rs[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int i = min(i0 + u * 64 + (int)lane, nitems - 1);
@@ -2099,12 +2103,25 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
             cc[u] = c0;
             has[u] = r < rrows && c0 < rcols;
             /* (a quad with residual lies inside one transform row) */
-            const GMEM int16_t *rs =
-                has[u] ? jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
-                             ((r & (T - 1)) << lt) + (c0 & (T - 1))
-                       : jres;
+            rs[u] = has[u] ? jres + k.coeff + (size_t)(((r >> lt) * (int)k.nx + (c0 >> lt)) << (2 * lt)) +
+                                 ((r & (T - 1)) << lt) + (c0 & (T - 1))
+                           : jres;
+        }
+        /* all four items' loads inside one branch (a branch's loads are
+         * waited for where it ends) */
+        if (ral) {
+            uint64_t w[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) rq[u][e] = rs[has[u] ? e : 0];
+            for (int u = 0; u < 4; u++) w[u] = *(const GMEM uint64_t *)rs[u];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) rq[u][e] = (int16_t)(uint16_t)(w[u] >> (16 * e));
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) rq[u][e] = rs[u][has[u] ? e : 0];
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
