@@ -1044,9 +1044,9 @@ __global__ __launch_bounds__(256) void k_nbr_mc(DBatch B, uint32_t nbr_wgs, uint
 /* The samples of an intra cell in the resolved layout (bytes 0-3 luma,
  * 4-5 U, 6-7 V), read from the owner's edge record: its three words (Y, U,
  * V) are loaded together and re-read, bounded, until each carries this
- * frame's epoch. The poll backs off (64, 512, then 2048 clocks between
- * reads), so thousands of waiting lanes do not crowd the memory system
- * that the blocks they wait for are using. */
+ * frame's epoch. The poll sleeps 64 clocks between reads: a ping-pong probe
+ * found thousands of other workgroups polling add only 0.1-0.2 us to a
+ * hand-off, and a longer sleep adds its length to every link of the chain. */
 __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32_t epoch, uint32_t *err,
                                                     uint32_t *err_host, uint64_t d, uint32_t spin_limit)
 {
@@ -1065,8 +1065,7 @@ __device__ __forceinline__ uint64_t cell_from_edges(const uint64_t *edge, uint32
         const bool sy = (uint32_t)(wy >> 32) != epoch, su = (uint32_t)(wu >> 32) != epoch,
                    sv = (uint32_t)(wv >> 32) != epoch;
         if (!(sy || su || sv)) break;
-        if (spins < 8) __builtin_amdgcn_s_sleep(1); /* 64 clocks */
-        else __builtin_amdgcn_s_sleep(4);           /* 256 clocks */
+        __builtin_amdgcn_s_sleep(1); /* 64 clocks */
         if (++spins > spin_limit) {
             flag_error(err, err_host, 0);
             break;
