@@ -190,6 +190,10 @@ struct DJob {
      * other entry in list order */
     const uint32_t *chain;
     uint32_t        n_chains;
+    /* intra-list entries for the deferred sample pass: the n_px_small blocks
+     * of at most 8x8 first, then the others */
+    const uint32_t *px;
+    uint32_t        n_px_small;
     uint32_t n_blocks, n_inter, n_intra, n_mc;
     uint32_t n_mc_a;               /* motion-compensation units [0, n_mc_a) run in the
                                       early launch, the rest in the late one */
@@ -216,6 +220,8 @@ struct DBatch {
     const uint32_t *pre_nv12;      /* NV12 pack: 64x64 luma tiles */
     const uint32_t *pre_fill;      /* 128-fill: 64 KiB chunks */
     const uint32_t *pre_exp;       /* compact coefficients: waves of 4 chunks */
+    const uint32_t *pre_px;        /* deferred intra samples: waves (four blocks of at most
+                                      8x8 to a wave, one larger block) */
     uint32_t       *jticket;       /* per-job intra ticket counters (zeroed by the descriptor
                                       copy), for jobs no persistent worker starts on */
     const uint32_t *jstart;        /* per job, the intra-list entry where the persistent

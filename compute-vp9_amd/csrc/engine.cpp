@@ -354,6 +354,7 @@ cvp9_err_t plan_measure(const vp9_parsed_frame_t *pf, PrepLayout &L, int threads
     L.off_blocks = off; off = align256(off + (uint64_t)L.n_blocks * sizeof(DBlock));
     L.off_mc_tab = off; off = align256(off + (uint64_t)L.n_mc * 16);
     L.off_intra = off;  off = align256(off + (uint64_t)L.n_intra * 4);
+    L.off_px = off;     off = align256(off + (uint64_t)L.n_intra * 4);
     if (L.chains) {
         L.off_chain = off;
         off = align256(off + ((uint64_t)L.n_intra + kMaxChains + 1) * 4);
@@ -513,20 +514,44 @@ static void chain_intra(const std::vector<IntraGeom> &g, const std::vector<uint3
         if (!taken[p]) ent[n++] = p;
 }
 
+/* Entries of the deferred sample pass: blocks of at most 8x8 (2x2 cells)
+ * first, each group in list order. */
+static uint32_t px_order(const std::vector<IntraGeom> &g, const uint32_t *entry_geom, uint32_t *px)
+{
+    const uint32_t ni = (uint32_t)g.size();
+    uint32_t ns = 0;
+    for (uint32_t i = 0; i < ni; i++) {
+        const IntraGeom &q = g[entry_geom[i]];
+        if (q.cx1 - q.cx0 < 2 && q.cy1 - q.cy0 < 2) px[ns++] = i;
+    }
+    uint32_t n = ns;
+    for (uint32_t i = 0; i < ni; i++) {
+        const IntraGeom &q = g[entry_geom[i]];
+        if (!(q.cx1 - q.cx0 < 2 && q.cy1 - q.cy0 < 2)) px[n++] = i;
+    }
+    return ns;
+}
+
 static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32_t W, uint32_t H,
                         std::vector<uint32_t> &levels, std::vector<uint32_t> &level_small,
-                        uint32_t *chain, uint32_t &n_chains, uint32_t &n_chained)
+                        uint32_t *chain, uint32_t &n_chains, uint32_t &n_chained,
+                        uint32_t *px = nullptr, uint32_t *n_px_small = nullptr)
 {
     levels.clear();
     level_small.clear();
     n_chains = n_chained = 0;
     const uint32_t ni = (uint32_t)g.size();
+    if (n_px_small) *n_px_small = 0;
     if (ni == 0) {
         if (chain) chain[0] = 0;
         return;
     }
     if (ni == 1) {
         intra[0] = g[0].b;
+        if (px) {
+            const uint32_t at0 = 0;
+            *n_px_small = px_order(g, &at0, px);
+        }
         levels = {0, 1};
         level_small = {intra_small(g[0]) ? 1u : 0u};
         if (chain) {
@@ -608,6 +633,12 @@ static void order_intra(const std::vector<IntraGeom> &g, uint32_t *intra, uint32
         const uint32_t at = (uint32_t)key[i];
         intra[i] = g[at].b;
         levels[lev[at] + 1]++;
+    }
+    if (px) {
+        thread_local std::vector<uint32_t> at_of;
+        at_of.resize(ni);
+        for (uint32_t i = 0; i < ni; i++) at_of[i] = (uint32_t)key[i];
+        *n_px_small = px_order(g, at_of.data(), px);
     }
     for (uint32_t l = 0; l < depth; l++) levels[l + 1] += levels[l];
     if (chain) chain_intra(g, lev, hgt, key, ow, cells, chain, n_chains, n_chained);
@@ -794,7 +825,8 @@ void plan_write(const vp9_parsed_frame_t *pf, PrepLayout &L, uint8_t *dst)
                 g.push_back(x);
             }
         order_intra(g, (uint32_t *)(dst + L.off_intra), L.W, L.H, L.levels, L.level_small,
-                    L.chains ? (uint32_t *)(dst + L.off_chain) : nullptr, L.n_chains, L.n_chained);
+                    L.chains ? (uint32_t *)(dst + L.off_chain) : nullptr, L.n_chains, L.n_chained,
+                    (uint32_t *)(dst + L.off_px), &L.n_px_small);
     };
     if (P == 1) {
         write_range(pf, L, dst, L.chunks[0], at[0]);
@@ -1021,6 +1053,8 @@ DJob StreamState::make_job(const PrepLayout &L, const uint8_t *dev, FrameBuf *ou
     j.coeffs = (const int16_t *)(dev + L.off_coeffs);
     j.mc_tab = (const uint4 *)(dev + L.off_mc_tab);
     j.intra = (const uint32_t *)(dev + L.off_intra);
+    j.px = (const uint32_t *)(dev + L.off_px);
+    j.n_px_small = L.n_px_small;
     for (int k = 0; k < 4; k++) {
         j.tx_tab[k] = (const uint32_t *)(dev + L.off_tx_tab[k]);
         j.cls_ntx[k] = L.cls_ntx[k];
@@ -1073,7 +1107,7 @@ size_t StreamState::resident_bytes() const
 
 size_t BatchBuilder::bytes_for(uint32_t njobs)
 {
-    return align256((uint64_t)njobs * sizeof(DJob)) + align256(14ull * (njobs + 1) * 4) + CVP9_BAR_BYTES +
+    return align256((uint64_t)njobs * sizeof(DJob)) + align256(15ull * (njobs + 1) * 4) + CVP9_BAR_BYTES +
            3ull * CVP9_MAX_HEAD_LEVELS * (njobs + 1) * 4;
 }
 
@@ -1084,8 +1118,8 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     const uint64_t pre_off = align256((uint64_t)njobs * sizeof(DJob));
     uint32_t *pre = (uint32_t *)(host + pre_off);
     const uint32_t stride = njobs + 1;
-    enum { P_FILL, P_BLK, P_INTER, P_INTER2, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_EXP, P_N };
-    static_assert(P_N + 2 <= 14, "prefix rows, the ticket row and the start row fit the descriptor");
+    enum { P_FILL, P_BLK, P_INTER, P_INTER2, P_G0, P_G1, P_G2, P_G3, P_INTRA, P_LF, P_NV12, P_EXP, P_PX, P_N };
+    static_assert(P_N + 2 <= 15, "prefix rows, the ticket row and the start row fit the descriptor");
     for (int a = 0; a < P_N; a++) pre[a * stride] = 0;
 
     for (uint32_t i = 0; i < njobs; i++) {
@@ -1108,6 +1142,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
         c[P_LF] = J.lf_thr > 0 ? J.lf_wx * J.lf_wy : 0;
         c[P_NV12] = J.nv12 ? div_up(J.out.w, 64) * div_up(J.out.h, 64) : 0;
         c[P_EXP] = div_up(J.n_chunks, 4u);
+        c[P_PX] = div_up(J.n_px_small, 4u) + (J.n_intra - J.n_px_small);
         for (int a = 0; a < P_N; a++) pre[a * stride + i + 1] = pre[a * stride + i] + c[a];
     }
     const uint32_t *dpre = (const uint32_t *)(dev + pre_off);
@@ -1122,9 +1157,10 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     b_.pre_lf = dpre + P_LF * stride;
     b_.pre_nv12 = dpre + P_NV12 * stride;
     b_.pre_exp = dpre + P_EXP * stride;
+    b_.pre_px = dpre + P_PX * stride;
     b_.jticket = (uint32_t *)(dev + pre_off) + P_N * stride;
     b_.jstart = (const uint32_t *)(dev + pre_off) + (P_N + 1) * stride;
-    const uint64_t tk_off = pre_off + align256(14ull * stride * 4);
+    const uint64_t tk_off = pre_off + align256(15ull * stride * 4);
     b_.bar = (uint32_t *)(dev + tk_off);
     /* the fused kernel's barrier words start at zero */
     const size_t bar_bytes = fused ? CVP9_BAR_BYTES : 256;
@@ -1141,6 +1177,7 @@ void BatchBuilder::build(const DJob *jobs, const PrepLayout *const *layouts, uin
     n_.lf = pre[P_LF * stride + njobs];
     n_.nv12 = pre[P_NV12 * stride + njobs];
     n_.exp = pre[P_EXP * stride + njobs];
+    n_.px = pre[P_PX * stride + njobs];
     n_.chains = njobs == 1 ? jobs[0].n_chains : 0;
     n_.chained = njobs == 1 && jobs[0].n_chains ? layouts[0]->n_chained : 0;
     n_.tune = tune;

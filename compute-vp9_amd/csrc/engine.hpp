@@ -85,6 +85,8 @@ struct PrepLayout {
                                        the unit table, the ones the previous frame (unless
                                        KEY) leaves alone */
     uint64_t off_blocks = 0, off_intra = 0, off_mc_tab = 0;
+    uint64_t off_px = 0;            /* intra-list entries, blocks of at most 8x8 first */
+    uint32_t n_px_small = 0;
     uint64_t off_tx_tab[4] = {0, 0, 0, 0};
     uint64_t off_coeffs = 0, bytes = 0;
     /* Compact coefficients (set compact before plan_measure): the parser's

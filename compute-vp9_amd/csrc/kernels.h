@@ -49,6 +49,7 @@ struct StageCounts {
     uint32_t intra_tail;      /* intra blocks left to the persistent workers */
     uint32_t chains;          /* chains listed by a one-job batch's frame (0 otherwise) */
     uint32_t chained;         /* and the intra blocks on them */
+    uint32_t px;              /* waves of the deferred sample pass (k_intra_px) */
     Tune tune;
 };
 

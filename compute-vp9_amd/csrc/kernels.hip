@@ -2057,12 +2057,16 @@ __global__ __launch_bounds__(256) void k_intra_level(DBatch B, uint32_t level, u
  * this pass but deblocking. A sample's above and left neighbours come from
  * the lanes holding their cells (ds_bpermute, with the whole wave active);
  * the above-left samples and DC values are wave-uniform. */
-__device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
+/* Intra-list entry tk of job J on L lanes: L = 64 (a wave, any block) or
+ * 16 (blocks of at most 8x8: four of them to a wave); the L lanes start at
+ * lane gb of the wave and hold the block's neighbour cells. */
+template <int L>
+__device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk, uint32_t gb = 0)
 {
     const DFrame out = J.out;
     const GMEM int16_t *const jres = gm(J.resid);
     const GMEM uint64_t *rec = gm(J.nbr) + (size_t)tk * CVP9_NB_WORDS;
-    const uint32_t lane = __lane_id();
+    const uint32_t lane = __lane_id() - gb; /* within the group */
     const uint64_t cell = rec[min(lane, (uint32_t)CVP9_NB_CELLS - 1)];
     const uint32_t dcw = (uint32_t)rec[CVP9_NB_MASK];
     DBlock k;
@@ -2074,9 +2078,15 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
     const int cbx = x >> 1, cby = y >> 1, cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
     const int ncx = (bw + 3) >> 2, ncy = (bh + 3) >> 2;
     const uint32_t lo = (uint32_t)cell, hi = (uint32_t)(cell >> 32);
-    const int tc = __builtin_amdgcn_readfirstlane(ncx + ncy);
-    const uint32_t tlo = (uint32_t)__builtin_amdgcn_readlane((int)lo, tc);
-    const uint32_t thi = (uint32_t)__builtin_amdgcn_readlane((int)hi, tc);
+    uint32_t tlo, thi;
+    if constexpr (L == 64) {
+        const int tc = __builtin_amdgcn_readfirstlane(ncx + ncy);
+        tlo = (uint32_t)__builtin_amdgcn_readlane((int)lo, tc);
+        thi = (uint32_t)__builtin_amdgcn_readlane((int)hi, tc);
+    } else {
+        tlo = lane_read(4u * (gb + (uint32_t)(ncx + ncy)), lo);
+        thi = lane_read(4u * (gb + (uint32_t)(ncx + ncy)), hi);
+    }
 
     /* luma: 4 samples per item, four items' residuals loaded before any
      * is used; lanes past the block compute and store nothing useful */
@@ -2089,14 +2099,14 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
     /* a quad's 4 residuals are one 8-byte word when the block's coefficients
      * start on a multiple of 4 (its transforms' offsets then all do) */
     const bool ral = (k.coeff & 3) == 0;
-    for (int i0 = 0; i0 < nitems; i0 += 256) {
+    for (int i0 = 0; i0 < nitems; i0 += 4 * L) {
         int16_t rq[4][4];
         bool has[4];
         int rr[4], cc[4];
         const GMEM int16_t *rs[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int i = min(i0 + u * 64 + (int)lane, nitems - 1);
+            const int i = min(i0 + u * L + (int)lane, nitems - 1);
             const int r = (int)udiv_small((uint32_t)i, (uint32_t)nq), c0 = (i - r * nq) * 4;
             rr[u] = r;
             cc[u] = c0;
@@ -2125,8 +2135,8 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int r = rr[u], c0 = cc[u];
-            const uint32_t aw = lane_read(4u * (uint32_t)(c0 >> 2), lo);
-            const uint32_t lw = lane_read(4u * (uint32_t)(ncx + (r >> 2)), lo);
+            const uint32_t aw = lane_read(4u * (gb + (uint32_t)(c0 >> 2)), lo);
+            const uint32_t lw = lane_read(4u * (gb + (uint32_t)(ncx + (r >> 2))), lo);
             const int l = (int)((lw >> (8 * (r & 3))) & 0xff);
             uint32_t word = 0;
 #pragma unroll
@@ -2135,22 +2145,22 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
                 if (has[u]) v = clampi(v + rq[u][e], 0, 255);
                 word |= (uint32_t)v << (8 * e);
             }
-            if (i0 + u * 64 + (int)lane < nitems) store_quad(Y + (size_t)r * out.sy + c0, word, bw - c0);
+            if (i0 + u * L + (int)lane < nitems) store_quad(Y + (size_t)r * out.sy + c0, word, bw - c0);
         }
     }
 
     /* chroma: U items then V items, 4 samples each */
     const int nqc = (cbw + 3) >> 2, nci = nqc * cbh;
     const int uvmode = k.uvmode;
-    for (int i0 = 0; i0 < 2 * nci; i0 += 64) {
+    for (int i0 = 0; i0 < 2 * nci; i0 += L) {
         const int i = min(i0 + (int)lane, 2 * nci - 1);
         const bool pu = i < nci;
         const int q = pu ? i : i - nci;
         const int r = (int)udiv_small((uint32_t)q, (uint32_t)nqc), c0 = (q - r * nqc) * 4;
         const uint32_t sh = pu ? 0u : 16u;
-        const uint32_t a0 = lane_read(4u * (uint32_t)(c0 >> 1), hi);
-        const uint32_t a1 = lane_read(4u * (uint32_t)min((c0 >> 1) + 1, 63), hi);
-        const uint32_t lw = lane_read(4u * (uint32_t)(ncx + (r >> 1)), hi);
+        const uint32_t a0 = lane_read(4u * (gb + (uint32_t)(c0 >> 1)), hi);
+        const uint32_t a1 = lane_read(4u * (gb + (uint32_t)min((c0 >> 1) + 1, L - 1)), hi);
+        const uint32_t lw = lane_read(4u * (gb + (uint32_t)(ncx + (r >> 1))), hi);
         const int l = (int)((lw >> (sh + 8 * (r & 1))) & 0xff);
         const int tl = pu ? (int)((thi >> 8) & 0xff) : (int)(thi >> 24);
         const int dc = pu ? (int)((dcw >> 8) & 0xff) : (int)((dcw >> 16) & 0xff);
@@ -2166,12 +2176,22 @@ __device__ __forceinline__ void intra_px_entry(const DJob &J, uint32_t tk)
     }
 }
 
+/* Wave w of the batch's sample pass: a job's first ceil(n_px_small / 4)
+ * waves take its blocks of at most 8x8 four at a time (16 lanes each, the
+ * entries J.px lists first), its other waves one entry each. */
 __global__ __launch_bounds__(256) void k_intra_px(DBatch B)
 {
     const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (w >= B.pre_intra[B.njobs]) return; /* uniform per wave */
-    const uint32_t j = find_job_wave(B.pre_intra, B.njobs, w);
-    intra_px_entry(B.jobs[j], w - B.pre_intra[j]);
+    if (w >= B.pre_px[B.njobs]) return; /* uniform per wave */
+    const uint32_t j = find_job_wave(B.pre_px, B.njobs, w);
+    const DJob &J = B.jobs[j];
+    const uint32_t lw = w - B.pre_px[j], ns = J.n_px_small, G = (ns + 3) / 4;
+    if (lw < G) {
+        const uint32_t sub = __lane_id() >> 4, e = 4 * lw + sub;
+        if (e < ns) intra_px_entry<16>(J, gm(J.px)[e], 16 * sub);
+    } else {
+        intra_px_entry<64>(J, __builtin_amdgcn_readfirstlane(gm(J.px)[ns + lw - G]));
+    }
 }
 
 /* The same for the chained entries of a one-job batch (chain workers leave
@@ -2182,7 +2202,7 @@ __global__ __launch_bounds__(256) void k_intra_px_chain(DBatch B, uint32_t n)
     if (w >= n) return; /* uniform per wave */
     const DJob &J = B.jobs[0];
     const uint32_t tk = __builtin_amdgcn_readfirstlane(gm(J.chain)[J.n_chains + 1 + w]);
-    intra_px_entry(J, tk);
+    intra_px_entry<64>(J, tk);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2708,7 +2728,7 @@ static hipError_t launch_intra(const DBatch &b, const StageCounts &n, hipStream_
             hipLaunchKernelGGL((k_intra<256, DEFER>), dim3(w < cap ? w : cap), dim3(256), 0, s, b);
         }
     }
-    if (DEFER && n.intra) hipLaunchKernelGGL(k_intra_px, dim3(div_up(n.intra, 4)), dim3(256), 0, s, b);
+    if (DEFER && n.px) hipLaunchKernelGGL(k_intra_px, dim3(div_up(n.px, 4)), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -180,6 +180,8 @@ cvp9_err_t cvp9_engine_run_static(const cvp9_prep_t *prep, const cvp9_frame_buf_
     j.blocks = (const DBlock *)(prep->dev + L.off_blocks);
     j.coeffs = (const int16_t *)(prep->dev + L.off_coeffs);
     j.mc_tab = (const uint4 *)(prep->dev + L.off_mc_tab);
+    j.px = (const uint32_t *)(prep->dev + L.off_px);
+    j.n_px_small = L.n_px_small;
     j.intra = (const uint32_t *)(prep->dev + L.off_intra);
     for (int k = 0; k < 4; k++) {
         j.tx_tab[k] = (const uint32_t *)(prep->dev + L.off_tx_tab[k]);
