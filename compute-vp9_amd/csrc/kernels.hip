@@ -568,7 +568,9 @@ __device__ __forceinline__ void mc_wg(const DBatch &B, uint32_t wg, uint32_t nwg
     mc_unit(J, e, R, threadIdx.x % CVP9_MC_UNIT, B.resid_all ? J.resid : nullptr);
 }
 
-__global__ __launch_bounds__(256) void k_mc(DBatch B, int late)
+/* (held to 6 waves per SIMD: 70 registers instead of 88, no spills; the
+ * latency-bound units then have 7 waves per SIMD in flight instead of 5) */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_mc(DBatch B, int late)
 {
     mc_wg(B, blockIdx.x, gridDim.x, late != 0);
 }
@@ -1032,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_owner_res(DBatch B, uint32_t owner_wgs,
     else res_small_wg(B, blockIdx.x - owner_wgs, wg, lds);
 }
 
-__global__ __launch_bounds__(256) void k_nbr_mc(DBatch B, uint32_t nbr_wgs, uint32_t nbr_used)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_nbr_mc(DBatch B, uint32_t nbr_wgs, uint32_t nbr_used)
 {
     if (blockIdx.x < nbr_wgs) {
         if (blockIdx.x < nbr_used) nbr_wg(B, blockIdx.x, nbr_used);
