@@ -31,9 +31,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL = os.path.join(HERE, "schedule_model")
 
 
+def plain_env(**extra):
+    """The environment for the model's own programs (make, the driver): a
+    sanitizer runtime preloaded into the test process is not theirs."""
+    env = {k: v for k, v in os.environ.items() if k not in ("LD_PRELOAD", "ASAN_OPTIONS", "UBSAN_OPTIONS")}
+    env.update(extra)
+    return env
+
+
 @pytest.fixture(scope="module")
 def model():
-    subprocess.run(["make", "-s", "-C", MODEL, "all", "overlap"], check=True)
+    subprocess.run(["make", "-s", "-C", MODEL, "all", "overlap"], check=True, env=plain_env())
     import importlib.util
     spec = importlib.util.spec_from_file_location("schedule_check", os.path.join(MODEL, "check.py"))
     mod = importlib.util.module_from_spec(spec)
@@ -43,7 +51,7 @@ def model():
 
 def trace(tmp_path, binary, query, args):
     out = tmp_path / f"{binary}_{query}_{'_'.join(map(str, args))}.txt"
-    env = dict(os.environ, CVP9_MODEL_TRACE=str(out), CVP9_MODEL_QUERY=query)
+    env = plain_env(CVP9_MODEL_TRACE=str(out), CVP9_MODEL_QUERY=query)
     r = subprocess.run([os.path.join(MODEL, "_build", binary), *map(str, args)], env=env,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -101,6 +109,19 @@ def test_four_groups_serialised_when_busy(model, tmp_path):
     run two persistent intra launches at once."""
     rep = check(model, trace(tmp_path, "model_driver", "busy", (8, 4, 7, "alt", 1, 2, 0, -1)))
     assert rep.max_concurrent_share == pytest.approx(0.5)
+
+
+@pytest.mark.parametrize("sched", [(8, 4, 7, "alt", 1, 2, 0, -1), (1, 1, 8, "alt", 1, 2, 0, -1),
+                                   (8, 2, 6, "alt", 1, 2, 0, 3)], ids=lambda s: "-".join(map(str, s)))
+def test_engine_host_code_under_asan(model, tmp_path, sched):
+    """The same host code under AddressSanitizer and UndefinedBehaviorSanitizer
+    (leaks included), through multi-group, one-stream and fused schedules."""
+    subprocess.run(["make", "-s", "-C", MODEL, "asan"], check=True, env=plain_env())
+    env = plain_env(CVP9_MODEL_TRACE=str(tmp_path / "t.txt"), CVP9_MODEL_QUERY="random3",
+                    ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([os.path.join(MODEL, "_build", "model_driver_asan"), *map(str, sched)], env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
 
 
 def _mutate(src, dst, keep):
