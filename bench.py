@@ -299,10 +299,16 @@ def multi_stream(P, args):
         k_next += steps
         return time.perf_counter() - t0
 
+    # warm both schedules over a whole cycle of the frames: a stream's
+    # scratch and the engine's descriptor slots grow (a free and an
+    # allocation, each a device synchronisation) the first time a frame or a
+    # batch needs more, which inside the first timed run made it ~20% slow
     for k in range(args.mwarmup):
         batch(k, G)
+    for k in range(args.mwarmup, args.mwarmup + F):
+        batch(k, 1)
     eng.sync()
-    k_next = args.mwarmup
+    k_next = args.mwarmup + F
     # the schedules alternate, args.mreps runs each, so drift on the box
     # affects both alike; the figure is the median run of the faster one
     runs = {1: [], G: []}
